@@ -1,0 +1,124 @@
+/*
+ * CPU oracle for the batched AC power-flow path.  TEST INFRASTRUCTURE ONLY:
+ * tests, the smoke check and the CPU-baseline leg of the benchmark load this
+ * library as the checker.  Nothing in the product (mapol_amd/, include/)
+ * links, loads or calls it.
+ *
+ * Plain C, FP64 throughout.  Every routine restates one function of the
+ * reference's CPU path (MAPoL, exec types S/O):
+ *   V0            src/powerflow/runpf.h:211-226      (mkl_computeVoltage)
+ *   Yf/Yt/Ybus    src/powerflow/makeYbus.h:17-274    (mkl_computeCfCt/YfYt/makeYbus)
+ *   mismatch      src/powerflow/newtonpf.h:18-71     (mkl_checkConvergence)
+ *   Ibus          src/powerflow/newtonpf.h:73-95     (mkl_computeDiagIbus)
+ *   J values      src/powerflow/newtonpf.h:97-186    (mkl_compuateJacobianMatrix)
+ *   V update      src/powerflow/newtonpf.h:188-213   (mkl_updateVoltage)
+ *   J pattern     src/powerflow/newtonpf.h:215-310   (mkl_computeNnzJacobianMatrix)
+ *   NR loop       src/powerflow/newtonpf.h:365-523   (mkl_newtonpf)
+ *   loss          src/powerflow/runpf.h:228-257      (mkl_computeLoss)
+ *   B', B''       src/powerflow/makeB.h:311-501      (mkl_makeB)
+ *   FDPF          src/powerflow/fdpf.h:369-596       (mkl_fdpf)
+ * The linear solver replaces MKL DSS / Eigen SparseLU (newtonpf.h:312-363)
+ * with this file's own sparse LU: minimum-degree column order on A+A^T and a
+ * left-looking (Gilbert-Peierls) numeric factorization with threshold
+ * partial pivoting (threshold 1.0 = Eigen's default diagpivotthresh).
+ *
+ * Extensions the reference does not have (defined here once, mirrored by the
+ * GPU engine): per-instance branch/generator outage masks, generator outage
+ * => PV bus becomes PQ, and MATPOWER-style PV->PQ switching on reactive
+ * limits (all violating buses at once, then re-solve from the current V).
+ */
+#ifndef PF_ORACLE_H
+#define PF_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_PQ = 1, ORC_PV = 2, ORC_SLACK = 3 };
+enum { ORC_NR = 0, ORC_FDXB = 1, ORC_FDBX = 2 };
+enum {
+  ORC_CONVERGED = 0,
+  ORC_MAX_ITER = 1,
+  ORC_SINGULAR = 2,
+  ORC_NONFINITE = 3
+};
+
+typedef struct {
+  int nbus, nbranch, ngen;
+  const int *bus_type;
+  const double *Pd, *Qd, *Gs, *Bs, *Vm, *Va; /* pu, radians */
+  const int *br_from, *br_to;
+  const double *R, *X, *B, *tap, *shift; /* tap 0 => 1; shift in degrees */
+  const unsigned char *br_status;
+  const int *gen_bus;
+  const double *Pg, *Qg, *Qmax, *Qmin, *VG;
+  const unsigned char *gen_status;
+} orc_case;
+
+/* Per-instance modifications; every pointer may be NULL.  Override arrays
+ * use NaN for "no override" (AVR set-point per bus, OLTC tap per branch,
+ * SHC susceptance per bus: the reference's control vector, runpf.h:218,
+ * makeYbus.h:58,74). */
+typedef struct {
+  const unsigned char *br_on;  /* [nbranch] */
+  const unsigned char *gen_on; /* [ngen] */
+  const double *vset;          /* [nbus] */
+  const double *tap;           /* [nbranch] */
+  const double *bsh;           /* [nbus] */
+  const double *V0;            /* [2*nbus] warm start (re,im), NULL = V0 rule */
+} orc_inst;
+
+typedef struct {
+  int alg;           /* ORC_NR / ORC_FDXB / ORC_FDBX */
+  double tol;        /* 1e-8 (runpf.h:47) */
+  int maxit_nr;      /* 10   (runpf.h:45) */
+  int maxit_fd;      /* 30   (runpf.h:46) */
+  int enforce_qlim;  /* 0/1 */
+  int max_qlim_rounds;
+  double qlim_tol;   /* pu */
+  double pivot_thresh; /* 1.0 */
+} orc_opts;
+
+typedef struct {
+  double *V;        /* [2*nbus] out */
+  int *bus_type;    /* [nbus] out: final types (after outages / switching) */
+  int iterations;   /* total Newton (or FD full) iterations over all rounds */
+  int verdict;
+  int qlim_rounds;
+  int converged;
+  double norm;      /* final inf-norm of the mismatch */
+  double loss;      /* DBL_MAX when not converged (runpf.h:351-353) */
+} orc_result;
+
+int orc_solve(const orc_case *c, const orc_inst *inst, const orc_opts *o,
+              orc_result *r);
+
+/* Stage-level entry points used by the tests. */
+/* Ybus in CSR (0-based, sorted columns); returns nnz or -needed when cap is
+ * too small.  val is interleaved (re,im). */
+int orc_make_ybus(const orc_case *c, const orc_inst *inst, int *rowptr,
+                  int *colind, double *val, int cap);
+/* Mismatch F in the reference layout [P(pv); P(pq); Q(pq)] at voltage V for
+ * the case's own bus types (after applying inst outages).  Returns nJ. */
+int orc_mismatch(const orc_case *c, const orc_inst *inst, const double *V,
+                 double *F, double *norm);
+/* Jacobian in the reference layout (CSR); returns nnzJ or -needed. */
+int orc_jacobian(const orc_case *c, const orc_inst *inst, const double *V,
+                 int *rowptr, int *colind, double *val, int cap);
+/* Initial voltage (runpf.h:211-226). */
+void orc_initial_voltage(const orc_case *c, const orc_inst *inst, double *V);
+/* Branch complex flows Sf, St (pu, interleaved) and the objective
+ * total_loss + w * sum(max(0,|S|-rate)^2) over both ends. */
+double orc_objective(const orc_case *c, const orc_inst *inst, const double *V,
+                     const double *rate, double w, double *Sf, double *St);
+/* Solve A x = b with the oracle's sparse LU; A in CSR. Returns 0 / verdict. */
+int orc_sparse_solve(int n, const int *rowptr, const int *colind,
+                     const double *val, const double *b, double *x,
+                     double pivot_thresh);
+/* Column order used by the sparse LU (minimum degree on A+A^T). */
+void orc_min_degree(int n, const int *rowptr, const int *colind, int *perm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
