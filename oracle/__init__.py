@@ -16,7 +16,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
+REF_SO = os.path.join(_HERE, "_ref", "libpfref.so")
 
 PQ, PV, SLACK = 1, 2, 3
 NR, FDXB, FDBX = 0, 1, 2
@@ -65,12 +66,23 @@ def build(force: bool = False) -> str:
     return so
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        so = os.path.join(_HERE, "liborcpf.so")
-        if not os.path.exists(so):
-            build()
+def have_ref() -> bool:
+    """True when the Eigen-linked build (oracle/_ref) is present."""
+    return os.path.exists(REF_SO)
+
+
+def lib(which: str = "own"):
+    """``own``: the oracle with its own sparse LU; ``ref``: the build whose
+    Newton steps go through the reference's Eigen SparseLU."""
+    if which not in _LIBS:
+        if which == "ref":
+            so = REF_SO
+            if not os.path.exists(so):
+                raise FileNotFoundError(so)
+        else:
+            so = os.path.join(_HERE, "liborcpf.so")
+            if not os.path.exists(so):
+                build()
         L = C.CDLL(so)
         L.orc_solve.argtypes = [C.POINTER(OrcCase), C.POINTER(OrcInst),
                                 C.POINTER(OrcOpts), C.POINTER(OrcResult)]
@@ -88,8 +100,8 @@ def lib():
         L.orc_sparse_solve.argtypes = [C.c_int, _pi, _pi, _pd, _pd, _pd,
                                        C.c_double]
         L.orc_min_degree.argtypes = [C.c_int, _pi, _pi, _pi]
-        _LIB = L
-    return _LIB
+        _LIBS[which] = L
+    return _LIBS[which]
 
 
 def _p(a, t):
@@ -99,8 +111,9 @@ def _p(a, t):
 class Oracle:
     """Holds one case in oracle layout; solves instances one at a time."""
 
-    def __init__(self, case):
+    def __init__(self, case, which: str = "own"):
         self.case = case
+        self.which = which
         c = case
         self._keep = {}
 
@@ -157,7 +170,7 @@ class Oracle:
         bt = np.zeros(n, dtype=np.int32)
         r = OrcResult(_p(V, _pd), _p(bt, _pi), 0, 0, 0, 0, 0.0, 0.0)
         i, keep = self._inst(**inst)
-        lib().orc_solve(C.byref(self.s), C.byref(i), C.byref(o), C.byref(r))
+        lib(self.which).orc_solve(C.byref(self.s), C.byref(i), C.byref(o), C.byref(r))
         return dict(V=V[0::2] + 1j * V[1::2], bus_type=bt,
                     iterations=r.iterations, verdict=r.verdict,
                     qlim_rounds=r.qlim_rounds, converged=bool(r.converged),
@@ -170,7 +183,7 @@ class Oracle:
         ci = np.zeros(cap, np.int32)
         v = np.zeros(2 * cap)
         i, keep = self._inst(**inst)
-        nnz = lib().orc_make_ybus(C.byref(self.s), C.byref(i), _p(rp, _pi),
+        nnz = lib(self.which).orc_make_ybus(C.byref(self.s), C.byref(i), _p(rp, _pi),
                                   _p(ci, _pi), _p(v, _pd), cap)
         assert nnz >= 0
         import scipy.sparse as sp
@@ -181,7 +194,7 @@ class Oracle:
         n = self.case.nbus
         V = np.zeros(2 * n)
         i, keep = self._inst(**inst)
-        lib().orc_initial_voltage(C.byref(self.s), C.byref(i), _p(V, _pd))
+        lib(self.which).orc_initial_voltage(C.byref(self.s), C.byref(i), _p(V, _pd))
         return V[0::2] + 1j * V[1::2]
 
     def mismatch(self, V, **inst):
@@ -190,7 +203,7 @@ class Oracle:
         F = np.zeros(2 * n)
         nrm = np.zeros(1)
         i, keep = self._inst(**inst)
-        nJ = lib().orc_mismatch(C.byref(self.s), C.byref(i), _p(Vd, _pd),
+        nJ = lib(self.which).orc_mismatch(C.byref(self.s), C.byref(i), _p(Vd, _pd),
                                 _p(F, _pd), _p(nrm, _pd))
         return F[:nJ], float(nrm[0])
 
@@ -203,7 +216,7 @@ class Oracle:
         ci = np.zeros(cap, np.int32)
         v = np.zeros(cap)
         i, keep = self._inst(**inst)
-        nnz = lib().orc_jacobian(C.byref(self.s), C.byref(i), _p(Vd, _pd),
+        nnz = lib(self.which).orc_jacobian(C.byref(self.s), C.byref(i), _p(Vd, _pd),
                                  _p(rp, _pi), _p(ci, _pi), _p(v, _pd), cap)
         assert nnz >= 0
         F, _ = self.mismatch(V, **inst)
@@ -218,7 +231,7 @@ class Oracle:
         St = np.zeros(2 * m)
         rt = None if rate is None else np.ascontiguousarray(rate, np.float64)
         i, keep = self._inst(**inst)
-        f = lib().orc_objective(C.byref(self.s), C.byref(i), _p(Vd, _pd),
+        f = lib(self.which).orc_objective(C.byref(self.s), C.byref(i), _p(Vd, _pd),
                                 _p(rt, _pd), w, _p(Sf, _pd), _p(St, _pd))
         return f, Sf[0::2] + 1j * Sf[1::2], St[0::2] + 1j * St[1::2]
 

@@ -688,6 +688,13 @@ int orc_sparse_solve(int n, const int *rowptr, const int *colind,
   return st;
 }
 
+#ifdef ORC_LINSOLVE_EIGEN
+/* oracle/_ref build: Newton steps solved by the reference's Eigen SparseLU
+ * option (newtonpf.h:341-363), implemented in ref_eigen.cpp */
+int orc_ref_linsolve(int n, const int *rowptr, const int *colind,
+                     const double *val, const double *b, double *x);
+#endif
+
 /* ------------------------------------------------------------------------- */
 /* voltage update (newtonpf.h:188-213)                                         */
 /* ------------------------------------------------------------------------- */
@@ -734,12 +741,17 @@ static int newtonpf(const ybus_t *Y, const inst_t *I, const int *type, cpx *V,
       orc_min_degree(nJ, J.rowptr, J.colind, q);
     }
     jac_values(Y, &O, V, Ibus, &J);
+#ifdef ORC_LINSOLVE_EIGEN
+    int st = orc_ref_linsolve(nJ, J.rowptr, J.colind, J.val, F, dx);
+    if (st) { verdict = st; break; }
+#else
     lu_t LU;
     memset(&LU, 0, sizeof(LU));
     int st = csr_factor(nJ, J.rowptr, J.colind, J.val, q, o->pivot_thresh, &LU);
     if (st) { lu_free(&LU); verdict = st; break; }
     lu_solve(&LU, F, dx);
     lu_free(&LU);
+#endif
     update_voltage(&O, V, dx);
     err = mismatch_nr(Y, I, &O, V, F);
     if (err < o->tol) verdict = ORC_CONVERGED;
