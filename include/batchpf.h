@@ -1,0 +1,206 @@
+/*
+ * batchpf.h -- C-ABI of the MI355X batched AC power-flow engine.
+ *
+ * The reference (MAPoL) exposes its power-flow path as C++ free functions over
+ * module globals; every entry point below replaces one of them.  Plain C
+ * types only: pointers, sizes, status codes.  All arrays are host memory owned
+ * by the caller and are copied in or out during the call; device memory and
+ * the symbolic analysis belong to the engine handle.
+ *
+ * Numerical failure is never a process exit (the reference exit()s on CUDA /
+ * MKL errors, src/util/helper_cuda.h:967-977, newtonpf.h:321): it is a
+ * per-instance verdict.  API misuse and HIP errors are status codes, with a
+ * message from bpf_last_error().
+ */
+#ifndef BATCHPF_H
+#define BATCHPF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bpf_engine bpf_engine;
+
+typedef enum {
+  BPF_OK = 0,
+  BPF_ERR_INVALID_ARG = 1,
+  BPF_ERR_HIP = 2,
+  BPF_ERR_OOM = 3,
+  BPF_ERR_STATE = 4,
+  BPF_ERR_UNSUPPORTED = 5,
+  BPF_ERR_IO = 6
+} bpf_status;
+
+/* bus types: MATPOWER codes as in src/powersystem/Bus.h:10-12 */
+enum { BPF_PQ = 1, BPF_PV = 2, BPF_SLACK = 3 };
+
+/* algorithms: src/powerflow/runpf.h:16-18 */
+enum { BPF_NR = 0, BPF_FDXB = 1, BPF_FDBX = 2 };
+
+/* precision of the Jacobian factorization and triangular solves; mismatch,
+ * norm and voltage update are always FP64 */
+enum { BPF_FP64 = 0, BPF_MIXED = 1 };
+
+/* per-instance verdicts (the reference reports only converged or not,
+ * runpf.h:337-353) */
+enum {
+  BPF_CONVERGED = 0,
+  BPF_MAX_ITER = 1,
+  BPF_SINGULAR = 2,
+  BPF_NONFINITE = 3
+};
+
+/* control kinds of the reference's particle vector
+ * (src/pso/Particula.h:26-60): AVR set-point, OLTC tap, switched shunt */
+enum { BPF_CTL_AVR = 1, BPF_CTL_OLTC = 2, BPF_CTL_SHC = 3 };
+
+/* Case data, per unit on baseMVA, angles in radians: the reference's Topology
+ * of Bus / Branch / Generator (src/powersystem/Topology.h:18-127,
+ * Bus.h:6-63, Branch.h:10-62, Generator.h:9-40) as structure-of-arrays.
+ * Loads and generation are separate; the net injection the reference stores
+ * in Bus.P/Bus.Q is sum(Pg of in-service generators) - Pd
+ * (src/files/loadMatpowerFormat.h:88,126-128). */
+typedef struct {
+  int32_t nbus, nbranch, ngen;
+  double baseMVA;
+  const int32_t *bus_type;                 /* [nbus] */
+  const double *Pd, *Qd, *Gs, *Bs;         /* [nbus] */
+  const double *Vm, *Va;                   /* [nbus] initial voltage table */
+  const int32_t *br_from, *br_to;          /* [nbranch], 0-based */
+  const double *R, *X, *B, *tap, *shift;   /* tap 0 => 1; shift in degrees */
+  const uint8_t *br_status;                /* [nbranch] */
+  const int32_t *gen_bus;                  /* [ngen] */
+  const double *Pg, *Qg, *Qmax, *Qmin, *VG;/* [ngen] */
+  const uint8_t *gen_status;               /* [ngen] */
+} bpf_case;
+
+typedef struct {
+  int32_t alg;             /* BPF_NR / BPF_FDXB / BPF_FDBX */
+  int32_t precision;       /* BPF_FP64 / BPF_MIXED */
+  int32_t refine_steps;    /* FP64 refinement steps of the Newton step (MIXED) */
+  int32_t maxit_nr;        /* 10, runpf.h:45 */
+  int32_t maxit_fd;        /* 30, runpf.h:46 */
+  int32_t enforce_qlim;    /* PV->PQ switching on reactive limits */
+  int32_t max_qlim_rounds;
+  int32_t use_graph;       /* replay each Newton iteration as one hipGraph */
+  double tol;              /* 1e-8, runpf.h:47 */
+  double qlim_tol;         /* pu */
+} bpf_options;
+
+/* one control of the reference's particle vector (Particula::Estrutura) */
+typedef struct {
+  int32_t kind;            /* BPF_CTL_* */
+  int32_t id;              /* bus (AVR, SHC) or branch (OLTC) index */
+} bpf_ctl;
+
+typedef struct {
+  int32_t nbus, nbranch, nblocks;  /* nblocks = non-slack buses */
+  int32_t nnz_ybus;                /* Ybus pattern entries */
+  int32_t nnz_jac_blocks;          /* 2x2 Jacobian blocks (superset pattern) */
+  int32_t nnz_lu_blocks;           /* 2x2 blocks of L+U (with fill) */
+  int32_t lu_levels;               /* elimination-tree levels */
+  int32_t fw_levels, bw_levels;    /* triangular-solve levels */
+  int64_t lu_pairs;                /* 2x2 block products per factorization */
+  int32_t instances;               /* T set by bpf_set_instances */
+  int32_t lds_solve;               /* 1: solve vector held in LDS */
+} bpf_info;
+
+/* Fill an options struct with the reference's defaults (runpf.h:45-47). */
+void bpf_default_options(bpf_options *o);
+
+/* hybrid_init (src/powerflow/runpf.h:795-982): copies the case, runs the
+ * symbolic analysis once (Ybus pattern, ordering, L+U pattern, schedules) and
+ * uploads the shared structures.  Replaces also mkl_init (runpf.h:359). */
+bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
+                      bpf_engine **out);
+
+/* Per-instance inputs: the reference's particle matrix d_enxame [T][nCtl]
+ * (runpf.h:611-616) generalised with branch and generator outage masks.
+ * br_on [T*nbranch] and gen_on [T*ngen] may be NULL (all in service);
+ * ctl/ctl_val may be NULL (defaults: VG, case tap, case Bs as main.cu:232-264
+ * builds them).  Builds each instance's bus types (a PV bus without an
+ * in-service generator becomes PQ), injections and set-points on the host
+ * and uploads them; inputs stay resident in HBM. */
+bpf_status bpf_set_instances(bpf_engine *e, int32_t T, const uint8_t *br_on,
+                             const uint8_t *gen_on, int32_t nctl,
+                             const bpf_ctl *ctl, const double *ctl_val);
+
+/* hybrid_runpf (runpf.h:606-793) minus the host round trips: initial
+ * voltage, Ybus assembly with outage masks, Newton-Raphson (or fast
+ * decoupled) for all instances, reactive-limit rounds, losses.  Results stay
+ * on the device until bpf_get_results.  Replaces mkl_runpf (runpf.h:259). */
+bpf_status bpf_solve(bpf_engine *e);
+
+/* Per instance: V (re,im) [T*nbus*2], total iterations, verdict, final
+ * inf-norm, loss (DBL_MAX when not converged, runpf.h:351-353), reactive-
+ * limit rounds, final bus types [T*nbus].  Any pointer may be NULL. */
+bpf_status bpf_get_results(bpf_engine *e, double *V, int32_t *iterations,
+                           int32_t *verdict, double *norm, double *loss,
+                           int32_t *qlim_rounds, int32_t *bus_type);
+
+/* Branch power flows Sf, St (re,im) [T*nbranch*2] at the last solution:
+ * the from/to ends of the loss kernel's Yf*V, Yt*V products (runpf.h:543-572). */
+bpf_status bpf_get_branch_flows(bpf_engine *e, double *Sf, double *St);
+
+/* hybrid_free (runpf.h:984-1078), mkl_clean (runpf.h:452). */
+void bpf_destroy(bpf_engine *e);
+
+const char *bpf_last_error(const bpf_engine *e);
+bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info);
+
+/* ---- stage-level entry points (tests, benchmarks) ---------------------- */
+
+/* makeYbus (src/powerflow/makeYbus.h:372-510; CPU :113-274): Ybus of
+ * instance t after bpf_solve, CSR with sorted columns; val interleaved. */
+bpf_status bpf_get_ybus(bpf_engine *e, int32_t t, int32_t *rowptr,
+                        int32_t *colind, double *val);
+
+/* checkConvergence / computeDiagIbus (newtonpf.h:18-95): mismatch of every
+ * instance at the given voltages V [T*nbus*2] for the instances' current bus
+ * types.  F is returned in the reference layout [P(pv); P(pq); Q(pq)]
+ * per instance, packed at stride 2*nbus; norm [T]. */
+bpf_status bpf_stage_mismatch(bpf_engine *e, const double *V, double *F,
+                              double *norm);
+
+/* compuateJacobianMatrix (newtonpf.h:97-186): Jacobian of instance t at V
+ * (all instances' voltages [T*nbus*2] are uploaded), written as COO triplets
+ * in the reference layout (row/col = newtonpf.h:256-291 indices); returns the
+ * entry count in *nnz (cap = capacity of the arrays). */
+bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V, int32_t t,
+                              int32_t cap, int32_t *row, int32_t *col,
+                              double *val, int32_t *nnz);
+
+/* Linear solve of the Newton step (newtonpf.h:312-363 / 1086-1198): factor
+ * every instance's Jacobian at V on the device (precision of the options)
+ * and solve J dx = F; dx in the reference layout, stride 2*nbus. */
+bpf_status bpf_stage_newton_step(bpf_engine *e, const double *V, double *dx);
+
+/* Adjoint sensitivities at the converged point of the last bpf_solve.
+ * Objective per instance: total active losses (runpf.h:228-257) plus
+ * w * sum over branch ends of max(0, |S| - rate)^2.  Solves J^T lambda =
+ * df/dx with FP64 factors and returns df/dP, df/dQ of bus injections
+ * [T*nbus] and df/dx of branch reactances [T*nbranch].  rate may be NULL
+ * (no penalty).  No counterpart in the reference. */
+bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
+                       double *grad_P, double *grad_Q, double *grad_x,
+                       double *objective);
+
+/* Kernel timing on the engine's stream with hipEvents (enable before
+ * bpf_solve; graph replay is turned off while enabled).  Names: ybus,
+ * mismatch, jacobian, lu_factor, lu_solve, update, loss, fd_solve,
+ * adjoint. */
+bpf_status bpf_set_timing(bpf_engine *e, int32_t enable);
+bpf_status bpf_get_kernel_time(const bpf_engine *e, const char *name,
+                               int32_t *launches, double *total_ms);
+
+/* Launch shapes (threads per workgroup) of the per-instance kernels; the
+ * autotuner in mapol_amd/tune.py sets them.  0 keeps the current value. */
+bpf_status bpf_set_launch(bpf_engine *e, int32_t lu_threads,
+                          int32_t solve_threads, int32_t mismatch_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,761 @@
+// C-ABI of the batched power-flow engine (include/batchpf.h): handle, per-
+// instance setup on the host, device buffers and the Newton driver.
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/batchpf.h"
+#include "kernels.hip"
+#include "symbolic.hpp"
+
+using namespace bpf;
+
+namespace {
+
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct TimedEvent {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct bpf_engine {
+  bpf_options opt;
+  std::string err;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  // case (host copies)
+  int n = 0, m = 0, ng = 0;
+  std::vector<int> bus_type, br_from, br_to, gen_bus;
+  std::vector<double> Pd, Qd, Gs, Bs, Vm, Va, R, X, B, tap, shift, Pg, Qg,
+      Qmax, Qmin, VG;
+  std::vector<uint8_t> br_status, gen_status;
+  Symbolic S;
+  // shared device arrays
+  std::vector<DBuf> owned;
+  DevTopo g{};
+  const int *d_lvA = nullptr, *d_tA_dst = nullptr, *d_tA_pp = nullptr,
+            *d_pairL = nullptr, *d_pairU = nullptr, *d_lvB = nullptr,
+            *d_tB_dst = nullptr, *d_tB_diag = nullptr, *d_fw_ptr = nullptr,
+            *d_fw_rows = nullptr, *d_bw_ptr = nullptr, *d_bw_rows = nullptr;
+  // per-instance device arrays
+  std::vector<DBuf> inst_owned;
+  int T = 0;
+  DevInst d{};
+  const int8_t *d_type0 = nullptr;
+  const double *d_Qinj0 = nullptr;
+  void *d_LU = nullptr;         // B4<float> or B4<double> [T][nnzLU]
+  int lu_fp64 = -1;             // precision the LU buffer was sized for
+  void *d_xglob = nullptr;      // solve vectors when LDS is too small
+  cd *d_Sf = nullptr, *d_St = nullptr;
+  double *d_loss = nullptr;
+  int *h_count = nullptr;       // pinned
+  // launch shapes
+  int lu_threads = 1024, solve_threads = 1024, mis_threads = 512;
+  int max_lds = 65536;
+  // graph of one Newton iteration
+  hipGraphExec_t graph = nullptr;
+  int graph_key = -1;
+  // timing
+  int timing = 0;
+  std::vector<TimedEvent> events;
+  std::map<std::string, std::pair<int, double>> ktime;
+  bool solved = false;
+};
+
+#define HIPCHK(expr)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      e->err = std::string(#expr) + ": " + hipGetErrorString(e_);           \
+      return BPF_ERR_HIP;                                                   \
+    }                                                                       \
+  } while (0)
+
+static bpf_status dalloc(bpf_engine *e, std::vector<DBuf> &pool, size_t bytes,
+                         void **out) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t r = hipMalloc(&p, bytes);
+  if (r != hipSuccess) {
+    e->err = "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(r);
+    return r == hipErrorOutOfMemory ? BPF_ERR_OOM : BPF_ERR_HIP;
+  }
+  pool.push_back({p, bytes});
+  *out = p;
+  return BPF_OK;
+}
+
+template <typename V>
+static bpf_status upload(bpf_engine *e, std::vector<DBuf> &pool, const V &v,
+                         const typename V::value_type **out) {
+  void *p;
+  size_t bytes = v.size() * sizeof(typename V::value_type);
+  bpf_status s = dalloc(e, pool, bytes, &p);
+  if (s) return s;
+  if (bytes) {
+    hipError_t r = hipMemcpy(p, v.data(), bytes, hipMemcpyHostToDevice);
+    if (r != hipSuccess) {
+      e->err = std::string("hipMemcpy: ") + hipGetErrorString(r);
+      return BPF_ERR_HIP;
+    }
+  }
+  *out = (const typename V::value_type *)p;
+  return BPF_OK;
+}
+
+static void free_pool(std::vector<DBuf> &pool) {
+  for (auto &b : pool) hipFree(b.p);
+  pool.clear();
+}
+
+extern "C" void bpf_default_options(bpf_options *o) {
+  memset(o, 0, sizeof(*o));
+  o->alg = BPF_NR;
+  o->precision = BPF_FP64;
+  o->refine_steps = 1;
+  o->maxit_nr = 10;
+  o->maxit_fd = 30;
+  o->enforce_qlim = 0;
+  o->max_qlim_rounds = 10;
+  o->use_graph = 1;
+  o->tol = 1e-8;
+  o->qlim_tol = 1e-6;
+}
+
+extern "C" const char *bpf_last_error(const bpf_engine *e) {
+  return e ? e->err.c_str() : "null engine";
+}
+
+extern "C" void bpf_destroy(bpf_engine *e) {
+  if (!e) return;
+  if (e->graph) hipGraphExecDestroy(e->graph);
+  for (auto &ev : e->events) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
+  free_pool(e->inst_owned);
+  free_pool(e->owned);
+  if (e->h_count) hipHostFree(e->h_count);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+}
+
+template <typename T>
+static std::vector<T> vcopy(const T *p, int n) {
+  return p ? std::vector<T>(p, p + n) : std::vector<T>();
+}
+
+extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
+                                 bpf_engine **out) {
+  if (!out) return BPF_ERR_INVALID_ARG;
+  *out = nullptr;
+  bpf_engine *e = new bpf_engine();
+  *out = e;
+  if (!c || c->nbus <= 0 || c->nbranch < 0 || c->ngen < 0) {
+    e->err = "invalid case sizes";
+    return BPF_ERR_INVALID_ARG;
+  }
+  const void *need[] = {c->bus_type, c->Pd, c->Qd, c->Gs, c->Bs, c->Vm, c->Va};
+  for (auto p : need)
+    if (!p) { e->err = "null bus array"; return BPF_ERR_INVALID_ARG; }
+  if (c->nbranch && (!c->br_from || !c->br_to || !c->R || !c->X || !c->B ||
+                     !c->tap || !c->shift || !c->br_status)) {
+    e->err = "null branch array";
+    return BPF_ERR_INVALID_ARG;
+  }
+  if (c->ngen && (!c->gen_bus || !c->Pg || !c->Qg || !c->Qmax || !c->Qmin ||
+                  !c->VG || !c->gen_status)) {
+    e->err = "null generator array";
+    return BPF_ERR_INVALID_ARG;
+  }
+  if (o) e->opt = *o; else bpf_default_options(&e->opt);
+  if (e->opt.alg < 0 || e->opt.alg > 2 || e->opt.precision < 0 ||
+      e->opt.precision > 1 || e->opt.refine_steps < 0 || e->opt.maxit_nr < 0 ||
+      e->opt.maxit_fd < 0 || !(e->opt.tol > 0)) {
+    e->err = "invalid options";
+    return BPF_ERR_INVALID_ARG;
+  }
+  e->n = c->nbus; e->m = c->nbranch; e->ng = c->ngen;
+  e->bus_type = vcopy(c->bus_type, e->n);
+  e->Pd = vcopy(c->Pd, e->n); e->Qd = vcopy(c->Qd, e->n);
+  e->Gs = vcopy(c->Gs, e->n); e->Bs = vcopy(c->Bs, e->n);
+  e->Vm = vcopy(c->Vm, e->n); e->Va = vcopy(c->Va, e->n);
+  e->br_from = vcopy(c->br_from, e->m); e->br_to = vcopy(c->br_to, e->m);
+  e->R = vcopy(c->R, e->m); e->X = vcopy(c->X, e->m); e->B = vcopy(c->B, e->m);
+  e->tap = vcopy(c->tap, e->m); e->shift = vcopy(c->shift, e->m);
+  e->br_status = vcopy(c->br_status, e->m);
+  e->gen_bus = vcopy(c->gen_bus, e->ng);
+  e->Pg = vcopy(c->Pg, e->ng); e->Qg = vcopy(c->Qg, e->ng);
+  e->Qmax = vcopy(c->Qmax, e->ng); e->Qmin = vcopy(c->Qmin, e->ng);
+  e->VG = vcopy(c->VG, e->ng); e->gen_status = vcopy(c->gen_status, e->ng);
+  for (int k = 0; k < e->ng; k++)
+    if (e->gen_bus[k] < 0 || e->gen_bus[k] >= e->n) {
+      e->err = "generator bus out of range";
+      return BPF_ERR_INVALID_ARG;
+    }
+  if (!analyse(e->n, e->m, e->br_from.data(), e->br_to.data(),
+               e->bus_type.data(), e->S, &e->err))
+    return BPF_ERR_INVALID_ARG;
+  // device
+  HIPCHK(hipGetDevice(&e->dev));
+  HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  HIPCHK(hipDeviceGetAttribute(&e->max_lds,
+                               hipDeviceAttributeMaxSharedMemoryPerBlock, e->dev));
+  HIPCHK(hipHostMalloc((void **)&e->h_count, sizeof(int), hipHostMallocDefault));
+  const Symbolic &S = e->S;
+  DevTopo &g = e->g;
+  g.n = e->n; g.m = e->m; g.nb = S.nb;
+  g.nnzY = (int)S.ycol.size();
+  g.nnzLU = (int)S.lcol.size();
+  bpf_status s;
+#define UP(dst, vec) \
+  if ((s = upload(e, e->owned, vec, &dst))) return s;
+  UP(g.yrow, S.yrow); UP(g.ycol, S.ycol); UP(g.yrowof, S.yrowof);
+  UP(g.ydiag_bus, S.ydiag_bus); UP(g.yf_ptr, S.yf_ptr);
+  UP(g.yf_item, S.yf_item); UP(g.yt_ptr, S.yt_ptr); UP(g.yt_item, S.yt_item);
+  UP(g.blk_of_bus, S.blk_of_bus); UP(g.bus_of_blk, S.bus_of_blk);
+  UP(g.y2lu, S.y2lu); UP(g.lrow, S.lrow); UP(g.lcol, S.lcol);
+  UP(g.ldiag, S.ldiag); UP(g.mirror, S.mirror);
+  UP(g.br_from, e->br_from); UP(g.br_to, e->br_to);
+  UP(g.R, e->R); UP(g.X, e->X); UP(g.B, e->B); UP(g.shift, e->shift);
+  UP(g.Gs, e->Gs); UP(g.Qd, e->Qd); UP(g.Va0, e->Va);
+  UP(g.base_type, e->bus_type);
+  UP(e->d_lvA, S.lvA); UP(e->d_tA_dst, S.tA_dst); UP(e->d_tA_pp, S.tA_pp);
+  UP(e->d_pairL, S.pairL); UP(e->d_pairU, S.pairU); UP(e->d_lvB, S.lvB);
+  UP(e->d_tB_dst, S.tB_dst); UP(e->d_tB_diag, S.tB_diag);
+  UP(e->d_fw_ptr, S.fw_ptr); UP(e->d_fw_rows, S.fw_rows);
+  UP(e->d_bw_ptr, S.bw_ptr); UP(e->d_bw_rows, S.bw_rows);
+#undef UP
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_set_launch(bpf_engine *e, int32_t lu, int32_t sv,
+                                     int32_t mis) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  auto ok = [](int v) { return v == 0 || (v >= 64 && v <= 1024 && v % 64 == 0); };
+  if (!ok(lu) || !ok(sv) || !ok(mis)) {
+    e->err = "launch sizes must be multiples of 64 in [64, 1024]";
+    return BPF_ERR_INVALID_ARG;
+  }
+  if (lu) e->lu_threads = lu;
+  if (sv) e->solve_threads = sv;
+  if (mis) e->mis_threads = mis;
+  if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_set_instances(bpf_engine *e, int32_t T,
+                                        const uint8_t *br_on,
+                                        const uint8_t *gen_on, int32_t nctl,
+                                        const bpf_ctl *ctl,
+                                        const double *ctl_val) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  if (T <= 0 || T > 65535 || nctl < 0 || (nctl > 0 && (!ctl || !ctl_val))) {
+    e->err = "invalid instance count or controls";
+    return BPF_ERR_INVALID_ARG;
+  }
+  const int n = e->n, m = e->m, ng = e->ng;
+  for (int c = 0; c < nctl; c++) {
+    int k = ctl[c].kind, id = ctl[c].id;
+    bool okk = (k == BPF_CTL_AVR || k == BPF_CTL_SHC) ? (id >= 0 && id < n)
+               : (k == BPF_CTL_OLTC) ? (id >= 0 && id < m) : false;
+    if (!okk) { e->err = "control " + std::to_string(c) + " out of range"; return BPF_ERR_INVALID_ARG; }
+  }
+  if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
+  free_pool(e->inst_owned);
+  e->T = T;
+  e->solved = false;
+  e->lu_fp64 = -1;
+  const size_t Tn = (size_t)T * n, Tm = (size_t)T * m;
+  std::vector<uint8_t> on(Tm);
+  std::vector<double> tapT(Tm), BsT(Tn), vset(Tn), P(Tn), Q(Tn), Qmx(Tn), Qmn(Tn);
+  std::vector<int8_t> ty(Tn);
+  std::vector<int> ngon(n);
+  std::vector<double> vg(n);
+  for (int t = 0; t < T; t++) {
+    const size_t on_ = (size_t)t * n, om = (size_t)t * m;
+    std::fill(ngon.begin(), ngon.end(), 0);
+    for (int i = 0; i < n; i++) {
+      ty[on_ + i] = (int8_t)e->bus_type[i];
+      P[on_ + i] = -e->Pd[i];
+      Q[on_ + i] = -e->Qd[i];
+      Qmx[on_ + i] = 0.0;
+      Qmn[on_ + i] = 0.0;
+      BsT[on_ + i] = e->Bs[i];
+      vg[i] = NAN;
+    }
+    for (int k = 0; k < ng; k++) {
+      int b = e->gen_bus[k];
+      bool g_on = e->gen_status[k] && (!gen_on || gen_on[(size_t)t * ng + k]);
+      if (!g_on) continue;
+      ngon[b]++;
+      P[on_ + b] += e->Pg[k];
+      Q[on_ + b] += e->Qg[k];
+      Qmx[on_ + b] += e->Qmax[k];
+      Qmn[on_ + b] += e->Qmin[k];
+      vg[b] = e->VG[k];   // the last in-service generator row sets VG
+    }
+    for (int i = 0; i < n; i++) {
+      if (e->bus_type[i] == BPF_PV && ngon[i] == 0) ty[on_ + i] = BPF_PQ;
+      double v = e->Vm[i];
+      if (e->bus_type[i] == BPF_PV && !std::isnan(vg[i])) v = vg[i];
+      vset[on_ + i] = v;
+    }
+    for (int b = 0; b < m; b++) {
+      on[om + b] = e->br_status[b] && (!br_on || br_on[om + b]);
+      tapT[om + b] = e->tap[b];
+    }
+    for (int c = 0; c < nctl; c++) {
+      double v = ctl_val[(size_t)t * nctl + c];
+      int id = ctl[c].id;
+      if (std::isnan(v)) continue;
+      if (ctl[c].kind == BPF_CTL_AVR) vset[on_ + id] = v;
+      else if (ctl[c].kind == BPF_CTL_SHC) BsT[on_ + id] = v;
+      else if (e->tap[id] != 0.0) tapT[om + id] = v;
+    }
+  }
+  DevInst &d = e->d;
+  d.T = T;
+  bpf_status s;
+  const uint8_t *pu8;
+  const double *pd;
+  const int8_t *pi8;
+#define UPI(dst, vec, tmp) \
+  if ((s = upload(e, e->inst_owned, vec, &tmp))) return s; dst = tmp;
+  UPI(d.br_on, on, pu8);
+  UPI(d.tap, tapT, pd);
+  UPI(d.Bs, BsT, pd);
+  UPI(d.vset, vset, pd);
+  UPI(d.Qmax, Qmx, pd);
+  UPI(d.Qmin, Qmn, pd);
+  UPI(e->d_type0, ty, pi8);   // read-only after upload
+  UPI(e->d_Qinj0, Q, pd);
+  if ((s = upload(e, e->inst_owned, P, &pd))) return s;
+  d.Pinj = const_cast<double *>(pd);
+#undef UPI
+  void *p;
+  const int nb = e->S.nb, nnzY = e->g.nnzY;
+#define AL(dst, type, count) \
+  if ((s = dalloc(e, e->inst_owned, sizeof(type) * (count), &p))) return s; dst = (type *)p;
+  AL(d.type, int8_t, Tn);
+  AL(d.Qinj, double, Tn);
+  AL(d.Y, cd, (size_t)T * nnzY);
+  AL(d.V, cd, Tn);
+  AL(d.Ib, cd, Tn);
+  AL(d.F, double, (size_t)T * 2 * nb);
+  AL(d.dx, double, (size_t)T * 2 * nb);
+  AL(d.st, InstState, T);
+  AL(d.active_count, int, 1);
+  AL(e->d_Sf, cd, Tm);
+  AL(e->d_St, cd, Tm);
+  AL(e->d_loss, double, T);
+#undef AL
+  return BPF_OK;
+}
+
+// (re)allocate the factor storage for the requested precision
+static bpf_status ensure_lu(bpf_engine *e, int fp64) {
+  if (e->lu_fp64 == fp64 && e->d_LU) return BPF_OK;
+  if (e->d_LU) {
+    for (auto it = e->inst_owned.begin(); it != e->inst_owned.end(); ++it)
+      if (it->p == e->d_LU) { hipFree(it->p); e->inst_owned.erase(it); break; }
+    for (auto it = e->inst_owned.begin(); it != e->inst_owned.end(); ++it)
+      if (it->p == e->d_xglob) { hipFree(it->p); e->inst_owned.erase(it); break; }
+    e->d_LU = e->d_xglob = nullptr;
+  }
+  size_t el = fp64 ? sizeof(B4<double>) : sizeof(B4<float>);
+  bpf_status s = dalloc(e, e->inst_owned, el * (size_t)e->T * e->g.nnzLU, &e->d_LU);
+  if (s) return s;
+  size_t xs = (fp64 ? 8 : 4) * (size_t)2 * e->S.nb;
+  if ((int64_t)xs > e->max_lds) {
+    s = dalloc(e, e->inst_owned, xs * e->T, &e->d_xglob);
+    if (s) return s;
+  }
+  e->lu_fp64 = fp64;
+  if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
+  return BPF_OK;
+}
+
+// ---- launches ---------------------------------------------------------------
+static void tbegin(bpf_engine *e, const char *name) {
+  if (!e->timing) return;
+  TimedEvent t;
+  t.name = name;
+  hipEventCreate(&t.a);
+  hipEventCreate(&t.b);
+  hipEventRecord(t.a, e->stream);
+  e->events.push_back(t);
+}
+static void tend(bpf_engine *e) {
+  if (!e->timing) return;
+  hipEventRecord(e->events.back().b, e->stream);
+}
+static void tcollect(bpf_engine *e) {
+  for (auto &ev : e->events) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ev.a, ev.b);
+    auto &slot = e->ktime[ev.name];
+    slot.first++;
+    slot.second += ms;
+    hipEventDestroy(ev.a);
+    hipEventDestroy(ev.b);
+  }
+  e->events.clear();
+}
+
+static unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
+
+static void launch_mismatch(bpf_engine *e, int mode) {
+  tbegin(e, "mismatch");
+  hipLaunchKernelGGL(k_mismatch, dim3(e->T), dim3(e->mis_threads), 0,
+                     e->stream, e->g, e->d, mode, e->opt.maxit_nr, e->opt.tol,
+                     e->opt.enforce_qlim, e->opt.max_qlim_rounds,
+                     e->opt.qlim_tol, 0);
+  tend(e);
+}
+
+template <typename T>
+static void launch_newton_step(bpf_engine *e) {
+  B4<T> *LU = (B4<T> *)e->d_LU;
+  tbegin(e, "jacobian");
+  hipLaunchKernelGGL(k_jacobian<T>, dim3(div_up(e->g.nnzY, 256), e->T),
+                     dim3(256), 0, e->stream, e->g, e->d, LU);
+  tend(e);
+  tbegin(e, "lu_factor");
+  hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,
+                     e->stream, e->g, e->d.st, e->S.nlev, e->d_lvA,
+                     e->d_tA_dst, e->d_tA_pp, e->d_pairL, e->d_pairU,
+                     e->d_lvB, e->d_tB_dst, e->d_tB_diag, LU);
+  tend(e);
+  size_t xs = sizeof(T) * 2 * (size_t)e->S.nb;
+  int use_lds = (int64_t)xs <= e->max_lds;
+  int refine = (sizeof(T) == 4) ? e->opt.refine_steps : 0;
+  tbegin(e, "lu_solve");
+  hipLaunchKernelGGL(k_lu_solve<T>, dim3(e->T), dim3(e->solve_threads),
+                     use_lds ? xs : 0, e->stream, e->g, e->d,
+                     (const B4<T> *)LU, refine, e->S.nfw, e->d_fw_ptr,
+                     e->d_fw_rows, e->S.nbw, e->d_bw_ptr, e->d_bw_rows,
+                     use_lds, (T *)e->d_xglob);
+  tend(e);
+}
+
+static void launch_iteration(bpf_engine *e) {
+  if (e->lu_fp64) launch_newton_step<double>(e);
+  else launch_newton_step<float>(e);
+  tbegin(e, "update");
+  hipLaunchKernelGGL(k_update, dim3(div_up(e->S.nb, 256), e->T), dim3(256), 0,
+                     e->stream, e->g, e->d);
+  tend(e);
+  launch_mismatch(e, 1);
+  hipMemcpyAsync(e->h_count, e->d.active_count, sizeof(int),
+                 hipMemcpyDeviceToHost, e->stream);
+}
+
+static bpf_status set_lds_limits(bpf_engine *e) {
+  // allow the dynamic LDS request of the solve kernels up to the device limit
+  hipFuncSetAttribute((const void *)k_lu_solve<float>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, e->max_lds);
+  hipFuncSetAttribute((const void *)k_lu_solve<double>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, e->max_lds);
+  (void)hipGetLastError();
+  return BPF_OK;
+}
+
+static bpf_status run_newton(bpf_engine *e) {
+  bpf_status s = ensure_lu(e, e->opt.precision == BPF_FP64 ? 1 : 0);
+  if (s) return s;
+  set_lds_limits(e);
+  const int key = e->lu_fp64 * 1000000 + e->lu_threads * 1000 + e->solve_threads;
+  bool use_graph = e->opt.use_graph && !e->timing;
+  if (use_graph && (!e->graph || e->graph_key != key)) {
+    if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
+    hipGraph_t gr;
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    launch_iteration(e);
+    HIPCHK(hipStreamEndCapture(e->stream, &gr));
+    hipError_t r = hipGraphInstantiate(&e->graph, gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    if (r != hipSuccess) {
+      e->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(r);
+      return BPF_ERR_HIP;
+    }
+    e->graph_key = key;
+  }
+  const int cap = e->opt.maxit_nr * (e->opt.max_qlim_rounds + 1) + 2;
+  for (int it = 0; it < cap; it++) {
+    if (use_graph) HIPCHK(hipGraphLaunch(e->graph, e->stream));
+    else launch_iteration(e);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipGetLastError());
+    if (*e->h_count == 0) break;
+  }
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_solve(bpf_engine *e) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  if (e->T <= 0) { e->err = "bpf_set_instances not called"; return BPF_ERR_STATE; }
+  if (e->opt.alg != BPF_NR) { e->err = "fast decoupled: see bpf_solve_fd"; }
+  const unsigned T = (unsigned)e->T;
+  tbegin(e, "ybus");
+  hipLaunchKernelGGL(k_ybus, dim3(div_up(e->g.nnzY, 256), T), dim3(256), 0,
+                     e->stream, e->g, e->d);
+  tend(e);
+  tbegin(e, "init");
+  hipLaunchKernelGGL(k_init, dim3(div_up(e->n, 256), T), dim3(256), 0,
+                     e->stream, e->g, e->d, (const int8_t *)e->d_type0,
+                     (const double *)e->d_Qinj0);
+  tend(e);
+  launch_mismatch(e, 0);
+  HIPCHK(hipMemcpyAsync(e->h_count, e->d.active_count, sizeof(int),
+                        hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipGetLastError());
+  if (*e->h_count > 0) {
+    bpf_status s = run_newton(e);
+    if (s) return s;
+  }
+  tbegin(e, "loss");
+  hipLaunchKernelGGL(k_loss, dim3(T), dim3(256), 0, e->stream, e->g, e->d,
+                     e->d_Sf, e->d_St, e->d_loss);
+  tend(e);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipGetLastError());
+  tcollect(e);
+  e->solved = true;
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_get_results(bpf_engine *e, double *V, int32_t *its,
+                                      int32_t *verdict, double *norm,
+                                      double *loss, int32_t *rounds,
+                                      int32_t *bus_type) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  if (!e->solved) { e->err = "no solution yet"; return BPF_ERR_STATE; }
+  const size_t T = e->T, n = e->n;
+  if (V) HIPCHK(hipMemcpy(V, e->d.V, sizeof(cd) * T * n, hipMemcpyDeviceToHost));
+  std::vector<InstState> st(T);
+  HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * T, hipMemcpyDeviceToHost));
+  for (size_t t = 0; t < T; t++) {
+    if (its) its[t] = st[t].it_total;
+    if (verdict) verdict[t] = st[t].verdict;
+    if (norm) norm[t] = st[t].norm;
+    if (rounds) rounds[t] = st[t].rounds;
+  }
+  if (loss) HIPCHK(hipMemcpy(loss, e->d_loss, sizeof(double) * T, hipMemcpyDeviceToHost));
+  if (bus_type) {
+    std::vector<int8_t> ty(T * n);
+    HIPCHK(hipMemcpy(ty.data(), e->d.type, T * n, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < T * n; k++) bus_type[k] = ty[k];
+  }
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_get_branch_flows(bpf_engine *e, double *Sf, double *St) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  if (!e->solved) { e->err = "no solution yet"; return BPF_ERR_STATE; }
+  size_t bytes = sizeof(cd) * (size_t)e->T * e->m;
+  if (Sf) HIPCHK(hipMemcpy(Sf, e->d_Sf, bytes, hipMemcpyDeviceToHost));
+  if (St) HIPCHK(hipMemcpy(St, e->d_St, bytes, hipMemcpyDeviceToHost));
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info) {
+  if (!e || !info) return BPF_ERR_INVALID_ARG;
+  const Symbolic &S = e->S;
+  info->nbus = e->n;
+  info->nbranch = e->m;
+  info->nblocks = S.nb;
+  info->nnz_ybus = (int)S.ycol.size();
+  info->nnz_jac_blocks = S.nnz_jac_blocks;
+  info->nnz_lu_blocks = (int)S.lcol.size();
+  info->lu_levels = S.nlev;
+  info->fw_levels = S.nfw;
+  info->bw_levels = S.nbw;
+  info->lu_pairs = S.npairs;
+  info->instances = e->T;
+  int fp64 = e->opt.precision == BPF_FP64;
+  info->lds_solve = (int64_t)((fp64 ? 8 : 4) * 2 * (size_t)S.nb) <= e->max_lds;
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_set_timing(bpf_engine *e, int32_t enable) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  e->timing = enable ? 1 : 0;
+  e->ktime.clear();
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_get_kernel_time(const bpf_engine *e, const char *name,
+                                          int32_t *launches, double *total_ms) {
+  if (!e || !name) return BPF_ERR_INVALID_ARG;
+  auto it = e->ktime.find(name);
+  if (launches) *launches = it == e->ktime.end() ? 0 : it->second.first;
+  if (total_ms) *total_ms = it == e->ktime.end() ? 0.0 : it->second.second;
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_get_ybus(bpf_engine *e, int32_t t, int32_t *rowptr,
+                                   int32_t *colind, double *val) {
+  if (!e || t < 0 || t >= e->T) return BPF_ERR_INVALID_ARG;
+  if (!e->solved) { e->err = "no solution yet"; return BPF_ERR_STATE; }
+  if (rowptr) memcpy(rowptr, e->S.yrow.data(), sizeof(int) * (e->n + 1));
+  if (colind) memcpy(colind, e->S.ycol.data(), sizeof(int) * e->S.ycol.size());
+  if (val)
+    HIPCHK(hipMemcpy(val, e->d.Y + (size_t)t * e->g.nnzY, sizeof(cd) * e->g.nnzY,
+                     hipMemcpyDeviceToHost));
+  return BPF_OK;
+}
+
+// upload voltages and evaluate the mismatch (mode 2: no verdicts)
+static bpf_status stage_eval(bpf_engine *e, const double *V) {
+  if (!e->solved) { e->err = "call bpf_solve first (Ybus is assembled there)"; return BPF_ERR_STATE; }
+  HIPCHK(hipMemcpy(e->d.V, V, sizeof(cd) * (size_t)e->T * e->n, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_mismatch, dim3(e->T), dim3(e->mis_threads), 0, e->stream,
+                     e->g, e->d, 2, e->opt.maxit_nr, e->opt.tol, 0, 0, 0.0, 0);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipGetLastError());
+  return BPF_OK;
+}
+
+// block-ordered vector [T][2nb] -> reference layout [P(pv); P(pq); Q(pq)]
+static void to_ref_layout(const bpf_engine *e, const int8_t *ty,
+                          const double *blk, double *out) {
+  const Symbolic &S = e->S;
+  std::vector<int> pv, pq;
+  for (int i = 0; i < e->n; i++) {
+    if (ty[i] == BPF_PV) pv.push_back(i);
+    else if (ty[i] == BPF_PQ && S.blk_of_bus[i] >= 0) pq.push_back(i);
+  }
+  int npv = (int)pv.size(), npq = (int)pq.size();
+  for (int k = 0; k < npv; k++) out[k] = blk[2 * S.blk_of_bus[pv[k]]];
+  for (int k = 0; k < npq; k++) {
+    out[npv + k] = blk[2 * S.blk_of_bus[pq[k]]];
+    out[npv + npq + k] = blk[2 * S.blk_of_bus[pq[k]] + 1];
+  }
+}
+
+extern "C" bpf_status bpf_stage_mismatch(bpf_engine *e, const double *V,
+                                         double *F, double *norm) {
+  if (!e || !V) return BPF_ERR_INVALID_ARG;
+  bpf_status s = stage_eval(e, V);
+  if (s) return s;
+  const size_t T = e->T, n = e->n, nb2 = 2 * (size_t)e->S.nb;
+  std::vector<double> Fb(T * nb2);
+  std::vector<int8_t> ty(T * n);
+  std::vector<InstState> st(T);
+  HIPCHK(hipMemcpy(Fb.data(), e->d.F, sizeof(double) * T * nb2, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ty.data(), e->d.type, T * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * T, hipMemcpyDeviceToHost));
+  for (size_t t = 0; t < T; t++) {
+    if (F) to_ref_layout(e, ty.data() + t * n, Fb.data() + t * nb2, F + t * 2 * n);
+    if (norm) norm[t] = st[t].norm;
+  }
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V,
+                                         int32_t t, int32_t cap, int32_t *row,
+                                         int32_t *col, double *val,
+                                         int32_t *nnz) {
+  if (!e || !V || t < 0 || t >= e->T || !nnz) return BPF_ERR_INVALID_ARG;
+  bpf_status s = stage_eval(e, V);
+  if (s) return s;
+  if ((s = ensure_lu(e, 1))) return s;
+  // mark instances active for the fill
+  std::vector<InstState> st(e->T);
+  HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * e->T, hipMemcpyDeviceToHost));
+  std::vector<InstState> st2 = st;
+  for (auto &x : st2) x.done = 0;
+  HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
+  B4<double> *LU = (B4<double> *)e->d_LU;
+  HIPCHK(hipMemset(LU, 0, sizeof(B4<double>) * (size_t)e->T * e->g.nnzLU));
+  hipLaunchKernelGGL(k_jacobian<double>, dim3(div_up(e->g.nnzY, 256), e->T),
+                     dim3(256), 0, e->stream, e->g, e->d, LU);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(e->d.st, st.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
+  std::vector<B4<double>> blk(e->g.nnzLU);
+  HIPCHK(hipMemcpy(blk.data(), LU + (size_t)t * e->g.nnzLU,
+                   sizeof(B4<double>) * e->g.nnzLU, hipMemcpyDeviceToHost));
+  std::vector<int8_t> ty(e->n);
+  HIPCHK(hipMemcpy(ty.data(), e->d.type + (size_t)t * e->n, e->n, hipMemcpyDeviceToHost));
+  // reference indices: P row/angle col = indicePVPQ, Q row/magnitude col =
+  // indicePVPQ + npq (newtonpf.h:256-291)
+  const Symbolic &S = e->S;
+  std::vector<int> idx(e->n, -1);
+  int npv = 0, npq = 0;
+  for (int i = 0; i < e->n; i++) if (ty[i] == BPF_PV) idx[i] = npv++;
+  for (int i = 0; i < e->n; i++) if (ty[i] == BPF_PQ && S.blk_of_bus[i] >= 0) idx[i] = npv + npq++;
+  int cnt = 0;
+  for (int p = 0; p < e->g.nnzY; p++) {
+    int q = S.y2lu[p];
+    if (q < 0) continue;
+    int i = S.yrowof[p], j = S.ycol[p];
+    const B4<double> &b = blk[q];
+    bool pqi = ty[i] == BPF_PQ, pqj = ty[j] == BPF_PQ;
+    auto put = [&](int r, int c, double v) {
+      if (cnt < cap && row && col && val) { row[cnt] = r; col[cnt] = c; val[cnt] = v; }
+      cnt++;
+    };
+    put(idx[i], idx[j], b.a);
+    if (pqj) put(idx[i], idx[j] + npq, b.b);
+    if (pqi) put(idx[i] + npq, idx[j], b.c);
+    if (pqi && pqj) put(idx[i] + npq, idx[j] + npq, b.d);
+  }
+  *nnz = cnt;
+  return cnt <= cap ? BPF_OK : BPF_ERR_INVALID_ARG;
+}
+
+extern "C" bpf_status bpf_stage_newton_step(bpf_engine *e, const double *V,
+                                            double *dx) {
+  if (!e || !V) return BPF_ERR_INVALID_ARG;
+  bpf_status s = stage_eval(e, V);
+  if (s) return s;
+  int fp64 = e->opt.precision == BPF_FP64;
+  if ((s = ensure_lu(e, fp64))) return s;
+  set_lds_limits(e);
+  std::vector<InstState> st(e->T);
+  HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * e->T, hipMemcpyDeviceToHost));
+  std::vector<InstState> st2 = st;
+  for (auto &x : st2) x.done = 0;
+  HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
+  if (fp64) launch_newton_step<double>(e); else launch_newton_step<float>(e);
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipGetLastError());
+  tcollect(e);
+  const size_t T = e->T, n = e->n, nb2 = 2 * (size_t)e->S.nb;
+  std::vector<InstState> st3(T);
+  HIPCHK(hipMemcpy(st3.data(), e->d.st, sizeof(InstState) * T, hipMemcpyDeviceToHost));
+  std::vector<double> xb(T * nb2);
+  std::vector<int8_t> ty(T * n);
+  HIPCHK(hipMemcpy(xb.data(), e->d.dx, sizeof(double) * T * nb2, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ty.data(), e->d.type, T * n, hipMemcpyDeviceToHost));
+  for (size_t t = 0; t < T; t++) {
+    if (st3[t].done && st3[t].verdict == BPF_V_SINGULAR) {
+      for (size_t k = 0; k < 2 * n; k++) dx[t * 2 * n + k] = NAN;
+      continue;
+    }
+    to_ref_layout(e, ty.data() + t * n, xb.data() + t * nb2, dx + t * 2 * n);
+  }
+  HIPCHK(hipMemcpy(e->d.st, st.data(), sizeof(InstState) * T, hipMemcpyHostToDevice));
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
+                                  double *gP, double *gQ, double *gx,
+                                  double *obj) {
+  (void)rate; (void)w; (void)gP; (void)gQ; (void)gx; (void)obj;
+  if (!e) return BPF_ERR_INVALID_ARG;
+  e->err = "adjoint not built yet";
+  return BPF_ERR_UNSUPPORTED;
+}

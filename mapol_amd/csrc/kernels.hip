@@ -1,0 +1,528 @@
+// HIP kernels of the batched AC power-flow engine (gfx950, wave64).
+//
+// Launch shapes:
+//   * element kernels (Ybus assembly, Jacobian fill, voltage update): 2-D grid
+//     (elements / 256, instances), one thread per element;
+//   * per-instance kernels (mismatch + verdicts, LU factorization, triangular
+//     solves with refinement, losses): one workgroup per instance; levels of
+//     the elimination tree are separated by workgroup barriers only, so no
+//     grid-wide synchronisation and no host round trip inside a stage.
+#include <float.h>
+
+#include "kernels.hpp"
+#include "symbolic.hpp"
+
+namespace bpf {
+
+// ---------------------------------------------------------------------------
+// complex FP64 helpers (division: Smith's 1962 scaled algorithm)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ cd mk(double x, double y) { cd z; z.x = x; z.y = y; return z; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cmul(cd a, cd b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ cd cconj(cd a) { return mk(a.x, -a.y); }
+__device__ __forceinline__ double cabs_(cd a) { return hypot(a.x, a.y); }
+__device__ __forceinline__ cd cdiv(cd n, cd d) {
+  if (fabs(d.x) >= fabs(d.y)) {
+    double r = d.y / d.x, den = d.x + d.y * r;
+    return mk((n.x + n.y * r) / den, (n.y - n.x * r) / den);
+  }
+  double r = d.x / d.y, den = d.x * r + d.y;
+  return mk((n.x * r + n.y) / den, (n.y * r - n.x) / den);
+}
+__device__ __forceinline__ cd cexpj(double th) {
+  double s, c;
+  sincos(th, &s, &c);
+  return mk(c, s);
+}
+
+// pi-model entry of branch b for instance arrays (makeYbus.h:63-80)
+__device__ cd branch_entry(const DevTopo &g, const DevInst &d, int t, int b,
+                           int kind) {
+  double stat = d.br_on[(size_t)t * g.m + b] ? 1.0 : 0.0;
+  cd Ys = cdiv(mk(stat, 0.0), mk(g.R[b], g.X[b]));
+  double tp = d.tap[(size_t)t * g.m + b];
+  cd tap = (tp != 0.0) ? mk(tp, 0.0) : mk(1.0, 0.0);
+  tap = cmul(tap, cexpj(M_PI / 180.0 * g.shift[b]));
+  switch (kind) {
+    case KFF: {
+      cd Bc = cmul(mk(stat, 0.0), mk(0.0, g.B[b]));
+      cd Ytt = cadd(Ys, cdiv(Bc, mk(2.0, 0.0)));
+      return cdiv(Ytt, cmul(tap, cconj(tap)));
+    }
+    case KFT: return cdiv(cmul(Ys, mk(-1.0, 0.0)), cconj(tap));
+    case KTF: return cdiv(cmul(Ys, mk(-1.0, 0.0)), tap);
+    default: {
+      cd Bc = cmul(mk(stat, 0.0), mk(0.0, g.B[b]));
+      return cadd(Ys, cdiv(Bc, mk(2.0, 0.0)));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ybus assembly with outage masks: one thread per (position, instance).
+// Sum order = the reference's: from-side terms, then to-side terms, then the
+// bus shunt (Cf*Yf + Ct*Yt + Ysh, makeYbus.h:196-266).
+// ---------------------------------------------------------------------------
+__global__ void k_ybus(DevTopo g, DevInst d) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.y;
+  if (p >= g.nnzY) return;
+  cd af = mk(0.0, 0.0), at = mk(0.0, 0.0);
+  for (int q = g.yf_ptr[p]; q < g.yf_ptr[p + 1]; q++) {
+    int it = g.yf_item[q];
+    af = cadd(af, branch_entry(g, d, t, it >> 2, it & 3));
+  }
+  for (int q = g.yt_ptr[p]; q < g.yt_ptr[p + 1]; q++) {
+    int it = g.yt_item[q];
+    at = cadd(at, branch_entry(g, d, t, it >> 2, it & 3));
+  }
+  cd y = cadd(af, at);
+  int i = g.ydiag_bus[p];
+  if (i >= 0) y = cadd(y, mk(g.Gs[i], d.Bs[(size_t)t * g.n + i]));
+  d.Y[(size_t)t * g.nnzY + p] = y;
+}
+
+// ---------------------------------------------------------------------------
+// Initial voltage (runpf.h:211-226) and per-solve state reset.
+// ---------------------------------------------------------------------------
+__global__ void k_init(DevTopo g, DevInst d, const int8_t *type0,
+                       const double *Qinj0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.y;
+  if (i == 0 && t == 0) *d.active_count = 0;
+  if (i == 0) {
+    InstState s;
+    s.done = 0; s.verdict = 1; s.it_round = 0; s.it_total = 0; s.rounds = 0;
+    s.fd_half = 0; s.norm = 0.0;
+    d.st[t] = s;
+  }
+  if (i >= g.n) return;
+  size_t o = (size_t)t * g.n + i;
+  double v = d.vset[o];
+  cd V = cmul(mk(v, 0.0), cexpj(g.Va0[i]));
+  int bt = g.base_type[i];
+  if (bt == 2 || bt == 3) V = cmul(mk(v / cabs_(V), 0.0), V);
+  d.V[o] = V;
+  d.type[o] = type0[o];
+  d.Qinj[o] = Qinj0[o];
+}
+
+// ---------------------------------------------------------------------------
+// workgroup reductions (wave64 shuffles + LDS)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_or(int v) {
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_add(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// all threads get the result
+__device__ double block_max(double v, double *sh) {
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = (blockDim.x + 63) >> 6;
+  v = wave_max(v);
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int k = 0; k < nw; k++) r = fmax(r, sh[k]);
+  return r;
+}
+__device__ double block_sum(double v, double *sh) {
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = (blockDim.x + 63) >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int k = 0; k < nw; k++) r += sh[k];
+  return r;
+}
+__device__ int block_or(int v, int *sh) {
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = (blockDim.x + 63) >> 6;
+  v = wave_or(v);
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  int r = 0;
+  for (int k = 0; k < nw; k++) r |= sh[k];
+  return r;
+}
+__device__ int block_add(int v, int *sh) {
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = (blockDim.x + 63) >> 6;
+  v = wave_add(v);
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  int r = 0;
+  for (int k = 0; k < nw; k++) r += sh[k];
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Mismatch S = V o conj(Ybus V) - Sbus, inf-norm, verdicts and reactive-
+// limit switching; one workgroup per instance (newtonpf.h:18-95,394-396,
+// 518-520).  F is kept in block order: F[2k] = P, F[2k+1] = Q (0 for PV).
+// mode: 0 initial check, 1 after a Newton step, 2 evaluation only.
+// ---------------------------------------------------------------------------
+__global__ void k_mismatch(DevTopo g, DevInst d, int mode, int maxit,
+                           double tol, int enforce_qlim, int max_rounds,
+                           double qtol, int fd_norm) {
+  __shared__ double shd[16];
+  __shared__ int shi[16];
+  const int t = blockIdx.x;
+  InstState *sp = d.st + t;
+  if (mode != 2 && sp->done) return;
+  InstState s = *sp;
+  const cd *Y = d.Y + (size_t)t * g.nnzY;
+  const cd *V = d.V + (size_t)t * g.n;
+  cd *Ib = d.Ib + (size_t)t * g.n;
+  int8_t *ty = d.type + (size_t)t * g.n;
+  double *Pinj = d.Pinj + (size_t)t * g.n;
+  double *Qinj = d.Qinj + (size_t)t * g.n;
+  double *F = d.F + (size_t)t * 2 * g.nb;
+  if (mode == 1) { s.it_round++; s.it_total++; }
+  bool active = false;
+  for (int guard = 0; guard <= max_rounds + 1; guard++) {
+    double lmax = 0.0;
+    int lnan = 0;
+    for (int k = threadIdx.x; k < g.nb; k += blockDim.x) {
+      int i = g.bus_of_blk[k];
+      cd I = mk(0.0, 0.0);
+      for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++)
+        I = cadd(I, cmul(Y[p], V[g.ycol[p]]));
+      Ib[i] = I;
+      cd mis = csub(cmul(V[i], cconj(I)), mk(Pinj[i], Qinj[i]));
+      if (fd_norm) mis = cdiv(mis, mk(cabs_(V[i]), 0.0));
+      bool pq = ty[i] == 1;
+      F[2 * k] = mis.x;
+      F[2 * k + 1] = pq ? mis.y : 0.0;
+      lmax = fmax(lmax, fabs(mis.x));
+      lnan |= isnan(mis.x);
+      if (pq) { lmax = fmax(lmax, fabs(mis.y)); lnan |= isnan(mis.y); }
+    }
+    double nrm = block_max(lmax, shd);
+    int anynan = block_or(lnan, shi);
+    s.norm = anynan ? NAN : nrm;
+    if (mode == 2) break;
+    if (anynan || isinf(nrm)) { s.verdict = BPF_V_NONFINITE; s.done = 1; break; }
+    if (nrm < tol) {
+      if (!enforce_qlim) { s.verdict = BPF_V_CONVERGED; s.done = 1; break; }
+      int lv = 0;
+      for (int k = threadIdx.x; k < g.nb; k += blockDim.x) {
+        int i = g.bus_of_blk[k];
+        if (ty[i] != 2) continue;
+        double qg = cmul(V[i], cconj(Ib[i])).y + g.Qd[i];
+        size_t o = (size_t)t * g.n + i;
+        if (qg > d.Qmax[o] + qtol || qg < d.Qmin[o] - qtol) lv++;
+      }
+      int nv = block_add(lv, shi);
+      if (nv == 0) { s.verdict = BPF_V_CONVERGED; s.done = 1; break; }
+      if (s.rounds >= max_rounds) { s.verdict = BPF_V_MAX_ITER; s.done = 1; break; }
+      s.rounds++;
+      s.it_round = 0;
+      for (int k = threadIdx.x; k < g.nb; k += blockDim.x) {
+        int i = g.bus_of_blk[k];
+        if (ty[i] != 2) continue;
+        double qg = cmul(V[i], cconj(Ib[i])).y + g.Qd[i];
+        size_t o = (size_t)t * g.n + i;
+        if (qg > d.Qmax[o] + qtol) { ty[i] = 1; Qinj[i] = d.Qmax[o] - g.Qd[i]; }
+        else if (qg < d.Qmin[o] - qtol) { ty[i] = 1; Qinj[i] = d.Qmin[o] - g.Qd[i]; }
+      }
+      __syncthreads();
+      continue;  // re-evaluate the mismatch with the new bus types
+    }
+    if (s.it_round >= maxit) { s.verdict = BPF_V_MAX_ITER; s.done = 1; break; }
+    active = true;
+    break;
+  }
+  if (threadIdx.x == 0) {
+    *sp = s;
+    if (active) atomicAdd(d.active_count, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian fill into the L+U storage (compuateJacobianMatrix,
+// newtonpf.h:97-186): one thread per (Ybus position, instance) writes one
+// 2x2 block.  Superset pattern: a PV bus keeps an identity Q row and a zero
+// magnitude column, so PV->PQ switching never changes the pattern.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *d.active_count = 0;
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.y;
+  if (p >= g.nnzY) return;
+  if (d.st[t].done) return;
+  int q = g.y2lu[p];
+  if (q < 0) return;
+  int ii = g.yrowof[p], jj = g.ycol[p];
+  const cd *V = d.V + (size_t)t * g.n;
+  cd adm = d.Y[(size_t)t * g.nnzY + p];
+  double yr = adm.x, yi = adm.y;
+  double vrj = V[jj].x, vij = V[jj].y, vri = V[ii].x, vii = V[ii].y;
+  double ibr = 0.0, ibi = 0.0;
+  if (ii == jj) { cd I = d.Ib[(size_t)t * g.n + ii]; ibr = I.x; ibi = I.y; }
+  double re = yr * vrj - yi * vij;
+  double im = yr * vij + yi * vrj;
+  double ab = sqrt(vrj * vrj + vij * vij);
+  double ren = yr * vrj / ab - yi * vij / ab;
+  double imn = yr * vij / ab + yi * vrj / ab;
+  double a = -vii * (ibr - re) - vri * (-ibi + im);
+  double b = vri * ren - vii * -imn + ibr * vrj / ab + ibi * vij / ab;
+  double c = -vii * (-ibi + im) + vri * (ibr - re);
+  double e = vri * -imn + vii * ren + ibr * vij / ab + -ibi * vrj / ab;
+  int ti = d.type[(size_t)t * g.n + ii], tj = d.type[(size_t)t * g.n + jj];
+  if (tj == 2) { b = 0.0; e = 0.0; }
+  if (ti == 2) { c = 0.0; e = (ii == jj) ? 1.0 : 0.0; }
+  B4<T> v;
+  v.a = (T)a; v.b = (T)b; v.c = (T)c; v.d = (T)e;
+  LU[(size_t)t * g.nnzLU + q] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 2x2 block helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void blk_fms(B4<T> &acc, const B4<T> &l, const B4<T> &u) {
+  acc.a -= l.a * u.a + l.b * u.c;
+  acc.b -= l.a * u.b + l.b * u.d;
+  acc.c -= l.c * u.a + l.d * u.c;
+  acc.d -= l.c * u.b + l.d * u.d;
+}
+template <typename T>
+__device__ __forceinline__ B4<T> blk_mul(const B4<T> &x, const B4<T> &y) {
+  B4<T> r;
+  r.a = x.a * y.a + x.b * y.c;
+  r.b = x.a * y.b + x.b * y.d;
+  r.c = x.c * y.a + x.d * y.c;
+  r.d = x.c * y.b + x.d * y.d;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Batched Crout LU of the 2x2-block Jacobian, static pivot order from the
+// symbolic analysis.  One workgroup per instance.  Level l of the
+// elimination tree: phase A computes every output of the level's pivots as
+// A(p) - sum L*U over its precomputed pair list (the pivot block is stored
+// inverted); phase B scales the level's L blocks by the pivot inverse.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_lu_factor(DevTopo g, InstState *st, int nlev,
+                            const int *lvA, const int *tA_dst, const int *tA_pp,
+                            const int *pairL, const int *pairU, const int *lvB,
+                            const int *tB_dst, const int *tB_diag, B4<T> *LU) {
+  const int t = blockIdx.x;
+  if (st[t].done) return;
+  __shared__ int sing;
+  if (threadIdx.x == 0) sing = 0;
+  B4<T> *A = LU + (size_t)t * g.nnzLU;
+  __syncthreads();
+  for (int l = 0; l < nlev; l++) {
+    for (int k = lvA[l] + threadIdx.x; k < lvA[l + 1]; k += blockDim.x) {
+      int dst = tA_dst[k];
+      int p = dst & TASK_POS_MASK;
+      B4<T> acc;
+      if (dst & TASK_HAS_A) acc = A[p];
+      else { acc.a = acc.b = acc.c = acc.d = (T)0; }
+      int q1 = tA_pp[k + 1];
+      for (int q = tA_pp[k]; q < q1; q++) blk_fms(acc, A[pairL[q]], A[pairU[q]]);
+      if (dst & TASK_DIAG) {
+        T det = acc.a * acc.d - acc.b * acc.c;
+        if (!(det != (T)0) || !isfinite((double)det)) sing = 1;
+        T r = (T)1 / det;
+        B4<T> inv;
+        inv.a = acc.d * r; inv.b = -acc.b * r; inv.c = -acc.c * r; inv.d = acc.a * r;
+        A[p] = inv;
+      } else {
+        A[p] = acc;
+      }
+    }
+    __syncthreads();
+    for (int k = lvB[l] + threadIdx.x; k < lvB[l + 1]; k += blockDim.x) {
+      int p = tB_dst[k];
+      A[p] = blk_mul(A[p], A[tB_diag[k]]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sing) {
+    InstState s = st[t];
+    s.done = 1;
+    s.verdict = BPF_V_SINGULAR;
+    s.it_round++;
+    s.it_total++;
+    st[t] = s;
+  }
+}
+
+// forward (unit-lower L) and backward (U, inverted pivots) sweeps on a
+// block vector X (LDS or global, reached through a generic pointer)
+template <typename T>
+__device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, T *X, int nfw,
+                          const int *fw_ptr, const int *fw_rows, int nbw,
+                          const int *bw_ptr, const int *bw_rows) {
+  for (int l = 0; l < nfw; l++) {
+    for (int r = fw_ptr[l] + threadIdx.x; r < fw_ptr[l + 1]; r += blockDim.x) {
+      int i = fw_rows[r];
+      T x0 = X[2 * i], x1 = X[2 * i + 1];
+      for (int p = g.lrow[i]; p < g.ldiag[i]; p++) {
+        int c = g.lcol[p];
+        B4<T> L = A[p];
+        T y0 = X[2 * c], y1 = X[2 * c + 1];
+        x0 -= L.a * y0 + L.b * y1;
+        x1 -= L.c * y0 + L.d * y1;
+      }
+      X[2 * i] = x0;
+      X[2 * i + 1] = x1;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < nbw; l++) {
+    for (int r = bw_ptr[l] + threadIdx.x; r < bw_ptr[l + 1]; r += blockDim.x) {
+      int i = bw_rows[r];
+      T x0 = X[2 * i], x1 = X[2 * i + 1];
+      int dg = g.ldiag[i];
+      for (int p = dg + 1; p < g.lrow[i + 1]; p++) {
+        int c = g.lcol[p];
+        B4<T> U = A[p];
+        T y0 = X[2 * c], y1 = X[2 * c + 1];
+        x0 -= U.a * y0 + U.b * y1;
+        x1 -= U.c * y0 + U.d * y1;
+      }
+      B4<T> D = A[dg];
+      X[2 * i] = D.a * x0 + D.b * x1;
+      X[2 * i + 1] = D.c * x0 + D.d * x1;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Newton step: dx = J^-1 F with the factors in precision T, then
+// `refine` FP64 iterative-refinement steps whose residual r = F - J dx is
+// evaluated matrix-free in FP64 from Ybus, V and Ibus:
+//   dS_i = dV_i conj(I_i) + V_i conj(sum_j Y_ij dV_j),
+//   dV_j = V_j (j dtheta_j + dVm_j / |V_j|).
+// One workgroup per instance.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, int refine,
+                           int nfw, const int *fw_ptr, const int *fw_rows,
+                           int nbw, const int *bw_ptr, const int *bw_rows,
+                           int use_lds, T *xglob) {
+  extern __shared__ unsigned char smem[];
+  const int t = blockIdx.x;
+  if (d.st[t].done) return;
+  T *X = use_lds ? reinterpret_cast<T *>(smem) : xglob + (size_t)t * 2 * g.nb;
+  const B4<T> *A = LU + (size_t)t * g.nnzLU;
+  const double *F = d.F + (size_t)t * 2 * g.nb;
+  double *dx = d.dx + (size_t)t * 2 * g.nb;
+  const cd *Y = d.Y + (size_t)t * g.nnzY;
+  const cd *V = d.V + (size_t)t * g.n;
+  const cd *Ib = d.Ib + (size_t)t * g.n;
+  const int8_t *ty = d.type + (size_t)t * g.n;
+  for (int s = 0; s <= refine; s++) {
+    if (s == 0) {
+      for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) X[k] = (T)F[k];
+    } else {
+      for (int k = threadIdx.x; k < g.nb; k += blockDim.x) {
+        int i = g.bus_of_blk[k];
+        cd acc = mk(0.0, 0.0);
+        for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++) {
+          int j = g.ycol[p];
+          int bj = g.blk_of_bus[j];
+          if (bj < 0) continue;
+          cd vj = V[j];
+          double m = dx[2 * bj + 1] / cabs_(vj);
+          cd dvj = cmul(vj, mk(m, dx[2 * bj]));
+          acc = cadd(acc, cmul(Y[p], dvj));
+        }
+        cd vi = V[i];
+        cd dvi = cmul(vi, mk(dx[2 * k + 1] / cabs_(vi), dx[2 * k]));
+        cd dS = cadd(cmul(dvi, cconj(Ib[i])), cmul(vi, cconj(acc)));
+        X[2 * k] = (T)(F[2 * k] - dS.x);
+        X[2 * k + 1] = (T)(ty[i] == 1 ? F[2 * k + 1] - dS.y : -dx[2 * k + 1]);
+      }
+    }
+    __syncthreads();
+    lu_sweeps<T>(g, A, X, nfw, fw_ptr, fw_rows, nbw, bw_ptr, bw_rows);
+    for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
+      dx[k] = (s == 0 ? 0.0 : dx[k]) + (double)X[k];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Polar update V = (|V| - dVm) e^{j(atan(Vi/Vr) - dtheta)} (newtonpf.h:188-213;
+// the reference's angle is atan(y/x), complexUtils.h:21-23).
+// ---------------------------------------------------------------------------
+__global__ void k_update(DevTopo g, DevInst d) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.y;
+  if (k >= g.nb) return;
+  if (d.st[t].done) return;
+  int i = g.bus_of_blk[k];
+  size_t o = (size_t)t * g.n + i;
+  const double *dx = d.dx + (size_t)t * 2 * g.nb;
+  cd v = d.V[o];
+  double mag = cabs_(v);
+  if (d.type[o] == 1) mag -= dx[2 * k + 1];
+  cd e = cexpj(atan(v.y / v.x) - dx[2 * k]);
+  d.V[o] = cmul(mk(mag, 0.0), e);
+}
+
+// ---------------------------------------------------------------------------
+// Losses and branch flows (computeLoss + reduceLoss, runpf.h:543-604; CPU
+// runpf.h:228-257).  One workgroup per instance.
+// ---------------------------------------------------------------------------
+__global__ void k_loss(DevTopo g, DevInst d, cd *Sf, cd *St, double *loss) {
+  __shared__ double sh[16];
+  const int t = blockIdx.x;
+  const cd *V = d.V + (size_t)t * g.n;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < g.m; b += blockDim.x) {
+    int f = g.br_from[b], to = g.br_to[b];
+    cd yff = branch_entry(g, d, t, b, KFF), yft = branch_entry(g, d, t, b, KFT);
+    cd ytf = branch_entry(g, d, t, b, KTF), ytt = branch_entry(g, d, t, b, KTT);
+    cd If, It;
+    if (f < to) {
+      If = cadd(cmul(yff, V[f]), cmul(yft, V[to]));
+      It = cadd(cmul(ytf, V[f]), cmul(ytt, V[to]));
+    } else {
+      If = cadd(cmul(yft, V[to]), cmul(yff, V[f]));
+      It = cadd(cmul(ytt, V[to]), cmul(ytf, V[f]));
+    }
+    cd sf = cmul(cconj(If), V[f]);
+    cd s2 = cmul(cconj(It), V[to]);
+    Sf[(size_t)t * g.m + b] = cconj(sf);
+    St[(size_t)t * g.m + b] = cconj(s2);
+    acc += cadd(sf, s2).x;
+  }
+  double tot = block_sum(acc, sh);
+  if (threadIdx.x == 0)
+    loss[t] = (d.st[t].verdict == BPF_V_CONVERGED) ? tot : DBL_MAX;
+}
+
+// explicit instantiations used by the engine
+template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *);
+template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *);
+template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, const int *, const int *, const int *, const int *, const int *, const int *, const int *, const int *, B4<float> *);
+template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, const int *, const int *, const int *, const int *, const int *, const int *, const int *, const int *, B4<double> *);
+template __global__ void k_lu_solve<float>(DevTopo, DevInst, const B4<float> *, int, int, const int *, const int *, int, const int *, const int *, int, float *);
+template __global__ void k_lu_solve<double>(DevTopo, DevInst, const B4<double> *, int, int, const int *, const int *, int, const int *, const int *, int, double *);
+
+}  // namespace bpf

@@ -1,0 +1,62 @@
+// Device-side data types and kernel declarations of the batched power-flow
+// engine.  Layout in HBM: every per-instance array is a slab [T][...] with
+// the instance index outermost, so a workgroup that owns one instance reads
+// one contiguous slab.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bpf {
+
+// verdict codes (same values as BPF_CONVERGED ... in include/batchpf.h)
+enum { BPF_V_CONVERGED = 0, BPF_V_MAX_ITER = 1, BPF_V_SINGULAR = 2,
+       BPF_V_NONFINITE = 3 };
+
+struct cd { double x, y; };  // complex FP64 (re, im), 16-byte element
+
+// 2x2 Jacobian block [a b; c d]: rows (P, Q), columns (angle, magnitude)
+template <typename T> struct alignas(4 * sizeof(T)) B4 { T a, b, c, d; };
+
+// per-instance solver state
+struct InstState {
+  int done;      // 1: no more Newton iterations for this instance
+  int verdict;   // BPF_CONVERGED / MAX_ITER / SINGULAR / NONFINITE
+  int it_round;  // Newton iterations in the current reactive-limit round
+  int it_total;  // all Newton iterations
+  int rounds;    // reactive-limit rounds used
+  int fd_half;   // FDPF: 0 before the P half-step, 1 before the Q half-step
+  double norm;   // last inf-norm of the mismatch
+};
+
+// shared (topology) arrays on the device
+struct DevTopo {
+  int n, m, nb, nnzY, nnzLU;
+  const int *yrow, *ycol, *yrowof, *ydiag_bus;
+  const int *yf_ptr, *yf_item, *yt_ptr, *yt_item;
+  const int *blk_of_bus, *bus_of_blk, *y2lu;
+  const int *lrow, *lcol, *ldiag, *mirror;
+  const int *br_from, *br_to;
+  const double *R, *X, *B, *shift, *Gs, *Qd, *Va0;
+  const int *base_type;
+};
+
+// per-instance arrays on the device
+struct DevInst {
+  int T;
+  const uint8_t *br_on;    // [T][m]
+  const double *tap;       // [T][m]
+  const double *Bs;        // [T][n]
+  const double *vset;      // [T][n]
+  int8_t *type;            // [T][n] current bus types
+  double *Pinj, *Qinj;     // [T][n]
+  const double *Qmax, *Qmin;  // [T][n]
+  cd *Y;                   // [T][nnzY]
+  cd *V;                   // [T][n]
+  cd *Ib;                  // [T][n] Ybus*V at the last mismatch
+  double *F;               // [T][2nb] block order (P, Q) per block
+  double *dx;              // [T][2nb]
+  InstState *st;           // [T]
+  int *active_count;       // [1]
+};
+
+}  // namespace bpf

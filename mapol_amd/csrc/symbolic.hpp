@@ -1,0 +1,52 @@
+// Host-side symbolic analysis of the batched power-flow engine, done once per
+// topology (bpf_create).  Everything here is shared by all instances; only
+// numerical values are per instance.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bpf {
+
+// Ybus contribution item: branch index * 4 + kind (0 Yff, 1 Yft, 2 Ytf, 3 Ytt)
+enum { KFF = 0, KFT = 1, KTF = 2, KTT = 3 };
+
+// Crout task destination flags (bits above the block position)
+constexpr int TASK_HAS_A = 1 << 30;  // the position holds a Jacobian entry
+constexpr int TASK_DIAG = 1 << 29;   // pivot block: store its inverse
+constexpr int TASK_POS_MASK = (1 << 29) - 1;
+
+struct Symbolic {
+  int n = 0, m = 0, nb = 0;
+  // Ybus pattern over all buses (every branch, in service or not: outages
+  // only zero values, as the reference keeps the structure, makeYbus.h:69-71)
+  std::vector<int> yrow, ycol, yrowof;  // CSR + row index of each position
+  std::vector<int> ydiag_bus;           // bus index if position is (i,i), else -1
+  // contributions per position, in the reference's summation order: from-side
+  // terms in branch order, then to-side terms in branch order
+  std::vector<int> yf_ptr, yf_item, yt_ptr, yt_item;
+  // non-slack buses become 2x2 blocks, in elimination order
+  std::vector<int> blk_of_bus, bus_of_blk;
+  // block CSR of L+U (structurally symmetric), rows in elimination order
+  std::vector<int> lrow, lcol, ldiag, mirror;
+  std::vector<int> y2lu;  // Ybus position -> block position (-1: slack)
+  int nnz_jac_blocks = 0;
+  // Crout factorization schedule (one level = one elimination-tree height)
+  int nlev = 0;
+  std::vector<int> lvA, tA_dst, tA_pp, pairL, pairU;  // phase A: dot products
+  std::vector<int> lvB, tB_dst, tB_diag;              // phase B: L scaling
+  int64_t npairs = 0;
+  // triangular solves: rows grouped by dependency level
+  int nfw = 0, nbw = 0;
+  std::vector<int> fw_ptr, fw_rows, bw_ptr, bw_rows;
+};
+
+// bus_type: MATPOWER codes; buses of type PV or PQ are unknowns, every other
+// bus keeps its voltage (slack).  Returns false with a message on bad input.
+bool analyse(int n, int m, const int *from, const int *to,
+             const int *bus_type, Symbolic &S, std::string *err);
+
+// minimum-degree order of a symmetric graph given as adjacency lists
+std::vector<int> min_degree_order(std::vector<std::vector<int>> adj);
+
+}  // namespace bpf

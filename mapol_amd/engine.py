@@ -1,0 +1,317 @@
+"""ctypes binding of ``libbatchpf.so`` (declared in include/batchpf.h).
+
+The product path: every call goes through the C-ABI into the HIP engine.
+There is no CPU fallback; a missing library or device is an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbatchpf.so")
+
+BPF_NR, BPF_FDXB, BPF_FDBX = 0, 1, 2
+BPF_FP64, BPF_MIXED = 0, 1
+CONVERGED, MAX_ITER, SINGULAR, NONFINITE = 0, 1, 2, 3
+CTL_AVR, CTL_OLTC, CTL_SHC = 1, 2, 3
+
+_pi = C.POINTER(C.c_int32)
+_pd = C.POINTER(C.c_double)
+_pu8 = C.POINTER(C.c_uint8)
+
+# every entry point of include/batchpf.h
+EXPORTS = [
+    "bpf_default_options", "bpf_create", "bpf_set_instances", "bpf_solve",
+    "bpf_get_results", "bpf_get_branch_flows", "bpf_destroy",
+    "bpf_last_error", "bpf_get_info", "bpf_get_ybus", "bpf_stage_mismatch",
+    "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_adjoint",
+    "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
+]
+
+
+class BpfCase(C.Structure):
+    _fields_ = [("nbus", C.c_int32), ("nbranch", C.c_int32),
+                ("ngen", C.c_int32), ("baseMVA", C.c_double),
+                ("bus_type", _pi), ("Pd", _pd), ("Qd", _pd), ("Gs", _pd),
+                ("Bs", _pd), ("Vm", _pd), ("Va", _pd), ("br_from", _pi),
+                ("br_to", _pi), ("R", _pd), ("X", _pd), ("B", _pd),
+                ("tap", _pd), ("shift", _pd), ("br_status", _pu8),
+                ("gen_bus", _pi), ("Pg", _pd), ("Qg", _pd), ("Qmax", _pd),
+                ("Qmin", _pd), ("VG", _pd), ("gen_status", _pu8)]
+
+
+class BpfOptions(C.Structure):
+    _fields_ = [("alg", C.c_int32), ("precision", C.c_int32),
+                ("refine_steps", C.c_int32), ("maxit_nr", C.c_int32),
+                ("maxit_fd", C.c_int32), ("enforce_qlim", C.c_int32),
+                ("max_qlim_rounds", C.c_int32), ("use_graph", C.c_int32),
+                ("tol", C.c_double), ("qlim_tol", C.c_double)]
+
+
+class BpfCtl(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("id", C.c_int32)]
+
+
+class BpfInfo(C.Structure):
+    _fields_ = [("nbus", C.c_int32), ("nbranch", C.c_int32),
+                ("nblocks", C.c_int32), ("nnz_ybus", C.c_int32),
+                ("nnz_jac_blocks", C.c_int32), ("nnz_lu_blocks", C.c_int32),
+                ("lu_levels", C.c_int32), ("fw_levels", C.c_int32),
+                ("bw_levels", C.c_int32), ("lu_pairs", C.c_int64),
+                ("instances", C.c_int32), ("lds_solve", C.c_int32)]
+
+
+_LIB = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the engine library (build it with ``python -m mapol_amd.build``)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise EngineError(f"{path} is missing: build it with "
+                          "`python -m mapol_amd.build`")
+    L = C.CDLL(path)
+    V = C.c_void_p
+    L.bpf_default_options.argtypes = [C.POINTER(BpfOptions)]
+    L.bpf_default_options.restype = None
+    L.bpf_create.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfOptions),
+                             C.POINTER(V)]
+    L.bpf_set_instances.argtypes = [V, C.c_int32, _pu8, _pu8, C.c_int32,
+                                    C.POINTER(BpfCtl), _pd]
+    L.bpf_solve.argtypes = [V]
+    L.bpf_get_results.argtypes = [V, _pd, _pi, _pi, _pd, _pd, _pi, _pi]
+    L.bpf_get_branch_flows.argtypes = [V, _pd, _pd]
+    L.bpf_destroy.argtypes = [V]
+    L.bpf_destroy.restype = None
+    L.bpf_last_error.argtypes = [V]
+    L.bpf_last_error.restype = C.c_char_p
+    L.bpf_get_info.argtypes = [V, C.POINTER(BpfInfo)]
+    L.bpf_get_ybus.argtypes = [V, C.c_int32, _pi, _pi, _pd]
+    L.bpf_stage_mismatch.argtypes = [V, _pd, _pd, _pd]
+    L.bpf_stage_jacobian.argtypes = [V, _pd, C.c_int32, C.c_int32, _pi, _pi,
+                                     _pd, _pi]
+    L.bpf_stage_newton_step.argtypes = [V, _pd, _pd]
+    L.bpf_adjoint.argtypes = [V, _pd, C.c_double, _pd, _pd, _pd, _pd]
+    L.bpf_set_timing.argtypes = [V, C.c_int32]
+    L.bpf_get_kernel_time.argtypes = [V, C.c_char_p, _pi, _pd]
+    L.bpf_set_launch.argtypes = [V, C.c_int32, C.c_int32, C.c_int32]
+    for name in EXPORTS:
+        getattr(L, name)
+    _LIB = L
+    return L
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Engine:
+    """One batched power-flow engine over one case (``synth.Case``)."""
+
+    def __init__(self, case, alg: int = BPF_NR, precision: int = BPF_FP64,
+                 refine_steps: int = 1, tol: float = 1e-8, maxit_nr: int = 10,
+                 maxit_fd: int = 30, enforce_qlim: bool = False,
+                 max_qlim_rounds: int = 10, qlim_tol: float = 1e-6,
+                 use_graph: bool = True):
+        self.L = load_library()
+        self.case = case
+        self._keep = []
+        c = case
+
+        def arr(v, dt):
+            a = np.ascontiguousarray(v, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        s = BpfCase()
+        s.nbus, s.nbranch, s.ngen = c.nbus, c.nbranch, c.ngen
+        s.baseMVA = float(getattr(c, "baseMVA", 100.0))
+        s.bus_type = _ptr(arr(c.bus_type, np.int32), _pi)
+        for f in ("Pd", "Qd", "Gs", "Bs", "Vm", "Va", "R", "X", "B", "tap",
+                  "shift", "Pg", "Qg", "Qmax", "Qmin", "VG"):
+            setattr(s, f, _ptr(arr(getattr(c, f), np.float64), _pd))
+        s.br_from = _ptr(arr(c.br_from, np.int32), _pi)
+        s.br_to = _ptr(arr(c.br_to, np.int32), _pi)
+        s.br_status = _ptr(arr(c.br_status, np.uint8), _pu8)
+        s.gen_bus = _ptr(arr(c.gen_bus, np.int32), _pi)
+        s.gen_status = _ptr(arr(c.gen_status, np.uint8), _pu8)
+        o = BpfOptions()
+        self.L.bpf_default_options(C.byref(o))
+        o.alg, o.precision, o.refine_steps = alg, precision, refine_steps
+        o.maxit_nr, o.maxit_fd, o.tol = maxit_nr, maxit_fd, tol
+        o.enforce_qlim, o.max_qlim_rounds = int(enforce_qlim), max_qlim_rounds
+        o.qlim_tol, o.use_graph = qlim_tol, int(use_graph)
+        self.options = o
+        h = C.c_void_p()
+        st = self.L.bpf_create(C.byref(s), C.byref(o), C.byref(h))
+        self.h = h
+        if st != 0:
+            msg = self.L.bpf_last_error(h).decode() if h else "bpf_create"
+            self.close()
+            raise EngineError(f"bpf_create failed ({st}): {msg}")
+        self.T = 0
+
+    # -- helpers -----------------------------------------------------------
+    def _check(self, st, what):
+        if st != 0:
+            raise EngineError(f"{what} failed ({st}): "
+                              f"{self.L.bpf_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.bpf_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- API ---------------------------------------------------------------
+    def set_instances(self, br_on=None, gen_on=None, T=None, ctl=None,
+                      ctl_val=None):
+        if br_on is not None:
+            br_on = np.ascontiguousarray(br_on, np.uint8)
+            T = br_on.shape[0]
+        if gen_on is not None:
+            gen_on = np.ascontiguousarray(gen_on, np.uint8)
+            T = gen_on.shape[0]
+        if T is None:
+            raise ValueError("T unknown")
+        nctl = 0
+        cptr = vptr = None
+        if ctl is not None:
+            nctl = len(ctl)
+            carr = (BpfCtl * nctl)(*[BpfCtl(int(k), int(i)) for k, i in ctl])
+            self._ctl = carr
+            cptr = carr
+            vals = np.ascontiguousarray(ctl_val, np.float64).reshape(T, nctl)
+            self._ctlv = vals
+            vptr = _ptr(vals, _pd)
+        self._check(self.L.bpf_set_instances(
+            self.h, int(T), _ptr(br_on, _pu8), _ptr(gen_on, _pu8), nctl,
+            cptr, vptr), "bpf_set_instances")
+        self.T = int(T)
+
+    def solve(self):
+        self._check(self.L.bpf_solve(self.h), "bpf_solve")
+
+    def results(self):
+        T, n = self.T, self.case.nbus
+        V = np.zeros((T, n, 2))
+        its = np.zeros(T, np.int32)
+        ver = np.zeros(T, np.int32)
+        nrm = np.zeros(T)
+        loss = np.zeros(T)
+        rnd = np.zeros(T, np.int32)
+        bt = np.zeros((T, n), np.int32)
+        self._check(self.L.bpf_get_results(
+            self.h, _ptr(V, _pd), _ptr(its, _pi), _ptr(ver, _pi),
+            _ptr(nrm, _pd), _ptr(loss, _pd), _ptr(rnd, _pi), _ptr(bt, _pi)),
+            "bpf_get_results")
+        return dict(V=V[..., 0] + 1j * V[..., 1], iterations=its, verdict=ver,
+                    norm=nrm, loss=loss, qlim_rounds=rnd, bus_type=bt)
+
+    def branch_flows(self):
+        T, m = self.T, self.case.nbranch
+        Sf = np.zeros((T, m, 2))
+        St = np.zeros((T, m, 2))
+        self._check(self.L.bpf_get_branch_flows(self.h, _ptr(Sf, _pd),
+                                                _ptr(St, _pd)),
+                    "bpf_get_branch_flows")
+        return Sf[..., 0] + 1j * Sf[..., 1], St[..., 0] + 1j * St[..., 1]
+
+    def info(self):
+        i = BpfInfo()
+        self._check(self.L.bpf_get_info(self.h, C.byref(i)), "bpf_get_info")
+        return {f: getattr(i, f) for f, _ in BpfInfo._fields_}
+
+    def ybus(self, t):
+        import scipy.sparse as sp
+        n = self.case.nbus
+        inf = self.info()
+        rp = np.zeros(n + 1, np.int32)
+        ci = np.zeros(inf["nnz_ybus"], np.int32)
+        v = np.zeros(2 * inf["nnz_ybus"])
+        self._check(self.L.bpf_get_ybus(self.h, int(t), _ptr(rp, _pi),
+                                        _ptr(ci, _pi), _ptr(v, _pd)),
+                    "bpf_get_ybus")
+        return sp.csr_matrix((v[0::2] + 1j * v[1::2], ci, rp), shape=(n, n))
+
+    @staticmethod
+    def _vin(V):
+        V = np.asarray(V)
+        return np.ascontiguousarray(np.stack([V.real, V.imag], -1),
+                                    np.float64)
+
+    def stage_mismatch(self, V):
+        T, n = self.T, self.case.nbus
+        Vd = self._vin(V)
+        F = np.zeros((T, 2 * n))
+        nrm = np.zeros(T)
+        self._check(self.L.bpf_stage_mismatch(self.h, _ptr(Vd, _pd),
+                                              _ptr(F, _pd), _ptr(nrm, _pd)),
+                    "bpf_stage_mismatch")
+        return F, nrm
+
+    def stage_jacobian(self, V, t):
+        import scipy.sparse as sp
+        Vd = self._vin(V)
+        cap = 4 * self.info()["nnz_ybus"]
+        r = np.zeros(cap, np.int32)
+        c = np.zeros(cap, np.int32)
+        v = np.zeros(cap)
+        nnz = C.c_int32()
+        self._check(self.L.bpf_stage_jacobian(
+            self.h, _ptr(Vd, _pd), int(t), cap, _ptr(r, _pi), _ptr(c, _pi),
+            _ptr(v, _pd), C.byref(nnz)), "bpf_stage_jacobian")
+        k = nnz.value
+        nJ = int(max(r[:k].max(), c[:k].max())) + 1 if k else 0
+        return sp.coo_matrix((v[:k], (r[:k], c[:k])), shape=(nJ, nJ)).tocsr()
+
+    def stage_newton_step(self, V):
+        T, n = self.T, self.case.nbus
+        Vd = self._vin(V)
+        dx = np.zeros((T, 2 * n))
+        self._check(self.L.bpf_stage_newton_step(self.h, _ptr(Vd, _pd),
+                                                 _ptr(dx, _pd)),
+                    "bpf_stage_newton_step")
+        return dx
+
+    def adjoint(self, rate=None, w=0.0):
+        T, n, m = self.T, self.case.nbus, self.case.nbranch
+        gP = np.zeros((T, n))
+        gQ = np.zeros((T, n))
+        gx = np.zeros((T, m))
+        obj = np.zeros(T)
+        r = None if rate is None else np.ascontiguousarray(rate, np.float64)
+        self._check(self.L.bpf_adjoint(self.h, _ptr(r, _pd), float(w),
+                                       _ptr(gP, _pd), _ptr(gQ, _pd),
+                                       _ptr(gx, _pd), _ptr(obj, _pd)),
+                    "bpf_adjoint")
+        return dict(grad_P=gP, grad_Q=gQ, grad_x=gx, objective=obj)
+
+    def set_timing(self, on: bool):
+        self._check(self.L.bpf_set_timing(self.h, int(on)), "bpf_set_timing")
+
+    def kernel_time(self, name: str):
+        n = C.c_int32()
+        ms = C.c_double()
+        self._check(self.L.bpf_get_kernel_time(self.h, name.encode(),
+                                               C.byref(n), C.byref(ms)),
+                    "bpf_get_kernel_time")
+        return n.value, ms.value
+
+    def set_launch(self, lu=0, solve=0, mismatch=0):
+        self._check(self.L.bpf_set_launch(self.h, lu, solve, mismatch),
+                    "bpf_set_launch")

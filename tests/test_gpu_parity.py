@@ -1,0 +1,179 @@
+"""Parity of the HIP engine (through the C-ABI) against the oracle and the
+golden vectors.  Bars: Ybus/mismatch/Jacobian to FP64 rounding; Newton
+solutions with identical iteration counts and verdicts and |dV| <= 1e-8 pu
+(FP64 and mixed precision)."""
+import numpy as np
+import pytest
+
+import oracle
+from mapol_amd import synth
+from mapol_amd.engine import (BPF_FP64, BPF_MIXED, CONVERGED, MAX_ITER,
+                              NONFINITE, SINGULAR, Engine)
+
+pytestmark = pytest.mark.gpu
+
+DV_TOL = 1e-8   # pu, the target bar for converged voltages
+
+
+def _engine(name, T=None, **kw):
+    case, cont = synth.workload(name, T=T)
+    e = Engine(case, **kw)
+    e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+    return case, cont, e
+
+
+def test_ybus_matches_oracle():
+    case, cont, e = _engine("pf118x200")
+    e.solve()
+    orc = oracle.Oracle(case)
+    for t in (0, 17, 199):
+        Yg = e.ybus(t)
+        Yo = orc.ybus(br_on=cont.br_on[t], gen_on=cont.gen_on[t])
+        np.testing.assert_array_equal(Yg.indptr, Yo.indptr)
+        np.testing.assert_array_equal(Yg.indices, Yo.indices)
+        scale = np.abs(Yo.data).max()
+        assert np.abs(Yg.data - Yo.data).max() <= 1e-14 * scale
+    e.close()
+
+
+def test_mismatch_and_jacobian_match_oracle():
+    case, cont, e = _engine("pf118x200", T=8)
+    e.solve()
+    orc = oracle.Oracle(case)
+    rng = np.random.default_rng(5)
+    n = case.nbus
+    V = (1 + 0.03 * rng.normal(size=(8, n))) * np.exp(1j * 0.2 * rng.normal(size=(8, n)))
+    F, nrm = e.stage_mismatch(V)
+    for t in range(8):
+        kw = dict(br_on=cont.br_on[t], gen_on=cont.gen_on[t])
+        Fo, no = orc.mismatch(V[t], **kw)
+        np.testing.assert_allclose(F[t, :len(Fo)], Fo, rtol=0, atol=1e-11)
+        assert nrm[t] == pytest.approx(no, rel=1e-12, abs=1e-12)
+    for t in (0, 5):
+        Jg = e.stage_jacobian(V, t)
+        Jo = orc.jacobian(V[t], br_on=cont.br_on[t], gen_on=cont.gen_on[t])
+        assert Jg.shape == Jo.shape
+        d = (Jg - Jo).tocsr()
+        assert abs(d).max() <= 1e-12 * abs(Jo).max()
+        assert Jg.nnz == Jo.nnz
+    e.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(BPF_FP64, 1e-11), (BPF_MIXED, 1e-9)])
+def test_newton_step_matches_sparse_solve(precision, tol):
+    import scipy.sparse.linalg as sla
+    case, cont, e = _engine("pf1354x200", T=4, precision=precision,
+                            refine_steps=2)
+    e.solve()
+    orc = oracle.Oracle(case)
+    rng = np.random.default_rng(7)
+    n = case.nbus
+    V = case.Vm * np.exp(1j * case.Va) * (1 + 0.01 * rng.normal(size=(4, n)))
+    dx = e.stage_newton_step(V)
+    for t in range(4):
+        kw = dict(br_on=cont.br_on[t], gen_on=cont.gen_on[t])
+        J = orc.jacobian(V[t], **kw)
+        F, _ = orc.mismatch(V[t], **kw)
+        ref = sla.spsolve(J.tocsc(), F)
+        got = dx[t, :len(ref)]
+        assert np.abs(got - ref).max() <= tol * max(1.0, np.abs(ref).max())
+    e.close()
+
+
+@pytest.mark.parametrize("precision", [BPF_FP64, BPF_MIXED])
+def test_nr_matches_golden_118x200(golden, precision):
+    case, cont, e = _engine("pf118x200", precision=precision)
+    e.solve()
+    r = e.results()
+    np.testing.assert_array_equal(r["iterations"], golden["pf118x200/nr/its"])
+    np.testing.assert_array_equal(r["verdict"], golden["pf118x200/nr/verdict"])
+    dv = np.abs(r["V"] - golden["pf118x200/nr/V"]).max()
+    assert dv <= DV_TOL, dv
+    np.testing.assert_allclose(r["loss"], golden["pf118x200/nr/loss"],
+                               rtol=1e-9, atol=1e-10)
+    e.close()
+
+
+@pytest.mark.parametrize("precision", [BPF_FP64, BPF_MIXED])
+def test_nr_matches_golden_14(golden, precision):
+    case, cont, e = _engine("pf14", precision=precision)
+    e.solve()
+    r = e.results()
+    assert r["iterations"][0] == golden["pf14/nr/its"][0]
+    assert r["verdict"][0] == golden["pf14/nr/verdict"][0]
+    assert np.abs(r["V"][0] - golden["pf14/nr/V"][0]).max() <= DV_TOL
+    e.close()
+
+
+@pytest.mark.parametrize("tag,qlim", [("nr", False), ("nrq", True)])
+@pytest.mark.parametrize("precision", [BPF_FP64, BPF_MIXED])
+def test_nr_matches_golden_1354(golden, tag, qlim, precision):
+    idx = golden[f"pf1354x200/{tag}/idx"]
+    case, cont = synth.workload("pf1354x200")
+    e = Engine(case, precision=precision, enforce_qlim=qlim)
+    e.set_instances(br_on=cont.br_on[idx], gen_on=cont.gen_on[idx])
+    e.solve()
+    r = e.results()
+    np.testing.assert_array_equal(r["iterations"], golden[f"pf1354x200/{tag}/its"])
+    np.testing.assert_array_equal(r["verdict"], golden[f"pf1354x200/{tag}/verdict"])
+    np.testing.assert_array_equal(r["qlim_rounds"], golden[f"pf1354x200/{tag}/rounds"])
+    ok = r["verdict"] == CONVERGED
+    assert np.abs(r["V"][ok] - golden[f"pf1354x200/{tag}/V"][ok]).max() <= DV_TOL
+    e.close()
+
+
+def test_failure_verdicts_match_oracle():
+    case, cont = synth.workload("pf118x200")
+    br = synth.bridges(case)
+    T = 3
+    br_on = np.ones((T, case.nbranch), np.uint8)
+    br_on[1, np.flatnonzero(br)[0]] = 0          # islands a bus
+    e = Engine(case, maxit_nr=10)
+    e.set_instances(br_on=br_on)
+    e.solve()
+    r = e.results()
+    orc = oracle.Oracle(case)
+    for t in range(T):
+        ro = orc.solve(br_on=br_on[t])
+        assert r["verdict"][t] == ro["verdict"]
+        assert r["iterations"][t] == ro["iterations"]
+    assert r["verdict"][1] == SINGULAR
+    assert r["loss"][1] == np.finfo(np.float64).max
+    e.close()
+    e = Engine(case, maxit_nr=1)
+    e.set_instances(T=2)
+    e.solve()
+    r = e.results()
+    assert list(r["verdict"]) == [MAX_ITER, MAX_ITER]
+    assert list(r["iterations"]) == [1, 1]
+    e.close()
+    c2, _ = synth.workload("pf118x200")
+    c2.Pd = c2.Pd.copy()
+    c2.Pd[5] = np.nan
+    e = Engine(c2)
+    e.set_instances(T=1)
+    e.solve()
+    assert e.results()["verdict"][0] == NONFINITE
+    e.close()
+
+
+@pytest.mark.parametrize("name,T", [("pf2869x100", 6), ("pf9241x200", 6)])
+def test_nr_matches_oracle_large(name, T):
+    case, cont = synth.workload(name)
+    qlim = synth.WORKLOADS[name]["qlim"]
+    orc = oracle.Oracle(case)
+    for precision in (BPF_FP64, BPF_MIXED):
+        e = Engine(case, precision=precision, enforce_qlim=qlim)
+        e.set_instances(br_on=cont.br_on[:T], gen_on=cont.gen_on[:T])
+        e.solve()
+        r = e.results()
+        for t in range(T):
+            ro = orc.solve(oracle.Oracle.opts(enforce_qlim=qlim),
+                           br_on=cont.br_on[t], gen_on=cont.gen_on[t])
+            assert r["verdict"][t] == ro["verdict"], t
+            assert r["iterations"][t] == ro["iterations"], t
+            assert r["qlim_rounds"][t] == ro["qlim_rounds"], t
+            if ro["converged"]:
+                assert np.abs(r["V"][t] - ro["V"]).max() <= DV_TOL
+                assert r["loss"][t] == pytest.approx(ro["loss"], rel=1e-8)
+        e.close()
