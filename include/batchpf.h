@@ -188,7 +188,8 @@ bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
                        double *objective);
 
 /* Kernel timing on the engine's stream with hipEvents (enable before
- * bpf_solve; graph replay is turned off while enabled).  Names: ybus,
+ * bpf_solve).  With graph replay the events are event-record nodes of the
+ * Newton-iteration graph, read after every replay.  Names: ybus, init,
  * mismatch, jacobian, lu_factor, lu_solve, update, loss, fd_solve,
  * adjoint. */
 bpf_status bpf_set_timing(bpf_engine *e, int32_t enable);

@@ -70,6 +70,9 @@ struct bpf_engine {
   // timing
   int timing = 0;
   std::vector<TimedEvent> events;
+  bool capturing = false;            // recording the iteration graph
+  int cap_slot = 0;
+  std::vector<TimedEvent> fixed_ev;  // event nodes of the timed graph
   std::map<std::string, std::pair<int, double>> ktime;
   bool solved = false;
 };
@@ -142,6 +145,7 @@ extern "C" void bpf_destroy(bpf_engine *e) {
   if (!e) return;
   if (e->graph) hipGraphExecDestroy(e->graph);
   for (auto &ev : e->events) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
+  for (auto &ev : e->fixed_ev) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
   free_pool(e->inst_owned);
   free_pool(e->owned);
   if (e->h_count) hipHostFree(e->h_count);
@@ -226,6 +230,7 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(g.y2lu, S.y2lu); UP(g.lrow, S.lrow); UP(g.lcol, S.lcol);
   UP(g.ldiag, S.ldiag); UP(g.mirror, S.mirror);
   UP(g.br_from, e->br_from); UP(g.br_to, e->br_to);
+  UP(g.bbr_ptr, S.bbr_ptr); UP(g.bbr, S.bbr);
   UP(g.R, e->R); UP(g.X, e->X); UP(g.B, e->B); UP(g.shift, e->shift);
   UP(g.Gs, e->Gs); UP(g.Qd, e->Qd); UP(g.Va0, e->Va);
   UP(g.base_type, e->bus_type);
@@ -388,6 +393,19 @@ static bpf_status ensure_lu(bpf_engine *e, int fp64) {
 // ---- launches ---------------------------------------------------------------
 static void tbegin(bpf_engine *e, const char *name) {
   if (!e->timing) return;
+  if (e->capturing) {
+    // fixed events become event-record nodes of the graph; their elapsed
+    // times are read after every replay
+    if (e->cap_slot >= (int)e->fixed_ev.size()) {
+      TimedEvent t;
+      t.name = name;
+      hipEventCreate(&t.a);
+      hipEventCreate(&t.b);
+      e->fixed_ev.push_back(t);
+    }
+    hipEventRecord(e->fixed_ev[e->cap_slot++].a, e->stream);
+    return;
+  }
   TimedEvent t;
   t.name = name;
   hipEventCreate(&t.a);
@@ -397,7 +415,20 @@ static void tbegin(bpf_engine *e, const char *name) {
 }
 static void tend(bpf_engine *e) {
   if (!e->timing) return;
+  if (e->capturing) {
+    hipEventRecord(e->fixed_ev[e->cap_slot - 1].b, e->stream);
+    return;
+  }
   hipEventRecord(e->events.back().b, e->stream);
+}
+static void tcollect_fixed(bpf_engine *e) {
+  for (auto &ev : e->fixed_ev) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ev.a, ev.b);
+    auto &slot = e->ktime[ev.name];
+    slot.first++;
+    slot.second += ms;
+  }
 }
 static void tcollect(bpf_engine *e) {
   for (auto &ev : e->events) {
@@ -416,10 +447,12 @@ static unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b
 
 static void launch_mismatch(bpf_engine *e, int mode) {
   tbegin(e, "mismatch");
+  const bool fd = e->opt.alg != BPF_NR;
   hipLaunchKernelGGL(k_mismatch, dim3(e->T), dim3(e->mis_threads), 0,
-                     e->stream, e->g, e->d, mode, e->opt.maxit_nr, e->opt.tol,
-                     e->opt.enforce_qlim, e->opt.max_qlim_rounds,
-                     e->opt.qlim_tol, 0);
+                     e->stream, e->g, e->d, mode,
+                     fd ? e->opt.maxit_fd : e->opt.maxit_nr, e->opt.tol,
+                     fd ? 0 : e->opt.enforce_qlim, e->opt.max_qlim_rounds,
+                     e->opt.qlim_tol, fd ? 1 : 0);
   tend(e);
 }
 
@@ -432,7 +465,7 @@ static void launch_newton_step(bpf_engine *e) {
   tend(e);
   tbegin(e, "lu_factor");
   hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, e->S.nlev, e->d_lvA,
+                     e->stream, e->g, e->d.st, 1, e->S.nlev, e->d_lvA,
                      e->d_tA_dst, e->d_tA_pp, e->d_pairL, e->d_pairU,
                      e->d_lvB, e->d_tB_dst, e->d_tB_diag, LU);
   tend(e);
@@ -442,7 +475,7 @@ static void launch_newton_step(bpf_engine *e) {
   tbegin(e, "lu_solve");
   hipLaunchKernelGGL(k_lu_solve<T>, dim3(e->T), dim3(e->solve_threads),
                      use_lds ? xs : 0, e->stream, e->g, e->d,
-                     (const B4<T> *)LU, refine, e->S.nfw, e->d_fw_ptr,
+                     (const B4<T> *)LU, refine, 0, e->S.nfw, e->d_fw_ptr,
                      e->d_fw_rows, e->S.nbw, e->d_bw_ptr, e->d_bw_rows,
                      use_lds, (T *)e->d_xglob);
   tend(e);
@@ -474,13 +507,20 @@ static bpf_status run_newton(bpf_engine *e) {
   bpf_status s = ensure_lu(e, e->opt.precision == BPF_FP64 ? 1 : 0);
   if (s) return s;
   set_lds_limits(e);
-  const int key = e->lu_fp64 * 1000000 + e->lu_threads * 1000 + e->solve_threads;
-  bool use_graph = e->opt.use_graph && !e->timing;
+  const int key = e->timing * 100000000 + e->lu_fp64 * 10000000 +
+                  e->lu_threads * 10000 + e->mis_threads * 10 / 64 +
+                  e->solve_threads / 64;
+  bool use_graph = e->opt.use_graph != 0;
   if (use_graph && (!e->graph || e->graph_key != key)) {
     if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
+    for (auto &ev : e->fixed_ev) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
+    e->fixed_ev.clear();
     hipGraph_t gr;
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    e->capturing = true;
+    e->cap_slot = 0;
     launch_iteration(e);
+    e->capturing = false;
     HIPCHK(hipStreamEndCapture(e->stream, &gr));
     hipError_t r = hipGraphInstantiate(&e->graph, gr, nullptr, nullptr, 0);
     hipGraphDestroy(gr);
@@ -496,7 +536,59 @@ static bpf_status run_newton(bpf_engine *e) {
     else launch_iteration(e);
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipGetLastError());
+    if (use_graph && e->timing) tcollect_fixed(e);
     if (*e->h_count == 0) break;
+  }
+  return BPF_OK;
+}
+
+// fast decoupled (mkl_fdpf, fdpf.h:450-596): B' and B'' of every instance
+// assembled and factored once in FP64, then P and Q half-steps; each half-step
+// is a masked solve with the same factors, an update and the |V|-normalised
+// mismatch with the reference's convergence checks
+static bpf_status run_fd(bpf_engine *e) {
+  bpf_status s = ensure_lu(e, 1);
+  if (s) return s;
+  set_lds_limits(e);
+  B4<double> *LU = (B4<double> *)e->d_LU;
+  const unsigned T = (unsigned)e->T;
+  tbegin(e, "bmat");
+  hipLaunchKernelGGL(k_bmat, dim3(div_up(e->g.nnzY, 256), T), dim3(256), 0,
+                     e->stream, e->g, e->d, e->opt.alg, LU);
+  tend(e);
+  tbegin(e, "lu_factor");
+  hipLaunchKernelGGL(k_lu_factor<double>, dim3(T), dim3(e->lu_threads), 0,
+                     e->stream, e->g, e->d.st, 0, e->S.nlev, e->d_lvA,
+                     e->d_tA_dst, e->d_tA_pp, e->d_pairL, e->d_pairU,
+                     e->d_lvB, e->d_tB_dst, e->d_tB_diag, LU);
+  tend(e);
+  size_t xs = sizeof(double) * 2 * (size_t)e->S.nb;
+  int use_lds = (int64_t)xs <= e->max_lds;
+  for (int it = 0; it < e->opt.maxit_fd + 1; it++) {
+    for (int half = 1; half <= 2; half++) {
+      tbegin(e, "fd_solve");
+      hipLaunchKernelGGL(k_lu_solve<double>, dim3(T), dim3(e->solve_threads),
+                         use_lds ? xs : 0, e->stream, e->g, e->d,
+                         (const B4<double> *)LU, 0, half, e->S.nfw,
+                         e->d_fw_ptr, e->d_fw_rows, e->S.nbw, e->d_bw_ptr,
+                         e->d_bw_rows, use_lds, (double *)e->d_xglob);
+      tend(e);
+      tbegin(e, "update");
+      hipLaunchKernelGGL(k_fd_update, dim3(div_up(e->S.nb, 256), T), dim3(256),
+                         0, e->stream, e->g, e->d, half);
+      tend(e);
+      tbegin(e, "mismatch");
+      hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, e->stream, e->d);
+      hipLaunchKernelGGL(k_mismatch, dim3(T), dim3(e->mis_threads), 0,
+                         e->stream, e->g, e->d, half == 1 ? 3 : 4,
+                         e->opt.maxit_fd, e->opt.tol, 0, 0, 0.0, 1);
+      tend(e);
+      HIPCHK(hipMemcpyAsync(e->h_count, e->d.active_count, sizeof(int),
+                            hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      HIPCHK(hipGetLastError());
+      if (*e->h_count == 0) return BPF_OK;
+    }
   }
   return BPF_OK;
 }
@@ -504,7 +596,11 @@ static bpf_status run_newton(bpf_engine *e) {
 extern "C" bpf_status bpf_solve(bpf_engine *e) {
   if (!e) return BPF_ERR_INVALID_ARG;
   if (e->T <= 0) { e->err = "bpf_set_instances not called"; return BPF_ERR_STATE; }
-  if (e->opt.alg != BPF_NR) { e->err = "fast decoupled: see bpf_solve_fd"; }
+  if (e->opt.alg != BPF_NR && e->opt.enforce_qlim) {
+    e->err = "reactive-limit switching is supported with Newton-Raphson only";
+    return BPF_ERR_UNSUPPORTED;
+  }
+  const bool fd = e->opt.alg != BPF_NR;
   const unsigned T = (unsigned)e->T;
   tbegin(e, "ybus");
   hipLaunchKernelGGL(k_ybus, dim3(div_up(e->g.nnzY, 256), T), dim3(256), 0,
@@ -521,7 +617,7 @@ extern "C" bpf_status bpf_solve(bpf_engine *e) {
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
   if (*e->h_count > 0) {
-    bpf_status s = run_newton(e);
+    bpf_status s = fd ? run_fd(e) : run_newton(e);
     if (s) return s;
   }
   tbegin(e, "loss");
@@ -754,8 +850,80 @@ extern "C" bpf_status bpf_stage_newton_step(bpf_engine *e, const double *V,
 extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
                                   double *gP, double *gQ, double *gx,
                                   double *obj) {
-  (void)rate; (void)w; (void)gP; (void)gQ; (void)gx; (void)obj;
   if (!e) return BPF_ERR_INVALID_ARG;
-  e->err = "adjoint not built yet";
-  return BPF_ERR_UNSUPPORTED;
+  if (!e->solved) { e->err = "call bpf_solve first"; return BPF_ERR_STATE; }
+  if (e->opt.alg != BPF_NR) {
+    e->err = "the adjoint uses the Newton Jacobian: solve with BPF_NR";
+    return BPF_ERR_UNSUPPORTED;
+  }
+  bpf_status s = ensure_lu(e, 1);
+  if (s) return s;
+  set_lds_limits(e);
+  const size_t T = e->T, n = e->n, m = e->m, nb2 = 2 * (size_t)e->S.nb;
+  std::vector<DBuf> tmp;
+  void *p;
+  double *d_rate = nullptr, *d_G, *d_lam, *d_gP, *d_gQ, *d_gx, *d_obj;
+  auto cleanup = [&]() { free_pool(tmp); };
+#define TAL(dst, count)                                                    \
+  if ((s = dalloc(e, tmp, sizeof(double) * (count), &p))) { cleanup(); return s; } \
+  dst = (double *)p;
+  if (rate) {
+    TAL(d_rate, m);
+    hipMemcpy(d_rate, rate, sizeof(double) * m, hipMemcpyHostToDevice);
+  }
+  TAL(d_G, T * nb2);
+  TAL(d_lam, T * nb2);
+  TAL(d_gP, T * n);
+  TAL(d_gQ, T * n);
+  TAL(d_gx, T * m);
+  TAL(d_obj, T);
+#undef TAL
+  // only converged instances take part
+  std::vector<InstState> st(T), st2;
+  HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * T, hipMemcpyDeviceToHost));
+  st2 = st;
+  for (auto &x : st2) x.done = x.verdict == BPF_V_CONVERGED ? 0 : 1;
+  HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * T, hipMemcpyHostToDevice));
+  B4<double> *LU = (B4<double> *)e->d_LU;
+  const unsigned TT = (unsigned)T;
+  tbegin(e, "adjoint");
+  hipLaunchKernelGGL(k_jacobian<double>, dim3(div_up(e->g.nnzY, 256), TT),
+                     dim3(256), 0, e->stream, e->g, e->d, LU);
+  hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
+                     e->stream, e->g, e->d.st, 0, e->S.nlev, e->d_lvA,
+                     e->d_tA_dst, e->d_tA_pp, e->d_pairL, e->d_pairU,
+                     e->d_lvB, e->d_tB_dst, e->d_tB_diag, LU);
+  hipLaunchKernelGGL(k_adj_rhs, dim3(div_up(e->S.nb, 256), TT), dim3(256), 0,
+                     e->stream, e->g, e->d, (const double *)d_rate, w, d_G);
+  size_t xs = sizeof(double) * nb2;
+  int use_lds = (int64_t)xs <= e->max_lds;
+  hipLaunchKernelGGL(k_adj_solve, dim3(TT), dim3(e->solve_threads),
+                     use_lds ? xs : 0, e->stream, e->g, e->d,
+                     (const B4<double> *)LU, (const double *)d_G, d_lam,
+                     e->S.nfw, e->d_fw_ptr, e->d_fw_rows, e->S.nbw,
+                     e->d_bw_ptr, e->d_bw_rows, use_lds, (double *)e->d_xglob);
+  hipLaunchKernelGGL(k_adj_grad, dim3(div_up(std::max(n, m), 256), TT),
+                     dim3(256), 0, e->stream, e->g, e->d,
+                     (const double *)d_lam, (const double *)d_rate, w, d_gP,
+                     d_gQ, d_gx);
+  hipLaunchKernelGGL(k_objective, dim3(TT), dim3(256), 0, e->stream, e->g,
+                     e->d, (const double *)d_rate, w, d_obj);
+  tend(e);
+  hipError_t r1 = hipStreamSynchronize(e->stream);
+  hipError_t r2 = hipGetLastError();
+  tcollect(e);
+  // factorization singularities are not expected at a converged point; keep
+  // the verdicts of the solve either way
+  hipMemcpy(e->d.st, st.data(), sizeof(InstState) * T, hipMemcpyHostToDevice);
+  if (r1 != hipSuccess || r2 != hipSuccess) {
+    e->err = std::string("adjoint: ") + hipGetErrorString(r1 != hipSuccess ? r1 : r2);
+    cleanup();
+    return BPF_ERR_HIP;
+  }
+  if (gP) hipMemcpy(gP, d_gP, sizeof(double) * T * n, hipMemcpyDeviceToHost);
+  if (gQ) hipMemcpy(gQ, d_gQ, sizeof(double) * T * n, hipMemcpyDeviceToHost);
+  if (gx) hipMemcpy(gx, d_gx, sizeof(double) * T * m, hipMemcpyDeviceToHost);
+  if (obj) hipMemcpy(obj, d_obj, sizeof(double) * T, hipMemcpyDeviceToHost);
+  cleanup();
+  return BPF_OK;
 }

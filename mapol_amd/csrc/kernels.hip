@@ -176,7 +176,10 @@ __device__ int block_add(int v, int *sh) {
 // Mismatch S = V o conj(Ybus V) - Sbus, inf-norm, verdicts and reactive-
 // limit switching; one workgroup per instance (newtonpf.h:18-95,394-396,
 // 518-520).  F is kept in block order: F[2k] = P, F[2k+1] = Q (0 for PV).
-// mode: 0 initial check, 1 after a Newton step, 2 evaluation only.
+// mode: 0 initial check, 1 after a Newton step, 2 evaluation only,
+// 3 fast decoupled after the P half-step (counts the iteration; converged or
+// go on), 4 fast decoupled after the Q half-step (converged, iteration cap or
+// go on) -- the loop of fdpf.h:505-590.  fd_norm divides by |V| (fdpf.h:401).
 // ---------------------------------------------------------------------------
 __global__ void k_mismatch(DevTopo g, DevInst d, int mode, int maxit,
                            double tol, int enforce_qlim, int max_rounds,
@@ -194,7 +197,7 @@ __global__ void k_mismatch(DevTopo g, DevInst d, int mode, int maxit,
   double *Pinj = d.Pinj + (size_t)t * g.n;
   double *Qinj = d.Qinj + (size_t)t * g.n;
   double *F = d.F + (size_t)t * 2 * g.nb;
-  if (mode == 1) { s.it_round++; s.it_total++; }
+  if (mode == 1 || mode == 3) { s.it_round++; s.it_total++; }
   bool active = false;
   for (int guard = 0; guard <= max_rounds + 1; guard++) {
     double lmax = 0.0;
@@ -245,7 +248,7 @@ __global__ void k_mismatch(DevTopo g, DevInst d, int mode, int maxit,
       __syncthreads();
       continue;  // re-evaluate the mismatch with the new bus types
     }
-    if (s.it_round >= maxit) { s.verdict = BPF_V_MAX_ITER; s.done = 1; break; }
+    if (mode != 3 && s.it_round >= maxit) { s.verdict = BPF_V_MAX_ITER; s.done = 1; break; }
     active = true;
     break;
   }
@@ -322,7 +325,7 @@ __device__ __forceinline__ B4<T> blk_mul(const B4<T> &x, const B4<T> &y) {
 // inverted); phase B scales the level's L blocks by the pivot inverse.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_lu_factor(DevTopo g, InstState *st, int nlev,
+__global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter, int nlev,
                             const int *lvA, const int *tA_dst, const int *tA_pp,
                             const int *pairL, const int *pairU, const int *lvB,
                             const int *tB_dst, const int *tB_diag, B4<T> *LU) {
@@ -363,8 +366,8 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int nlev,
     InstState s = st[t];
     s.done = 1;
     s.verdict = BPF_V_SINGULAR;
-    s.it_round++;
-    s.it_total++;
+    s.it_round += count_iter;
+    s.it_total += count_iter;
     st[t] = s;
   }
 }
@@ -421,7 +424,7 @@ __device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, T *X, int nfw,
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, int refine,
-                           int nfw, const int *fw_ptr, const int *fw_rows,
+                           int rhs_mode, int nfw, const int *fw_ptr, const int *fw_rows,
                            int nbw, const int *bw_ptr, const int *bw_rows,
                            int use_lds, T *xglob) {
   extern __shared__ unsigned char smem[];
@@ -437,7 +440,9 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, int refine,
   const int8_t *ty = d.type + (size_t)t * g.n;
   for (int s = 0; s <= refine; s++) {
     if (s == 0) {
-      for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) X[k] = (T)F[k];
+      // rhs_mode 0: (P, Q); 1: (P, 0) and 2: (0, Q) for the decoupled steps
+      for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
+        X[k] = (rhs_mode == 0 || (rhs_mode == 1) == ((k & 1) == 0)) ? (T)F[k] : (T)0;
     } else {
       for (int k = threadIdx.x; k < g.nb; k += blockDim.x) {
         int i = g.bus_of_blk[k];
@@ -485,6 +490,278 @@ __global__ void k_update(DevTopo g, DevInst d) {
   d.V[o] = cmul(mk(mag, 0.0), e);
 }
 
+__global__ void k_zero_count(DevInst d) { *d.active_count = 0; }
+
+// ---------------------------------------------------------------------------
+// Fast decoupled: half-step updates (fdpf.h:416-448).  part 1: angles of all
+// unknown buses, part 2: magnitudes of PQ buses.
+// ---------------------------------------------------------------------------
+__global__ void k_fd_update(DevTopo g, DevInst d, int part) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.y;
+  if (k >= g.nb) return;
+  if (d.st[t].done) return;
+  int i = g.bus_of_blk[k];
+  size_t o = (size_t)t * g.n + i;
+  const double *dx = d.dx + (size_t)t * 2 * g.nb;
+  cd v = d.V[o];
+  if (part == 1) {
+    d.V[o] = cmul(mk(cabs_(v), 0.0), cexpj(atan(v.y / v.x) - dx[2 * k]));
+  } else if (d.type[o] == 1) {
+    d.V[o] = cmul(mk(cabs_(v) - dx[2 * k + 1], 0.0), cexpj(atan(v.y / v.x)));
+  }
+}
+
+// one Ybus-style entry of the modified networks behind B' and B''
+// (makeB.h:311-501): B' zeroes line charging, taps and bus shunts (and R for
+// XB); B'' zeroes phase shifts (and R for BX)
+__device__ cd branch_entry_fd(const DevTopo &g, const DevInst &d, int t, int b,
+                              int kind, int which, int zeroR) {
+  double stat = d.br_on[(size_t)t * g.m + b] ? 1.0 : 0.0;
+  double R = zeroR ? 0.0 : g.R[b];
+  cd Ys = cdiv(mk(stat, 0.0), mk(R, g.X[b]));
+  double Bch = (which == 1) ? 0.0 : g.B[b];
+  double tp = (which == 1) ? 0.0 : d.tap[(size_t)t * g.m + b];
+  double sh = (which == 1) ? g.shift[b] : 0.0;
+  cd tap = (tp != 0.0) ? mk(tp, 0.0) : mk(1.0, 0.0);
+  tap = cmul(tap, cexpj(M_PI / 180.0 * sh));
+  cd Bc = cmul(mk(stat, 0.0), mk(0.0, Bch));
+  switch (kind) {
+    case KFF: return cdiv(cadd(Ys, cdiv(Bc, mk(2.0, 0.0))), cmul(tap, cconj(tap)));
+    case KFT: return cdiv(cmul(Ys, mk(-1.0, 0.0)), cconj(tap));
+    case KTF: return cdiv(cmul(Ys, mk(-1.0, 0.0)), tap);
+    default: return cadd(Ys, cdiv(Bc, mk(2.0, 0.0)));
+  }
+}
+
+// B' and B'' of every instance as one block-diagonal 2x2 system in the L+U
+// storage: block (i,j) = [B'_ij 0; 0 B''_ij], with an identity Q row for PV
+// buses; B = -Im(Ybus of the modified network).  alg: 1 XB, 2 BX.
+__global__ void k_bmat(DevTopo g, DevInst d, int alg, B4<double> *LU) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.y;
+  if (p >= g.nnzY) return;
+  int q = g.y2lu[p];
+  if (q < 0) return;
+  double v[2];
+  for (int which = 1; which <= 2; which++) {
+    int zeroR = (which == 1 && alg == 1) || (which == 2 && alg == 2);
+    cd af = mk(0.0, 0.0), at = mk(0.0, 0.0);
+    for (int r = g.yf_ptr[p]; r < g.yf_ptr[p + 1]; r++) {
+      int it = g.yf_item[r];
+      af = cadd(af, branch_entry_fd(g, d, t, it >> 2, it & 3, which, zeroR));
+    }
+    for (int r = g.yt_ptr[p]; r < g.yt_ptr[p + 1]; r++) {
+      int it = g.yt_item[r];
+      at = cadd(at, branch_entry_fd(g, d, t, it >> 2, it & 3, which, zeroR));
+    }
+    cd y = cadd(af, at);
+    int i = g.ydiag_bus[p];
+    if (i >= 0) {
+      double bs = (which == 1) ? 0.0 : d.Bs[(size_t)t * g.n + i];
+      y = cadd(y, mk(g.Gs[i], bs));
+    }
+    v[which - 1] = -y.y;
+  }
+  int ii = g.yrowof[p], jj = g.ycol[p];
+  int ti = d.type[(size_t)t * g.n + ii], tj = d.type[(size_t)t * g.n + jj];
+  B4<double> blk;
+  blk.a = v[0];
+  blk.b = 0.0;
+  blk.c = 0.0;
+  blk.d = (ti == 2 || tj == 2) ? ((ii == jj) ? 1.0 : 0.0) : v[1];
+  LU[(size_t)t * g.nnzLU + q] = blk;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint sensitivities (no counterpart in the reference).  Objective per
+// instance: f = sum_b Re(Sf_b + St_b) + w sum_ends max(0, |S| - rate_b)^2,
+// Sf = Vf conj(If), St = Vt conj(It) (the loss of runpf.h:228-257 plus a
+// branch-flow penalty).  With F(x, p) = 0 at the solution and
+// J^T lambda = df/dx:  df/dp = df/dp|explicit - lambda^T dF/dp.
+// ---------------------------------------------------------------------------
+struct BranchState {
+  cd Vf, Vt, Sf, St, yff, yft, ytf, ytt, af, at;  // a = d f / d S (complex)
+};
+
+__device__ BranchState branch_state(const DevTopo &g, const DevInst &d, int t,
+                                    int b, const double *rate, double w) {
+  BranchState s;
+  const cd *V = d.V + (size_t)t * g.n;
+  int f = g.br_from[b], to = g.br_to[b];
+  s.Vf = V[f];
+  s.Vt = V[to];
+  s.yff = branch_entry(g, d, t, b, KFF);
+  s.yft = branch_entry(g, d, t, b, KFT);
+  s.ytf = branch_entry(g, d, t, b, KTF);
+  s.ytt = branch_entry(g, d, t, b, KTT);
+  cd If = cadd(cmul(s.yff, s.Vf), cmul(s.yft, s.Vt));
+  cd It = cadd(cmul(s.ytf, s.Vf), cmul(s.ytt, s.Vt));
+  s.Sf = cmul(s.Vf, cconj(If));
+  s.St = cmul(s.Vt, cconj(It));
+  // d/dz [Re S + w max(0,|S|-r)^2] = Re(a dS/dz), a = 1 + 2w(|S|-r)/|S| conj(S)
+  double pf = 0.0, pt = 0.0;
+  if (rate) {
+    double r = rate[b], mf = cabs_(s.Sf), mt = cabs_(s.St);
+    if (mf > r) pf = 2.0 * w * (mf - r) / mf;
+    if (mt > r) pt = 2.0 * w * (mt - r) / mt;
+  }
+  s.af = cadd(mk(1.0, 0.0), cmul(mk(pf, 0.0), cconj(s.Sf)));
+  s.at = cadd(mk(1.0, 0.0), cmul(mk(pt, 0.0), cconj(s.St)));
+  return s;
+}
+
+// right-hand side G = df/dx in block order (angle, magnitude of PQ buses)
+__global__ void k_adj_rhs(DevTopo g, DevInst d, const double *rate, double w,
+                          double *G) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.y;
+  if (k >= g.nb) return;
+  if (d.st[t].done) return;
+  int i = g.bus_of_blk[k];
+  double ga = 0.0, gm = 0.0;
+  for (int q = g.bbr_ptr[i]; q < g.bbr_ptr[i + 1]; q++) {
+    int b = g.bbr[q];
+    BranchState s = branch_state(g, d, t, b, rate, w);
+    cd dSf_a, dSt_a, dSf_m, dSt_m;
+    const cd J = mk(0.0, 1.0);
+    if (g.br_from[b] == i) {
+      double vf2 = s.Vf.x * s.Vf.x + s.Vf.y * s.Vf.y, vf = sqrt(vf2);
+      dSf_a = cmul(J, csub(s.Sf, cmul(mk(vf2, 0.0), cconj(s.yff))));
+      dSt_a = cmul(mk(0.0, -1.0), cmul(s.Vt, cconj(cmul(s.ytf, s.Vf))));
+      dSf_m = cdiv(cadd(s.Sf, cmul(mk(vf2, 0.0), cconj(s.yff))), mk(vf, 0.0));
+      dSt_m = cdiv(cmul(s.Vt, cconj(cmul(s.ytf, s.Vf))), mk(vf, 0.0));
+    } else {
+      double vt2 = s.Vt.x * s.Vt.x + s.Vt.y * s.Vt.y, vt = sqrt(vt2);
+      dSt_a = cmul(J, csub(s.St, cmul(mk(vt2, 0.0), cconj(s.ytt))));
+      dSf_a = cmul(mk(0.0, -1.0), cmul(s.Vf, cconj(cmul(s.yft, s.Vt))));
+      dSt_m = cdiv(cadd(s.St, cmul(mk(vt2, 0.0), cconj(s.ytt))), mk(vt, 0.0));
+      dSf_m = cdiv(cmul(s.Vf, cconj(cmul(s.yft, s.Vt))), mk(vt, 0.0));
+    }
+    ga += cmul(s.af, dSf_a).x + cmul(s.at, dSt_a).x;
+    gm += cmul(s.af, dSf_m).x + cmul(s.at, dSt_m).x;
+  }
+  double *Gt = G + (size_t)t * 2 * g.nb;
+  Gt[2 * k] = ga;
+  Gt[2 * k + 1] = (d.type[(size_t)t * g.n + i] == 1) ? gm : 0.0;
+}
+
+// J^T lambda = G with the FP64 factors: U^T z = G (forward levels, U blocks
+// reached through the mirror map), then L^T lambda = z (backward levels).
+// One workgroup per instance.
+__global__ void k_adj_solve(DevTopo g, DevInst d, const B4<double> *LU,
+                            const double *G, double *lam, int nfw,
+                            const int *fw_ptr, const int *fw_rows, int nbw,
+                            const int *bw_ptr, const int *bw_rows, int use_lds,
+                            double *xglob) {
+  extern __shared__ unsigned char smem[];
+  const int t = blockIdx.x;
+  if (d.st[t].done) return;
+  double *X = use_lds ? reinterpret_cast<double *>(smem) : xglob + (size_t)t * 2 * g.nb;
+  const B4<double> *A = LU + (size_t)t * g.nnzLU;
+  const double *Gt = G + (size_t)t * 2 * g.nb;
+  for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) X[k] = Gt[k];
+  __syncthreads();
+  for (int l = 0; l < nfw; l++) {
+    for (int r = fw_ptr[l] + threadIdx.x; r < fw_ptr[l + 1]; r += blockDim.x) {
+      int i = fw_rows[r];
+      double x0 = X[2 * i], x1 = X[2 * i + 1];
+      for (int p = g.lrow[i]; p < g.ldiag[i]; p++) {
+        int c = g.lcol[p];
+        B4<double> U = A[g.mirror[p]];   // block (c, i) of U
+        double y0 = X[2 * c], y1 = X[2 * c + 1];
+        x0 -= U.a * y0 + U.c * y1;
+        x1 -= U.b * y0 + U.d * y1;
+      }
+      B4<double> D = A[g.ldiag[i]];     // inverse pivot, used transposed
+      X[2 * i] = D.a * x0 + D.c * x1;
+      X[2 * i + 1] = D.b * x0 + D.d * x1;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < nbw; l++) {
+    for (int r = bw_ptr[l] + threadIdx.x; r < bw_ptr[l + 1]; r += blockDim.x) {
+      int i = bw_rows[r];
+      double x0 = X[2 * i], x1 = X[2 * i + 1];
+      for (int p = g.ldiag[i] + 1; p < g.lrow[i + 1]; p++) {
+        int c = g.lcol[p];
+        B4<double> L = A[g.mirror[p]];   // block (c, i) of L
+        double y0 = X[2 * c], y1 = X[2 * c + 1];
+        x0 -= L.a * y0 + L.c * y1;
+        x1 -= L.b * y0 + L.d * y1;
+      }
+      X[2 * i] = x0;
+      X[2 * i + 1] = x1;
+    }
+    __syncthreads();
+  }
+  double *lt = lam + (size_t)t * 2 * g.nb;
+  for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) lt[k] = X[k];
+}
+
+// gradients: df/dP_i = lambda_P(i), df/dQ_i = lambda_Q(i) (PQ buses), and
+// df/dx_b = explicit - lambda^T dF/dx_b through dYs/dx = -j Ys^2 (in service)
+__global__ void k_adj_grad(DevTopo g, DevInst d, const double *lam,
+                           const double *rate, double w, double *gP,
+                           double *gQ, double *gx) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = blockIdx.y;
+  const bool ok = d.st[t].verdict == BPF_V_CONVERGED;
+  const double *lt = lam + (size_t)t * 2 * g.nb;
+  if (e < g.n) {
+    int k = g.blk_of_bus[e];
+    size_t o = (size_t)t * g.n + e;
+    gP[o] = ok ? (k >= 0 ? lt[2 * k] : 0.0) : NAN;
+    gQ[o] = ok ? ((k >= 0 && d.type[o] == 1) ? lt[2 * k + 1] : 0.0) : NAN;
+  }
+  if (e >= g.m) return;
+  size_t ob = (size_t)t * g.m + e;
+  if (!ok) { gx[ob] = NAN; return; }
+  if (!d.br_on[ob]) { gx[ob] = 0.0; return; }
+  BranchState s = branch_state(g, d, t, e, rate, w);
+  cd Ys = cdiv(mk(1.0, 0.0), mk(g.R[e], g.X[e]));
+  cd dYs = cmul(mk(0.0, -1.0), cmul(Ys, Ys));
+  double tp = d.tap[ob];
+  cd tap = (tp != 0.0) ? mk(tp, 0.0) : mk(1.0, 0.0);
+  tap = cmul(tap, cexpj(M_PI / 180.0 * g.shift[e]));
+  cd dyff = cdiv(dYs, cmul(tap, cconj(tap)));
+  cd dyft = cdiv(cmul(dYs, mk(-1.0, 0.0)), cconj(tap));
+  cd dytf = cdiv(cmul(dYs, mk(-1.0, 0.0)), tap);
+  cd dSf = cmul(s.Vf, cconj(cadd(cmul(dyff, s.Vf), cmul(dyft, s.Vt))));
+  cd dSt = cmul(s.Vt, cconj(cadd(cmul(dytf, s.Vf), cmul(dYs, s.Vt))));
+  double val = cmul(s.af, dSf).x + cmul(s.at, dSt).x;
+  int bf = g.blk_of_bus[g.br_from[e]], bt = g.blk_of_bus[g.br_to[e]];
+  if (bf >= 0) {
+    double lq = d.type[(size_t)t * g.n + g.br_from[e]] == 1 ? lt[2 * bf + 1] : 0.0;
+    val -= lt[2 * bf] * dSf.x + lq * dSf.y;
+  }
+  if (bt >= 0) {
+    double lq = d.type[(size_t)t * g.n + g.br_to[e]] == 1 ? lt[2 * bt + 1] : 0.0;
+    val -= lt[2 * bt] * dSt.x + lq * dSt.y;
+  }
+  gx[ob] = val;
+}
+
+// objective value per instance (one workgroup per instance)
+__global__ void k_objective(DevTopo g, DevInst d, const double *rate, double w,
+                            double *obj) {
+  __shared__ double sh[16];
+  const int t = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < g.m; b += blockDim.x) {
+    BranchState s = branch_state(g, d, t, b, rate, w);
+    acc += cadd(s.Sf, s.St).x;
+    if (rate) {
+      double ef = cabs_(s.Sf) - rate[b], et = cabs_(s.St) - rate[b];
+      if (ef > 0) acc += w * ef * ef;
+      if (et > 0) acc += w * et * et;
+    }
+  }
+  double tot = block_sum(acc, sh);
+  if (threadIdx.x == 0)
+    obj[t] = (d.st[t].verdict == BPF_V_CONVERGED) ? tot : NAN;
+}
+
 // ---------------------------------------------------------------------------
 // Losses and branch flows (computeLoss + reduceLoss, runpf.h:543-604; CPU
 // runpf.h:228-257).  One workgroup per instance.
@@ -520,9 +797,9 @@ __global__ void k_loss(DevTopo g, DevInst d, cd *Sf, cd *St, double *loss) {
 // explicit instantiations used by the engine
 template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *);
 template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *);
-template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, const int *, const int *, const int *, const int *, const int *, const int *, const int *, const int *, B4<float> *);
-template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, const int *, const int *, const int *, const int *, const int *, const int *, const int *, const int *, B4<double> *);
-template __global__ void k_lu_solve<float>(DevTopo, DevInst, const B4<float> *, int, int, const int *, const int *, int, const int *, const int *, int, float *);
-template __global__ void k_lu_solve<double>(DevTopo, DevInst, const B4<double> *, int, int, const int *, const int *, int, const int *, const int *, int, double *);
+template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, int, const int *, const int *, const int *, const int *, const int *, const int *, const int *, const int *, B4<float> *);
+template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, int, const int *, const int *, const int *, const int *, const int *, const int *, const int *, const int *, B4<double> *);
+template __global__ void k_lu_solve<float>(DevTopo, DevInst, const B4<float> *, int, int, int, const int *, const int *, int, const int *, const int *, int, float *);
+template __global__ void k_lu_solve<double>(DevTopo, DevInst, const B4<double> *, int, int, int, const int *, const int *, int, const int *, const int *, int, double *);
 
 }  // namespace bpf

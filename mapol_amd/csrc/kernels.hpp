@@ -35,7 +35,7 @@ struct DevTopo {
   const int *yf_ptr, *yf_item, *yt_ptr, *yt_item;
   const int *blk_of_bus, *bus_of_blk, *y2lu;
   const int *lrow, *lcol, *ldiag, *mirror;
-  const int *br_from, *br_to;
+  const int *br_from, *br_to, *bbr_ptr, *bbr;
   const double *R, *X, *B, *shift, *Gs, *Qd, *Va0;
   const int *base_type;
 };

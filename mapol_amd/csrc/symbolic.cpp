@@ -117,6 +117,20 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     S.yf_item.insert(S.yf_item.end(), fl[p].begin(), fl[p].end());
     S.yt_item.insert(S.yt_item.end(), tl[p].begin(), tl[p].end());
   }
+  S.bbr_ptr.assign(n + 1, 0);
+  for (int b = 0; b < m; b++) {
+    S.bbr_ptr[from[b] + 1]++;
+    if (to[b] != from[b]) S.bbr_ptr[to[b] + 1]++;
+  }
+  for (int i = 0; i < n; i++) S.bbr_ptr[i + 1] += S.bbr_ptr[i];
+  S.bbr.resize(S.bbr_ptr[n]);
+  {
+    std::vector<int> w(S.bbr_ptr.begin(), S.bbr_ptr.end() - 1);
+    for (int b = 0; b < m; b++) {
+      S.bbr[w[from[b]]++] = b;
+      if (to[b] != from[b]) S.bbr[w[to[b]]++] = b;
+    }
+  }
   // ---- blocks and ordering -------------------------------------------------
   std::vector<int> unk;
   for (int i = 0; i < n; i++)

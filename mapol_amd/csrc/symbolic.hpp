@@ -25,6 +25,8 @@ struct Symbolic {
   // contributions per position, in the reference's summation order: from-side
   // terms in branch order, then to-side terms in branch order
   std::vector<int> yf_ptr, yf_item, yt_ptr, yt_item;
+  // branches incident to each bus (from or to end), ascending branch index
+  std::vector<int> bbr_ptr, bbr;
   // non-slack buses become 2x2 blocks, in elimination order
   std::vector<int> blk_of_bus, bus_of_blk;
   // block CSR of L+U (structurally symmetric), rows in elimination order

@@ -37,3 +37,54 @@ def test_default_options_follow_reference():
     L.bpf_default_options(C.byref(o))
     assert (o.maxit_nr, o.maxit_fd, o.tol) == (10, 30, 1e-8)
     assert o.alg == engine.BPF_NR and o.precision == engine.BPF_FP64
+
+
+def _build_demo(tmp_path):
+    import subprocess
+    exe = str(tmp_path / "mirror_demo")
+    cmd = ["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "mirror_demo.cpp"),
+           "-L", os.path.dirname(engine.LIB_PATH), "-lbatchpf",
+           f"-Wl,-rpath,{os.path.dirname(engine.LIB_PATH)}", "-o", exe]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return exe
+
+
+@pytest.mark.skipif(not os.path.exists(engine.LIB_PATH),
+                    reason="libbatchpf.so not built")
+def test_cpp_mirror_compiles_and_links(tmp_path):
+    assert os.path.exists(_build_demo(tmp_path))
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_runs_like_reference(tmp_path):
+    """The C++ mirror gives each particle the loss the oracle computes for
+    the same controls (AVR set-point, OLTC tap)."""
+    import subprocess
+    import numpy as np
+    import oracle
+    from mapol_amd import synth
+    out = subprocess.run([_build_demo(tmp_path)], check=True,
+                         capture_output=True, text=True).stdout.split("\n")
+    losses = [float(l.split()[-1]) for l in out if l.startswith("particle")]
+    # the same 3-bus case in oracle form (net injections as negative loads)
+    case = synth.Case(
+        bus_type=np.array([3, 2, 1], np.int32), Pd=np.array([0.0, -0.4, 0.9]),
+        Qd=np.array([0.0, 0.0, 0.3]), Gs=np.zeros(3), Bs=np.zeros(3),
+        Vm=np.array([1.02, 1.0, 1.0]), Va=np.zeros(3),
+        br_from=np.array([0, 1, 0], np.int32), br_to=np.array([1, 2, 2], np.int32),
+        R=np.full(3, 0.02), X=np.full(3, 0.08), B=np.full(3, 0.02),
+        tap=np.array([0, 0, 0.98]), shift=np.zeros(3),
+        br_status=np.ones(3, np.uint8), gen_bus=np.array([0, 1], np.int32),
+        Pg=np.zeros(2), Qg=np.zeros(2), Qmax=np.full(2, np.inf),
+        Qmin=np.full(2, -np.inf), VG=np.array([1.0, 1.01]),
+        gen_status=np.ones(2, np.uint8))
+    orc = oracle.Oracle(case)
+    for k, got in enumerate(losses):
+        vset = np.full(3, np.nan)
+        vset[1] = 0.98 + 0.02 * k
+        tap = np.full(3, np.nan)
+        tap[2] = 0.95 + 0.03 * k
+        r = orc.solve(vset=vset, tap=tap)
+        assert r["converged"]
+        assert got == pytest.approx(r["loss"], rel=1e-9)

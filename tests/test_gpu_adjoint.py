@@ -1,0 +1,145 @@
+"""Adjoint gradients of losses + branch-flow penalty (HIP engine, FP64
+J^T lambda solve) against central finite differences (h = 1e-6) of the
+oracle's Newton re-solved at perturbed inputs; bar: 1e-6 relative.
+
+The torch.autograd.Function below exists in the test harness only."""
+import numpy as np
+import pytest
+
+import oracle
+from mapol_amd import synth
+from mapol_amd.engine import BPF_MIXED, CONVERGED, Engine
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-6
+W = 1.0
+
+
+def _rates(case, seed=3):
+    """ratings U(0.8, 1.5) x base-case |Sf| (as in the 2869-bus workload)"""
+    orc = oracle.Oracle(case)
+    r = orc.solve()
+    _, Sf, St = orc.objective(r["V"])
+    rng = np.random.default_rng(seed)
+    base = np.maximum(np.abs(Sf), np.abs(St))
+    return np.maximum(base, 1e-3) * rng.uniform(0.8, 1.5, size=case.nbranch)
+
+
+def _fd(case, inst, rate, kind, idx):
+    def f(delta):
+        c = synth.Case(**{k: (np.array(v, copy=True) if isinstance(v, np.ndarray)
+                              else v) for k, v in case.__dict__.items()})
+        if kind == "P":
+            c.Pd[idx] -= delta          # injection = Pg - Pd
+        elif kind == "Q":
+            c.Qd[idx] -= delta
+        else:
+            c.X[idx] += delta
+        orc = oracle.Oracle(c)
+        r = orc.solve(oracle.Oracle.opts(tol=1e-13, maxit_nr=20), **inst)
+        assert r["converged"]
+        return orc.objective(r["V"], rate=rate, w=W, **inst)[0]
+    return (f(H) - f(-H)) / (2 * H)
+
+
+@pytest.mark.parametrize("name,T", [("pf118x200", 4), ("pf2869x100", 2)])
+def test_adjoint_matches_finite_differences(name, T):
+    case, cont = synth.workload(name, T=T)
+    rate = _rates(case)
+    e = Engine(case, tol=1e-12, maxit_nr=20)
+    e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+    e.solve()
+    r = e.results()
+    assert (r["verdict"] == CONVERGED).all()
+    adj = e.adjoint(rate=rate, w=W)
+    rng = np.random.default_rng(11)
+    pq = np.flatnonzero(case.bus_type == 1)
+    pv = np.flatnonzero(case.bus_type == 2)
+    for t in range(T):
+        inst = dict(br_on=cont.br_on[t], gen_on=cont.gen_on[t])
+        orc = oracle.Oracle(case)
+        ro = orc.solve(oracle.Oracle.opts(tol=1e-13, maxit_nr=20), **inst)
+        fo = orc.objective(ro["V"], rate=rate, w=W, **inst)[0]
+        assert adj["objective"][t] == pytest.approx(fo, rel=1e-10)
+        on = np.flatnonzero(cont.br_on[t])
+        checks = ([("P", int(i)) for i in rng.choice(pq, 2, replace=False)]
+                  + [("P", int(rng.choice(pv)))]
+                  + [("Q", int(i)) for i in rng.choice(pq, 2, replace=False)]
+                  + [("x", int(b)) for b in rng.choice(on, 3, replace=False)])
+        for kind, idx in checks:
+            g = {"P": adj["grad_P"], "Q": adj["grad_Q"],
+                 "x": adj["grad_x"]}[kind][t, idx]
+            gfd = _fd(case, inst, rate, kind, idx)
+            scale = max(abs(gfd), 1e-3)
+            assert abs(g - gfd) <= 1e-6 * scale + 1e-9, (kind, idx, g, gfd)
+    e.close()
+
+
+def test_adjoint_slack_and_pv_q_are_zero():
+    case, cont = synth.workload("pf118x200", T=2)
+    e = Engine(case)
+    e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+    e.solve()
+    adj = e.adjoint()
+    slack = np.flatnonzero(case.bus_type == 3)
+    pv = np.flatnonzero(case.bus_type == 2)
+    assert np.all(adj["grad_P"][:, slack] == 0)
+    assert np.all(adj["grad_Q"][:, pv] == 0)
+    e.close()
+
+
+def test_autograd_wrapper_matches_adjoint():
+    """torch.autograd.Function over the engine: loads and reactances in,
+    per-contingency objective out; backward through the adjoint kernel."""
+    import torch
+
+    case, cont = synth.workload("pf118x200", T=3)
+    rate = _rates(case)
+
+    class PowerFlowObjective(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, Pd, Qd, X):
+            c = synth.Case(**{k: (np.array(v, copy=True) if isinstance(v, np.ndarray)
+                                  else v) for k, v in case.__dict__.items()})
+            c.Pd = Pd.detach().cpu().double().numpy()
+            c.Qd = Qd.detach().cpu().double().numpy()
+            c.X = X.detach().cpu().double().numpy()
+            e = Engine(c, precision=BPF_MIXED, refine_steps=2, tol=1e-12,
+                       maxit_nr=20)
+            e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+            e.solve()
+            adj = e.adjoint(rate=rate, w=W)
+            e.close()
+            ctx.save_for_backward(torch.from_numpy(adj["grad_P"]),
+                                  torch.from_numpy(adj["grad_Q"]),
+                                  torch.from_numpy(adj["grad_x"]))
+            return torch.from_numpy(adj["objective"])
+
+        @staticmethod
+        def backward(ctx, go):
+            gP, gQ, gx = ctx.saved_tensors
+            # injection = Pg - Pd: d f / d Pd = -d f / d P_injection
+            return (-(go[:, None] * gP).sum(0), -(go[:, None] * gQ).sum(0),
+                    (go[:, None] * gx).sum(0))
+
+    Pd = torch.tensor(case.Pd, dtype=torch.float64, requires_grad=True)
+    Qd = torch.tensor(case.Qd, dtype=torch.float64, requires_grad=True)
+    X = torch.tensor(case.X, dtype=torch.float64, requires_grad=True)
+    out = PowerFlowObjective.apply(Pd, Qd, X)
+    out.sum().backward()
+    # directional finite difference of the summed objective
+    rng = np.random.default_rng(4)
+    dP, dQ, dX = (rng.normal(size=case.nbus), rng.normal(size=case.nbus),
+                  rng.normal(size=case.nbranch) * 0.01)
+    with torch.no_grad():
+        fp = PowerFlowObjective.apply(Pd + H * torch.from_numpy(dP),
+                                      Qd + H * torch.from_numpy(dQ),
+                                      X + H * torch.from_numpy(dX)).sum()
+        fm = PowerFlowObjective.apply(Pd - H * torch.from_numpy(dP),
+                                      Qd - H * torch.from_numpy(dQ),
+                                      X - H * torch.from_numpy(dX)).sum()
+    fd = float((fp - fm) / (2 * H))
+    an = float((Pd.grad.numpy() * dP).sum() + (Qd.grad.numpy() * dQ).sum()
+               + (X.grad.numpy() * dX).sum())
+    assert abs(an - fd) <= 1e-6 * max(abs(fd), 1e-3)

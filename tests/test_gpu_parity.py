@@ -177,3 +177,18 @@ def test_nr_matches_oracle_large(name, T):
                 assert np.abs(r["V"][t] - ro["V"]).max() <= DV_TOL
                 assert r["loss"][t] == pytest.approx(ro["loss"], rel=1e-8)
         e.close()
+
+
+@pytest.mark.parametrize("alg,tag", [(1, "fdxb"), (2, "fdbx")])
+def test_fdpf_matches_golden(golden, alg, tag):
+    idx = golden[f"pf118x200/{tag}/idx"]
+    case, cont = synth.workload("pf118x200")
+    e = Engine(case, alg=alg)
+    e.set_instances(br_on=cont.br_on[idx], gen_on=cont.gen_on[idx])
+    e.solve()
+    r = e.results()
+    np.testing.assert_array_equal(r["iterations"], golden[f"pf118x200/{tag}/its"])
+    np.testing.assert_array_equal(r["verdict"], golden[f"pf118x200/{tag}/verdict"])
+    ok = r["verdict"] == CONVERGED
+    assert np.abs(r["V"][ok] - golden[f"pf118x200/{tag}/V"][ok]).max() <= DV_TOL
+    e.close()
