@@ -105,6 +105,8 @@ typedef struct {
   int64_t lu_pairs;                /* 2x2 block products per factorization */
   int32_t instances;               /* T set by bpf_set_instances */
   int32_t lds_solve;               /* 1: solve vector held in LDS */
+  int32_t crout_tasks;             /* phase-A outputs (one per L+U block) */
+  int32_t crout_scale_tasks;       /* phase-B L-block scalings */
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */
@@ -149,6 +151,23 @@ void bpf_destroy(bpf_engine *e);
 
 const char *bpf_last_error(const bpf_engine *e);
 bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info);
+
+/* Host-only symbolic analysis (no device needed): the block order, the L+U
+ * block pattern and the level schedules bpf_create builds (the J pattern of
+ * newtonpf.h:215-310 extended by the fill of the factorization).  Sizes come
+ * back in *info; every array may be NULL, otherwise it must hold:
+ * bus_of_blk[nblocks], lu_rowptr[nblocks+1], lu_colind[nnz_lu_blocks],
+ * lvA[lu_levels+1], tA_dst/tA_ptr[crout_tasks(+1)], pairL/pairU[lu_pairs],
+ * lvB[lu_levels+1], tB_dst/tB_diag[crout_scale_tasks],
+ * fw_ptr[fw_levels+1], fw_rows[nblocks], bw_ptr[bw_levels+1],
+ * bw_rows[nblocks].  tA_dst carries flags: bit 30 = position holds a
+ * Jacobian entry, bit 29 = pivot block (stored inverted). */
+bpf_status bpf_analyse(const bpf_case *c, bpf_info *info, int32_t *bus_of_blk,
+                       int32_t *lu_rowptr, int32_t *lu_colind, int32_t *lvA,
+                       int32_t *tA_dst, int32_t *tA_ptr, int32_t *pairL,
+                       int32_t *pairU, int32_t *lvB, int32_t *tB_dst,
+                       int32_t *tB_diag, int32_t *fw_ptr, int32_t *fw_rows,
+                       int32_t *bw_ptr, int32_t *bw_rows);
 
 /* ---- stage-level entry points (tests, benchmarks) ---------------------- */
 

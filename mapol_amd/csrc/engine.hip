@@ -499,6 +499,8 @@ static bpf_status set_lds_limits(bpf_engine *e) {
                       hipFuncAttributeMaxDynamicSharedMemorySize, e->max_lds);
   hipFuncSetAttribute((const void *)k_lu_solve<double>,
                       hipFuncAttributeMaxDynamicSharedMemorySize, e->max_lds);
+  hipFuncSetAttribute((const void *)k_adj_solve,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, e->max_lds);
   (void)hipGetLastError();
   return BPF_OK;
 }
@@ -665,11 +667,9 @@ extern "C" bpf_status bpf_get_branch_flows(bpf_engine *e, double *Sf, double *St
   return BPF_OK;
 }
 
-extern "C" bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info) {
-  if (!e || !info) return BPF_ERR_INVALID_ARG;
-  const Symbolic &S = e->S;
-  info->nbus = e->n;
-  info->nbranch = e->m;
+static void fill_info(const Symbolic &S, bpf_info *info) {
+  info->nbus = S.n;
+  info->nbranch = S.m;
   info->nblocks = S.nb;
   info->nnz_ybus = (int)S.ycol.size();
   info->nnz_jac_blocks = S.nnz_jac_blocks;
@@ -678,6 +678,43 @@ extern "C" bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info) {
   info->fw_levels = S.nfw;
   info->bw_levels = S.nbw;
   info->lu_pairs = S.npairs;
+  info->crout_tasks = (int)S.tA_dst.size();
+  info->crout_scale_tasks = (int)S.tB_dst.size();
+}
+
+extern "C" bpf_status bpf_analyse(const bpf_case *c, bpf_info *info,
+                                  int32_t *bus_of_blk, int32_t *lu_rowptr,
+                                  int32_t *lu_colind, int32_t *lvA,
+                                  int32_t *tA_dst, int32_t *tA_ptr,
+                                  int32_t *pairL, int32_t *pairU, int32_t *lvB,
+                                  int32_t *tB_dst, int32_t *tB_diag,
+                                  int32_t *fw_ptr, int32_t *fw_rows,
+                                  int32_t *bw_ptr, int32_t *bw_rows) {
+  if (!c || !info || c->nbus <= 0 || c->nbranch < 0 || !c->bus_type ||
+      (c->nbranch && (!c->br_from || !c->br_to)))
+    return BPF_ERR_INVALID_ARG;
+  Symbolic S;
+  std::string err;
+  if (!analyse(c->nbus, c->nbranch, c->br_from, c->br_to, c->bus_type, S, &err))
+    return BPF_ERR_INVALID_ARG;
+  memset(info, 0, sizeof(*info));
+  fill_info(S, info);
+  auto put = [](int32_t *dst, const std::vector<int> &v) {
+    if (dst && !v.empty()) memcpy(dst, v.data(), sizeof(int32_t) * v.size());
+  };
+  put(bus_of_blk, S.bus_of_blk); put(lu_rowptr, S.lrow); put(lu_colind, S.lcol);
+  put(lvA, S.lvA); put(tA_dst, S.tA_dst); put(tA_ptr, S.tA_pp);
+  put(pairL, S.pairL); put(pairU, S.pairU); put(lvB, S.lvB);
+  put(tB_dst, S.tB_dst); put(tB_diag, S.tB_diag);
+  put(fw_ptr, S.fw_ptr); put(fw_rows, S.fw_rows);
+  put(bw_ptr, S.bw_ptr); put(bw_rows, S.bw_rows);
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info) {
+  if (!e || !info) return BPF_ERR_INVALID_ARG;
+  const Symbolic &S = e->S;
+  fill_info(S, info);
   info->instances = e->T;
   int fp64 = e->opt.precision == BPF_FP64;
   info->lds_solve = (int64_t)((fp64 ? 8 : 4) * 2 * (size_t)S.nb) <= e->max_lds;

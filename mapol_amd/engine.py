@@ -29,6 +29,7 @@ EXPORTS = [
     "bpf_last_error", "bpf_get_info", "bpf_get_ybus", "bpf_stage_mismatch",
     "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_adjoint",
     "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
+    "bpf_analyse",
 ]
 
 
@@ -61,7 +62,8 @@ class BpfInfo(C.Structure):
                 ("nnz_jac_blocks", C.c_int32), ("nnz_lu_blocks", C.c_int32),
                 ("lu_levels", C.c_int32), ("fw_levels", C.c_int32),
                 ("bw_levels", C.c_int32), ("lu_pairs", C.c_int64),
-                ("instances", C.c_int32), ("lds_solve", C.c_int32)]
+                ("instances", C.c_int32), ("lds_solve", C.c_int32),
+                ("crout_tasks", C.c_int32), ("crout_scale_tasks", C.c_int32)]
 
 
 _LIB = None
@@ -104,6 +106,7 @@ def load_library(path: str = LIB_PATH):
     L.bpf_set_timing.argtypes = [V, C.c_int32]
     L.bpf_get_kernel_time.argtypes = [V, C.c_char_p, _pi, _pd]
     L.bpf_set_launch.argtypes = [V, C.c_int32, C.c_int32, C.c_int32]
+    L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 15
     for name in EXPORTS:
         getattr(L, name)
     _LIB = L
@@ -112,6 +115,61 @@ def load_library(path: str = LIB_PATH):
 
 def _ptr(a, t):
     return None if a is None else a.ctypes.data_as(t)
+
+
+def _case_struct(c, keep):
+    def arr(v, dt):
+        a = np.ascontiguousarray(v, dtype=dt)
+        keep.append(a)
+        return a
+
+    s = BpfCase()
+    s.nbus, s.nbranch, s.ngen = c.nbus, c.nbranch, c.ngen
+    s.baseMVA = float(getattr(c, "baseMVA", 100.0))
+    s.bus_type = _ptr(arr(c.bus_type, np.int32), _pi)
+    for f in ("Pd", "Qd", "Gs", "Bs", "Vm", "Va", "R", "X", "B", "tap",
+              "shift", "Pg", "Qg", "Qmax", "Qmin", "VG"):
+        setattr(s, f, _ptr(arr(getattr(c, f), np.float64), _pd))
+    s.br_from = _ptr(arr(c.br_from, np.int32), _pi)
+    s.br_to = _ptr(arr(c.br_to, np.int32), _pi)
+    s.br_status = _ptr(arr(c.br_status, np.uint8), _pu8)
+    s.gen_bus = _ptr(arr(c.gen_bus, np.int32), _pi)
+    s.gen_status = _ptr(arr(c.gen_status, np.uint8), _pu8)
+    return s
+
+
+def analyse(case):
+    """Host-only symbolic analysis (``bpf_analyse``): block order, L+U block
+    pattern and level schedules, as numpy arrays (no GPU needed)."""
+    L = load_library()
+    keep = []
+    s = _case_struct(case, keep)
+    info = BpfInfo()
+    st = L.bpf_analyse(C.byref(s), C.byref(info), *([None] * 15))
+    if st != 0:
+        raise EngineError(f"bpf_analyse failed ({st})")
+    i = {f: getattr(info, f) for f, _ in BpfInfo._fields_}
+    nb, nl = i["nblocks"], i["lu_levels"]
+    out = dict(
+        bus_of_blk=np.zeros(nb, np.int32), lu_rowptr=np.zeros(nb + 1, np.int32),
+        lu_colind=np.zeros(i["nnz_lu_blocks"], np.int32),
+        lvA=np.zeros(nl + 1, np.int32), tA_dst=np.zeros(i["crout_tasks"], np.int32),
+        tA_ptr=np.zeros(i["crout_tasks"] + 1, np.int32),
+        pairL=np.zeros(i["lu_pairs"], np.int32), pairU=np.zeros(i["lu_pairs"], np.int32),
+        lvB=np.zeros(nl + 1, np.int32),
+        tB_dst=np.zeros(i["crout_scale_tasks"], np.int32),
+        tB_diag=np.zeros(i["crout_scale_tasks"], np.int32),
+        fw_ptr=np.zeros(i["fw_levels"] + 1, np.int32), fw_rows=np.zeros(nb, np.int32),
+        bw_ptr=np.zeros(i["bw_levels"] + 1, np.int32), bw_rows=np.zeros(nb, np.int32))
+    order = ["bus_of_blk", "lu_rowptr", "lu_colind", "lvA", "tA_dst", "tA_ptr",
+             "pairL", "pairU", "lvB", "tB_dst", "tB_diag", "fw_ptr", "fw_rows",
+             "bw_ptr", "bw_rows"]
+    st = L.bpf_analyse(C.byref(s), C.byref(info),
+                       *[_ptr(out[k], _pi) for k in order])
+    if st != 0:
+        raise EngineError(f"bpf_analyse failed ({st})")
+    out["info"] = i
+    return out
 
 
 class Engine:
