@@ -107,6 +107,7 @@ typedef struct {
   int32_t lds_solve;               /* 1: solve vector held in LDS */
   int32_t crout_tasks;             /* phase-A outputs (one per L+U block) */
   int32_t crout_scale_tasks;       /* phase-B L-block scalings */
+  int32_t graph_used;              /* last solve replayed the iteration graph */
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */
@@ -149,6 +150,8 @@ bpf_status bpf_get_branch_flows(bpf_engine *e, double *Sf, double *St);
 /* hybrid_free (runpf.h:984-1078), mkl_clean (runpf.h:452). */
 void bpf_destroy(bpf_engine *e);
 
+/* Last error message; when there is none but hipGraph replay had to be
+ * abandoned (the solve then ran with direct launches), its reason. */
 const char *bpf_last_error(const bpf_engine *e);
 bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info);
 

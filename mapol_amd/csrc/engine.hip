@@ -3,6 +3,7 @@
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <chrono>
@@ -67,9 +68,15 @@ struct bpf_engine {
   // graph of one Newton iteration
   hipGraphExec_t graph = nullptr;
   int graph_key = -1;
+  bool graph_broken = false;   // a replay failed: direct launches from then on
+  std::string graph_err;
+  int graph_used = 0;          // the last solve replayed graphs
   // timing
   int timing = 0;
   std::vector<TimedEvent> events;
+  std::string launch_err;            // first failed launch (kernel name)
+  const char *cur_kernel = "";
+  int debug_sync = 0;                // BPF_DEBUG_SYNC: sync after launches
   bool capturing = false;            // recording the iteration graph
   int cap_slot = 0;
   std::vector<TimedEvent> fixed_ev;  // event nodes of the timed graph
@@ -82,6 +89,8 @@ struct bpf_engine {
     hipError_t e_ = (expr);                                                 \
     if (e_ != hipSuccess) {                                                 \
       e->err = std::string(#expr) + ": " + hipGetErrorString(e_);           \
+      if (!e->launch_err.empty()) e->err += " [first failed launch: " +     \
+                                            e->launch_err + "]";            \
       return BPF_ERR_HIP;                                                   \
     }                                                                       \
   } while (0)
@@ -138,7 +147,9 @@ extern "C" void bpf_default_options(bpf_options *o) {
 }
 
 extern "C" const char *bpf_last_error(const bpf_engine *e) {
-  return e ? e->err.c_str() : "null engine";
+  if (!e) return "null engine";
+  if (e->err.empty() && e->graph_broken) return e->graph_err.c_str();
+  return e->err.c_str();
 }
 
 extern "C" void bpf_destroy(bpf_engine *e) {
@@ -210,6 +221,8 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
                e->bus_type.data(), e->S, &e->err))
     return BPF_ERR_INVALID_ARG;
   // device
+  const char *dbg = getenv("BPF_DEBUG_SYNC");
+  e->debug_sync = (dbg && dbg[0] == '1') ? 1 : 0;
   HIPCHK(hipGetDevice(&e->dev));
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   HIPCHK(hipDeviceGetAttribute(&e->max_lds,
@@ -381,7 +394,7 @@ static bpf_status ensure_lu(bpf_engine *e, int fp64) {
   bpf_status s = dalloc(e, e->inst_owned, el * (size_t)e->T * e->g.nnzLU, &e->d_LU);
   if (s) return s;
   size_t xs = (fp64 ? 8 : 4) * (size_t)2 * e->S.nb;
-  if ((int64_t)xs > e->max_lds) {
+  if (xs > (size_t)SOLVE_LDS_BYTES) {
     s = dalloc(e, e->inst_owned, xs * e->T, &e->d_xglob);
     if (s) return s;
   }
@@ -392,6 +405,7 @@ static bpf_status ensure_lu(bpf_engine *e, int fp64) {
 
 // ---- launches ---------------------------------------------------------------
 static void tbegin(bpf_engine *e, const char *name) {
+  e->cur_kernel = name;
   if (!e->timing) return;
   if (e->capturing) {
     // fixed events become event-record nodes of the graph; their elapsed
@@ -413,7 +427,20 @@ static void tbegin(bpf_engine *e, const char *name) {
   hipEventRecord(t.a, e->stream);
   e->events.push_back(t);
 }
+// record the first launch error with the kernel's name; in debug mode also
+// wait for the kernel so that execution errors are attributed too
+static void tcheck(bpf_engine *e, const char *name) {
+  hipError_t r = hipGetLastError();
+  if (r == hipSuccess && e->debug_sync && !e->capturing) {
+    r = hipStreamSynchronize(e->stream);
+    if (r == hipSuccess) r = hipGetLastError();
+  }
+  if (r != hipSuccess && e->launch_err.empty())
+    e->launch_err = std::string(name) + ": " + hipGetErrorString(r);
+}
 static void tend(bpf_engine *e) {
+  if (!e->events.empty() || !e->fixed_ev.empty()) {}
+  tcheck(e, e->cur_kernel);
   if (!e->timing) return;
   if (e->capturing) {
     hipEventRecord(e->fixed_ev[e->cap_slot - 1].b, e->stream);
@@ -470,11 +497,11 @@ static void launch_newton_step(bpf_engine *e) {
                      e->d_lvB, e->d_tB_dst, e->d_tB_diag, LU);
   tend(e);
   size_t xs = sizeof(T) * 2 * (size_t)e->S.nb;
-  int use_lds = (int64_t)xs <= e->max_lds;
+  int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
   int refine = (sizeof(T) == 4) ? e->opt.refine_steps : 0;
   tbegin(e, "lu_solve");
   hipLaunchKernelGGL(k_lu_solve<T>, dim3(e->T), dim3(e->solve_threads),
-                     use_lds ? xs : 0, e->stream, e->g, e->d,
+                     0, e->stream, e->g, e->d,
                      (const B4<T> *)LU, refine, 0, e->S.nfw, e->d_fw_ptr,
                      e->d_fw_rows, e->S.nbw, e->d_bw_ptr, e->d_bw_rows,
                      use_lds, (T *)e->d_xglob);
@@ -493,26 +520,14 @@ static void launch_iteration(bpf_engine *e) {
                  hipMemcpyDeviceToHost, e->stream);
 }
 
-static bpf_status set_lds_limits(bpf_engine *e) {
-  // allow the dynamic LDS request of the solve kernels up to the device limit
-  hipFuncSetAttribute((const void *)k_lu_solve<float>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, e->max_lds);
-  hipFuncSetAttribute((const void *)k_lu_solve<double>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, e->max_lds);
-  hipFuncSetAttribute((const void *)k_adj_solve,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, e->max_lds);
-  (void)hipGetLastError();
-  return BPF_OK;
-}
 
-static bpf_status run_newton(bpf_engine *e) {
+static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
   bpf_status s = ensure_lu(e, e->opt.precision == BPF_FP64 ? 1 : 0);
   if (s) return s;
-  set_lds_limits(e);
   const int key = e->timing * 100000000 + e->lu_fp64 * 10000000 +
                   e->lu_threads * 10000 + e->mis_threads * 10 / 64 +
                   e->solve_threads / 64;
-  bool use_graph = e->opt.use_graph != 0;
+  bool use_graph = e->opt.use_graph != 0 && !e->graph_broken;
   if (use_graph && (!e->graph || e->graph_key != key)) {
     if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
     for (auto &ev : e->fixed_ev) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
@@ -527,17 +542,37 @@ static bpf_status run_newton(bpf_engine *e) {
     hipError_t r = hipGraphInstantiate(&e->graph, gr, nullptr, nullptr, 0);
     hipGraphDestroy(gr);
     if (r != hipSuccess) {
-      e->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(r);
-      return BPF_ERR_HIP;
+      e->graph = nullptr;
+      e->graph_broken = true;
+      e->graph_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(r);
+      (void)hipGetLastError();
+      use_graph = false;
+    } else {
+      e->graph_key = key;
     }
-    e->graph_key = key;
   }
+  e->graph_used = use_graph ? 1 : 0;
   const int cap = e->opt.maxit_nr * (e->opt.max_qlim_rounds + 1) + 2;
   for (int it = 0; it < cap; it++) {
-    if (use_graph) HIPCHK(hipGraphLaunch(e->graph, e->stream));
+    hipError_t r = hipSuccess;
+    if (use_graph) r = hipGraphLaunch(e->graph, e->stream);
     else launch_iteration(e);
-    HIPCHK(hipStreamSynchronize(e->stream));
-    HIPCHK(hipGetLastError());
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    if (r == hipSuccess) r = hipGetLastError();
+    if (r != hipSuccess) {
+      if (use_graph) {
+        // the runtime refused the replay: remember why and let bpf_solve
+        // start the solve again with direct launches
+        e->graph_broken = true;
+        e->graph_err = std::string("graph replay: ") + hipGetErrorString(r);
+        (void)hipGetLastError();
+        *graph_failed = true;
+        return BPF_OK;
+      }
+      e->err = std::string("Newton iteration: ") + hipGetErrorString(r) +
+               (e->launch_err.empty() ? "" : " [" + e->launch_err + "]");
+      return BPF_ERR_HIP;
+    }
     if (use_graph && e->timing) tcollect_fixed(e);
     if (*e->h_count == 0) break;
   }
@@ -551,7 +586,6 @@ static bpf_status run_newton(bpf_engine *e) {
 static bpf_status run_fd(bpf_engine *e) {
   bpf_status s = ensure_lu(e, 1);
   if (s) return s;
-  set_lds_limits(e);
   B4<double> *LU = (B4<double> *)e->d_LU;
   const unsigned T = (unsigned)e->T;
   tbegin(e, "bmat");
@@ -565,12 +599,12 @@ static bpf_status run_fd(bpf_engine *e) {
                      e->d_lvB, e->d_tB_dst, e->d_tB_diag, LU);
   tend(e);
   size_t xs = sizeof(double) * 2 * (size_t)e->S.nb;
-  int use_lds = (int64_t)xs <= e->max_lds;
+  int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
   for (int it = 0; it < e->opt.maxit_fd + 1; it++) {
     for (int half = 1; half <= 2; half++) {
       tbegin(e, "fd_solve");
       hipLaunchKernelGGL(k_lu_solve<double>, dim3(T), dim3(e->solve_threads),
-                         use_lds ? xs : 0, e->stream, e->g, e->d,
+                         0, e->stream, e->g, e->d,
                          (const B4<double> *)LU, 0, half, e->S.nfw,
                          e->d_fw_ptr, e->d_fw_rows, e->S.nbw, e->d_bw_ptr,
                          e->d_bw_rows, use_lds, (double *)e->d_xglob);
@@ -595,13 +629,7 @@ static bpf_status run_fd(bpf_engine *e) {
   return BPF_OK;
 }
 
-extern "C" bpf_status bpf_solve(bpf_engine *e) {
-  if (!e) return BPF_ERR_INVALID_ARG;
-  if (e->T <= 0) { e->err = "bpf_set_instances not called"; return BPF_ERR_STATE; }
-  if (e->opt.alg != BPF_NR && e->opt.enforce_qlim) {
-    e->err = "reactive-limit switching is supported with Newton-Raphson only";
-    return BPF_ERR_UNSUPPORTED;
-  }
+static bpf_status solve_once(bpf_engine *e, bool *graph_failed) {
   const bool fd = e->opt.alg != BPF_NR;
   const unsigned T = (unsigned)e->T;
   tbegin(e, "ybus");
@@ -619,8 +647,8 @@ extern "C" bpf_status bpf_solve(bpf_engine *e) {
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
   if (*e->h_count > 0) {
-    bpf_status s = fd ? run_fd(e) : run_newton(e);
-    if (s) return s;
+    bpf_status s = fd ? run_fd(e) : run_newton(e, graph_failed);
+    if (s || *graph_failed) return s;
   }
   tbegin(e, "loss");
   hipLaunchKernelGGL(k_loss, dim3(T), dim3(256), 0, e->stream, e->g, e->d,
@@ -628,7 +656,27 @@ extern "C" bpf_status bpf_solve(bpf_engine *e) {
   tend(e);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_solve(bpf_engine *e) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  e->launch_err.clear();
+  if (e->T <= 0) { e->err = "bpf_set_instances not called"; return BPF_ERR_STATE; }
+  if (e->opt.alg != BPF_NR && e->opt.enforce_qlim) {
+    e->err = "reactive-limit switching is supported with Newton-Raphson only";
+    return BPF_ERR_UNSUPPORTED;
+  }
+  e->graph_used = 0;
+  bool graph_failed = false;
+  bpf_status s = solve_once(e, &graph_failed);
+  if (!s && graph_failed) {
+    for (auto &ev : e->events) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
+    e->events.clear();
+    s = solve_once(e, &graph_failed);
+  }
   tcollect(e);
+  if (s) return s;
   e->solved = true;
   return BPF_OK;
 }
@@ -716,8 +764,9 @@ extern "C" bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info) {
   const Symbolic &S = e->S;
   fill_info(S, info);
   info->instances = e->T;
+  info->graph_used = e->graph_used;
   int fp64 = e->opt.precision == BPF_FP64;
-  info->lds_solve = (int64_t)((fp64 ? 8 : 4) * 2 * (size_t)S.nb) <= e->max_lds;
+  info->lds_solve = ((fp64 ? 8 : 4) * 2 * (size_t)S.nb) <= (size_t)SOLVE_LDS_BYTES;
   return BPF_OK;
 }
 
@@ -856,7 +905,6 @@ extern "C" bpf_status bpf_stage_newton_step(bpf_engine *e, const double *V,
   if (s) return s;
   int fp64 = e->opt.precision == BPF_FP64;
   if ((s = ensure_lu(e, fp64))) return s;
-  set_lds_limits(e);
   std::vector<InstState> st(e->T);
   HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * e->T, hipMemcpyDeviceToHost));
   std::vector<InstState> st2 = st;
@@ -895,7 +943,6 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   }
   bpf_status s = ensure_lu(e, 1);
   if (s) return s;
-  set_lds_limits(e);
   const size_t T = e->T, n = e->n, m = e->m, nb2 = 2 * (size_t)e->S.nb;
   std::vector<DBuf> tmp;
   void *p;
@@ -933,9 +980,9 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   hipLaunchKernelGGL(k_adj_rhs, dim3(div_up(e->S.nb, 256), TT), dim3(256), 0,
                      e->stream, e->g, e->d, (const double *)d_rate, w, d_G);
   size_t xs = sizeof(double) * nb2;
-  int use_lds = (int64_t)xs <= e->max_lds;
+  int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
   hipLaunchKernelGGL(k_adj_solve, dim3(TT), dim3(e->solve_threads),
-                     use_lds ? xs : 0, e->stream, e->g, e->d,
+                     0, e->stream, e->g, e->d,
                      (const B4<double> *)LU, (const double *)d_G, d_lam,
                      e->S.nfw, e->d_fw_ptr, e->d_fw_rows, e->S.nbw,
                      e->d_bw_ptr, e->d_bw_rows, use_lds, (double *)e->d_xglob);

@@ -422,15 +422,20 @@ __device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, T *X, int nfw,
 //   dV_j = V_j (j dtheta_j + dVm_j / |V_j|).
 // One workgroup per instance.
 // ---------------------------------------------------------------------------
+// LDS reserved for the solve vector of one instance: 150 KB of gfx950's
+// 160 KB, i.e. 19200 blocks in FP32 or 9600 in FP64; larger systems keep the
+// vector in global memory (use_lds = 0)
+constexpr int SOLVE_LDS_BYTES = 150 * 1024;
+
 template <typename T>
 __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, int refine,
                            int rhs_mode, int nfw, const int *fw_ptr, const int *fw_rows,
                            int nbw, const int *bw_ptr, const int *bw_rows,
                            int use_lds, T *xglob) {
-  extern __shared__ unsigned char smem[];
+  __shared__ T smem[SOLVE_LDS_BYTES / sizeof(T)];
   const int t = blockIdx.x;
   if (d.st[t].done) return;
-  T *X = use_lds ? reinterpret_cast<T *>(smem) : xglob + (size_t)t * 2 * g.nb;
+  T *X = use_lds ? smem : xglob + (size_t)t * 2 * g.nb;
   const B4<T> *A = LU + (size_t)t * g.nnzLU;
   const double *F = d.F + (size_t)t * 2 * g.nb;
   double *dx = d.dx + (size_t)t * 2 * g.nb;
@@ -654,10 +659,10 @@ __global__ void k_adj_solve(DevTopo g, DevInst d, const B4<double> *LU,
                             const int *fw_ptr, const int *fw_rows, int nbw,
                             const int *bw_ptr, const int *bw_rows, int use_lds,
                             double *xglob) {
-  extern __shared__ unsigned char smem[];
+  __shared__ double smem[SOLVE_LDS_BYTES / sizeof(double)];
   const int t = blockIdx.x;
   if (d.st[t].done) return;
-  double *X = use_lds ? reinterpret_cast<double *>(smem) : xglob + (size_t)t * 2 * g.nb;
+  double *X = use_lds ? smem : xglob + (size_t)t * 2 * g.nb;
   const B4<double> *A = LU + (size_t)t * g.nnzLU;
   const double *Gt = G + (size_t)t * 2 * g.nb;
   for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) X[k] = Gt[k];

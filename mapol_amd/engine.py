@@ -63,7 +63,8 @@ class BpfInfo(C.Structure):
                 ("lu_levels", C.c_int32), ("fw_levels", C.c_int32),
                 ("bw_levels", C.c_int32), ("lu_pairs", C.c_int64),
                 ("instances", C.c_int32), ("lds_solve", C.c_int32),
-                ("crout_tasks", C.c_int32), ("crout_scale_tasks", C.c_int32)]
+                ("crout_tasks", C.c_int32), ("crout_scale_tasks", C.c_int32),
+                ("graph_used", C.c_int32)]
 
 
 _LIB = None
@@ -223,6 +224,9 @@ class Engine:
         if st != 0:
             raise EngineError(f"{what} failed ({st}): "
                               f"{self.L.bpf_last_error(self.h).decode()}")
+
+    def last_error(self) -> str:
+        return self.L.bpf_last_error(self.h).decode()
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
