@@ -59,13 +59,13 @@ def algorithmic_bytes(info, case, T, its_total, s, refine):
 
 
 def run_gpu(case, cont, precision, refine, qlim, warmup, steps, timing=True,
-            tune=True):
+            tune=True, graph=True):
     import torch
     from mapol_amd.engine import Engine
     from mapol_amd.tune import autotune
 
     e = Engine(case, precision=precision, refine_steps=refine,
-               enforce_qlim=qlim)
+               enforce_qlim=qlim, use_graph=graph)
     e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
     launch = autotune(e) if tune else None
     if launch:
@@ -87,6 +87,20 @@ def run_gpu(case, cont, precision, refine, qlim, warmup, steps, timing=True,
     e.set_timing(False)
     e.close()
     return dict(seconds=dt, kt=kt, res=r, info=info, launch=launch)
+
+
+def copy_calibration(mib):
+    """One plain device-to-device copy of ``mib`` MiB, for scaling the HBM
+    byte counters of a profiler run (FETCH_SIZE / WRITE_SIZE of this copy
+    should read ``mib`` MiB each)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copied {mib} MiB ({n * 4} bytes)")
 
 
 def cpu_baseline(case, cont, qlim, threads, limit_s=30.0):
@@ -125,6 +139,12 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each kernel directly instead of replaying "
+                         "the captured Newton-iteration graph")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one plain copy of MIB MiB (profiler "
+                         "counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: contingencies are independent "
@@ -132,6 +152,10 @@ def main():
               "(see DESIGN.md); multi-GPU runs are not part of this build.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import numpy as np
     import torch
@@ -145,12 +169,12 @@ def main():
     log(f"[bench] workload {args.workload}: {case.counts()} T={T} qlim={qlim}")
 
     mix = run_gpu(case, cont, BPF_MIXED, args.refine, qlim, args.warmup,
-                  args.steps, tune=not args.no_tune)
+                  args.steps, tune=not args.no_tune, graph=not args.no_graph)
     log(f"[bench] mixed: {mix['seconds'] / args.steps * 1e3:.2f} ms/step")
     fp = None
     if not args.no_fp64:
         fp = run_gpu(case, cont, BPF_FP64, 0, qlim, args.warmup, args.steps,
-                     tune=not args.no_tune)
+                     tune=not args.no_tune, graph=not args.no_graph)
         log(f"[bench] fp64: {fp['seconds'] / args.steps * 1e3:.2f} ms/step")
 
     value = T * args.steps / mix["seconds"]
@@ -201,6 +225,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "newton_instance_iterations": its_total,
+        "graph_replayed": bool(info["graph_used"]),
         "converged": int((rm["verdict"] == CONVERGED).sum()),
     }
     if fp is not None:

@@ -108,6 +108,10 @@ typedef struct {
   int32_t crout_tasks;             /* phase-A outputs (one per L+U block) */
   int32_t crout_scale_tasks;       /* phase-B L-block scalings */
   int32_t graph_used;              /* last solve replayed the iteration graph */
+  int32_t tail_level;              /* levels >= tail_level: dense LDS block */
+  int32_t tail_n;                  /* pivots in the dense tail (<= 64) */
+  int32_t tail_entries, tail_pairs;
+  int32_t tail_row_entries;        /* non-tail L entries of the tail rows */
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */
@@ -162,15 +166,27 @@ bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info);
  * bus_of_blk[nblocks], lu_rowptr[nblocks+1], lu_colind[nnz_lu_blocks],
  * lvA[lu_levels+1], tA_dst/tA_ptr[crout_tasks(+1)], pairL/pairU[lu_pairs],
  * lvB[lu_levels+1], tB_dst/tB_diag[crout_scale_tasks],
- * fw_ptr[fw_levels+1], fw_rows[nblocks], bw_ptr[bw_levels+1],
- * bw_rows[nblocks].  tA_dst carries flags: bit 30 = position holds a
- * Jacobian entry, bit 29 = pivot block (stored inverted). */
+ * fw_ptr[fw_levels+1], fw_rows[nblocks-tail_n], bw_ptr[bw_levels+1],
+ * bw_rows[nblocks-tail_n], tl_dense/tl_pos[tail_entries],
+ * tl_ptr[tail_entries+1], tl_pairL/tl_pairU[tail_pairs], tail_piv[tail_n],
+ * ts_ptr[tail_n+1], ts_pos/ts_col[tail_row_entries].  tA_dst carries flags:
+ * bit 30 = position holds a Jacobian entry, bit 29 = pivot block (stored
+ * inverted).  The factorization kernel runs the Crout levels below
+ * tail_level and then inverts the Schur complement of the top tail_n pivots
+ * (blocks tail_piv) as one dense block by Gauss-Jordan (tl_dense = row *
+ * tail_n + col of the dense block, same flag bits; its pairs are the
+ * non-tail contributions).  The sweeps solve the non-tail rows by levels
+ * (fw/bw lists) and the tail rows with that inverse after subtracting their
+ * non-tail L entries (ts lists: L position and column per tail row). */
 bpf_status bpf_analyse(const bpf_case *c, bpf_info *info, int32_t *bus_of_blk,
                        int32_t *lu_rowptr, int32_t *lu_colind, int32_t *lvA,
                        int32_t *tA_dst, int32_t *tA_ptr, int32_t *pairL,
                        int32_t *pairU, int32_t *lvB, int32_t *tB_dst,
                        int32_t *tB_diag, int32_t *fw_ptr, int32_t *fw_rows,
-                       int32_t *bw_ptr, int32_t *bw_rows);
+                       int32_t *bw_ptr, int32_t *bw_rows, int32_t *tl_dense,
+                       int32_t *tl_pos, int32_t *tl_ptr, int32_t *tl_pairL,
+                       int32_t *tl_pairU, int32_t *tail_piv, int32_t *ts_ptr,
+                       int32_t *ts_pos, int32_t *ts_col);
 
 /* ---- stage-level entry points (tests, benchmarks) ---------------------- */
 

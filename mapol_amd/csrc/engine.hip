@@ -31,6 +31,8 @@ struct TimedEvent {
 
 }  // namespace
 
+static unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
+
 struct bpf_engine {
   bpf_options opt;
   std::string err;
@@ -46,10 +48,8 @@ struct bpf_engine {
   // shared device arrays
   std::vector<DBuf> owned;
   DevTopo g{};
-  const int *d_lvA = nullptr, *d_tA_dst = nullptr, *d_tA_pp = nullptr,
-            *d_pairL = nullptr, *d_pairU = nullptr, *d_lvB = nullptr,
-            *d_tB_dst = nullptr, *d_tB_diag = nullptr, *d_fw_ptr = nullptr,
-            *d_fw_rows = nullptr, *d_bw_ptr = nullptr, *d_bw_rows = nullptr;
+  FactorPlan fplan{};            // factorization schedule (device arrays)
+  SolvePlan splan{};             // sweep schedule (device arrays)
   // per-instance device arrays
   std::vector<DBuf> inst_owned;
   int T = 0;
@@ -59,11 +59,12 @@ struct bpf_engine {
   void *d_LU = nullptr;         // B4<float> or B4<double> [T][nnzLU]
   int lu_fp64 = -1;             // precision the LU buffer was sized for
   void *d_xglob = nullptr;      // solve vectors when LDS is too small
+  void *d_TL = nullptr;         // inverse tail Schur complements [T][tn*tn]
   cd *d_Sf = nullptr, *d_St = nullptr;
   double *d_loss = nullptr;
   int *h_count = nullptr;       // pinned
   // launch shapes
-  int lu_threads = 1024, solve_threads = 1024, mis_threads = 512;
+  int lu_threads = 1024, solve_threads = 1024, mis_threads = 256;
   int max_lds = 65536;
   // graph of one Newton iteration
   hipGraphExec_t graph = nullptr;
@@ -247,11 +248,25 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(g.R, e->R); UP(g.X, e->X); UP(g.B, e->B); UP(g.shift, e->shift);
   UP(g.Gs, e->Gs); UP(g.Qd, e->Qd); UP(g.Va0, e->Va);
   UP(g.base_type, e->bus_type);
-  UP(e->d_lvA, S.lvA); UP(e->d_tA_dst, S.tA_dst); UP(e->d_tA_pp, S.tA_pp);
-  UP(e->d_pairL, S.pairL); UP(e->d_pairU, S.pairU); UP(e->d_lvB, S.lvB);
-  UP(e->d_tB_dst, S.tB_dst); UP(e->d_tB_diag, S.tB_diag);
-  UP(e->d_fw_ptr, S.fw_ptr); UP(e->d_fw_rows, S.fw_rows);
-  UP(e->d_bw_ptr, S.bw_ptr); UP(e->d_bw_rows, S.bw_rows);
+  FactorPlan &F = e->fplan;
+  F.nlev = S.tail_level;
+  UP(F.lvA, S.lvA); UP(F.lvA_max, S.lvA_max); UP(F.tA_dst, S.tA_dst);
+  UP(F.tA_pp, S.tA_pp); UP(F.pairL, S.pairL); UP(F.pairU, S.pairU);
+  UP(F.lvB, S.lvB); UP(F.tB_dst, S.tB_dst); UP(F.tB_diag, S.tB_diag);
+  F.tn = S.tail_n;
+  F.tnent = (int)S.tl_pos.size();
+  F.tmax = S.tl_max;
+  UP(F.tdense, S.tl_dense); UP(F.tpos, S.tl_pos); UP(F.tpp, S.tl_pp);
+  UP(F.tpairL, S.tl_pairL); UP(F.tpairU, S.tl_pairU);
+  SolvePlan &SP = e->splan;
+  SP.nfw = S.nfw;
+  SP.nbw = S.nbw;
+  UP(SP.fw_ptr, S.fw_ptr); UP(SP.fw_rows, S.fw_rows); UP(SP.fw_max, S.fw_max);
+  UP(SP.bw_ptr, S.bw_ptr); UP(SP.bw_rows, S.bw_rows); UP(SP.bw_max, S.bw_max);
+  SP.tn = S.tail_n;
+  SP.tsmax = S.ts_max;
+  UP(SP.tpiv, S.tail_piv); UP(SP.ts_ptr, S.ts_ptr); UP(SP.ts_pos, S.ts_pos);
+  UP(SP.ts_col, S.ts_col); UP(SP.ts_mir, S.ts_mir);
 #undef UP
   return BPF_OK;
 }
@@ -376,6 +391,7 @@ extern "C" bpf_status bpf_set_instances(bpf_engine *e, int32_t T,
   AL(e->d_Sf, cd, Tm);
   AL(e->d_St, cd, Tm);
   AL(e->d_loss, double, T);
+  AL(d.part, double, (size_t)T * div_up(nb, 64));
 #undef AL
   return BPF_OK;
 }
@@ -388,10 +404,15 @@ static bpf_status ensure_lu(bpf_engine *e, int fp64) {
       if (it->p == e->d_LU) { hipFree(it->p); e->inst_owned.erase(it); break; }
     for (auto it = e->inst_owned.begin(); it != e->inst_owned.end(); ++it)
       if (it->p == e->d_xglob) { hipFree(it->p); e->inst_owned.erase(it); break; }
-    e->d_LU = e->d_xglob = nullptr;
+    for (auto it = e->inst_owned.begin(); it != e->inst_owned.end(); ++it)
+      if (it->p == e->d_TL) { hipFree(it->p); e->inst_owned.erase(it); break; }
+    e->d_LU = e->d_xglob = e->d_TL = nullptr;
   }
   size_t el = fp64 ? sizeof(B4<double>) : sizeof(B4<float>);
   bpf_status s = dalloc(e, e->inst_owned, el * (size_t)e->T * e->g.nnzLU, &e->d_LU);
+  if (s) return s;
+  s = dalloc(e, e->inst_owned, el * (size_t)e->T * e->S.tail_n * e->S.tail_n,
+             &e->d_TL);
   if (s) return s;
   size_t xs = (fp64 ? 8 : 4) * (size_t)2 * e->S.nb;
   if (xs > (size_t)SOLVE_LDS_BYTES) {
@@ -470,16 +491,23 @@ static void tcollect(bpf_engine *e) {
   e->events.clear();
 }
 
-static unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
+
+// mismatch rows over many workgroups, then one verdict workgroup per instance
+static void mismatch_pair(bpf_engine *e, int mode, int maxit, int qlim,
+                          int fd_norm) {
+  const unsigned nparts = div_up(e->S.nb, e->mis_threads);
+  hipLaunchKernelGGL(k_mismatch_mb, dim3(nparts, e->T), dim3(e->mis_threads), 0,
+                     e->stream, e->g, e->d, mode, fd_norm);
+  hipLaunchKernelGGL(k_verdict, dim3(e->T), dim3(256), 0, e->stream, e->g, e->d,
+                     (int)nparts, mode, maxit, e->opt.tol, qlim,
+                     e->opt.max_qlim_rounds, e->opt.qlim_tol, fd_norm);
+}
 
 static void launch_mismatch(bpf_engine *e, int mode) {
   tbegin(e, "mismatch");
   const bool fd = e->opt.alg != BPF_NR;
-  hipLaunchKernelGGL(k_mismatch, dim3(e->T), dim3(e->mis_threads), 0,
-                     e->stream, e->g, e->d, mode,
-                     fd ? e->opt.maxit_fd : e->opt.maxit_nr, e->opt.tol,
-                     fd ? 0 : e->opt.enforce_qlim, e->opt.max_qlim_rounds,
-                     e->opt.qlim_tol, fd ? 1 : 0);
+  mismatch_pair(e, mode, fd ? e->opt.maxit_fd : e->opt.maxit_nr,
+                fd ? 0 : e->opt.enforce_qlim, fd ? 1 : 0);
   tend(e);
 }
 
@@ -492,9 +520,8 @@ static void launch_newton_step(bpf_engine *e) {
   tend(e);
   tbegin(e, "lu_factor");
   hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, 1, e->S.nlev, e->d_lvA,
-                     e->d_tA_dst, e->d_tA_pp, e->d_pairL, e->d_pairU,
-                     e->d_lvB, e->d_tB_dst, e->d_tB_diag, LU);
+                     e->stream, e->g, e->d.st, 1, e->fplan, LU,
+                     (B4<T> *)e->d_TL);
   tend(e);
   size_t xs = sizeof(T) * 2 * (size_t)e->S.nb;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
@@ -502,9 +529,8 @@ static void launch_newton_step(bpf_engine *e) {
   tbegin(e, "lu_solve");
   hipLaunchKernelGGL(k_lu_solve<T>, dim3(e->T), dim3(e->solve_threads),
                      0, e->stream, e->g, e->d,
-                     (const B4<T> *)LU, refine, 0, e->S.nfw, e->d_fw_ptr,
-                     e->d_fw_rows, e->S.nbw, e->d_bw_ptr, e->d_bw_rows,
-                     use_lds, (T *)e->d_xglob);
+                     (const B4<T> *)LU, (const B4<T> *)e->d_TL, refine, 0,
+                     e->splan, use_lds, (T *)e->d_xglob);
   tend(e);
 }
 
@@ -516,8 +542,10 @@ static void launch_iteration(bpf_engine *e) {
                      e->stream, e->g, e->d);
   tend(e);
   launch_mismatch(e, 1);
-  hipMemcpyAsync(e->h_count, e->d.active_count, sizeof(int),
-                 hipMemcpyDeviceToHost, e->stream);
+  // the active-instance count is read outside the captured graph
+  if (!e->capturing)
+    hipMemcpyAsync(e->h_count, e->d.active_count, sizeof(int),
+                   hipMemcpyDeviceToHost, e->stream);
 }
 
 
@@ -555,8 +583,14 @@ static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
   const int cap = e->opt.maxit_nr * (e->opt.max_qlim_rounds + 1) + 2;
   for (int it = 0; it < cap; it++) {
     hipError_t r = hipSuccess;
-    if (use_graph) r = hipGraphLaunch(e->graph, e->stream);
-    else launch_iteration(e);
+    if (use_graph) {
+      r = hipGraphLaunch(e->graph, e->stream);
+      if (r == hipSuccess)
+        r = hipMemcpyAsync(e->h_count, e->d.active_count, sizeof(int),
+                           hipMemcpyDeviceToHost, e->stream);
+    } else {
+      launch_iteration(e);
+    }
     if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
     if (r == hipSuccess) r = hipGetLastError();
     if (r != hipSuccess) {
@@ -594,9 +628,8 @@ static bpf_status run_fd(bpf_engine *e) {
   tend(e);
   tbegin(e, "lu_factor");
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(T), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, 0, e->S.nlev, e->d_lvA,
-                     e->d_tA_dst, e->d_tA_pp, e->d_pairL, e->d_pairU,
-                     e->d_lvB, e->d_tB_dst, e->d_tB_diag, LU);
+                     e->stream, e->g, e->d.st, 0, e->fplan, LU,
+                     (B4<double> *)e->d_TL);
   tend(e);
   size_t xs = sizeof(double) * 2 * (size_t)e->S.nb;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
@@ -604,10 +637,9 @@ static bpf_status run_fd(bpf_engine *e) {
     for (int half = 1; half <= 2; half++) {
       tbegin(e, "fd_solve");
       hipLaunchKernelGGL(k_lu_solve<double>, dim3(T), dim3(e->solve_threads),
-                         0, e->stream, e->g, e->d,
-                         (const B4<double> *)LU, 0, half, e->S.nfw,
-                         e->d_fw_ptr, e->d_fw_rows, e->S.nbw, e->d_bw_ptr,
-                         e->d_bw_rows, use_lds, (double *)e->d_xglob);
+                         0, e->stream, e->g, e->d, (const B4<double> *)LU,
+                         (const B4<double> *)e->d_TL, 0, half, e->splan,
+                         use_lds, (double *)e->d_xglob);
       tend(e);
       tbegin(e, "update");
       hipLaunchKernelGGL(k_fd_update, dim3(div_up(e->S.nb, 256), T), dim3(256),
@@ -615,9 +647,7 @@ static bpf_status run_fd(bpf_engine *e) {
       tend(e);
       tbegin(e, "mismatch");
       hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, e->stream, e->d);
-      hipLaunchKernelGGL(k_mismatch, dim3(T), dim3(e->mis_threads), 0,
-                         e->stream, e->g, e->d, half == 1 ? 3 : 4,
-                         e->opt.maxit_fd, e->opt.tol, 0, 0, 0.0, 1);
+      mismatch_pair(e, half == 1 ? 3 : 4, e->opt.maxit_fd, 0, 1);
       tend(e);
       HIPCHK(hipMemcpyAsync(e->h_count, e->d.active_count, sizeof(int),
                             hipMemcpyDeviceToHost, e->stream));
@@ -728,6 +758,11 @@ static void fill_info(const Symbolic &S, bpf_info *info) {
   info->lu_pairs = S.npairs;
   info->crout_tasks = (int)S.tA_dst.size();
   info->crout_scale_tasks = (int)S.tB_dst.size();
+  info->tail_level = S.tail_level;
+  info->tail_n = S.tail_n;
+  info->tail_entries = (int)S.tl_pos.size();
+  info->tail_pairs = (int)S.tl_pairL.size();
+  info->tail_row_entries = (int)S.ts_pos.size();
 }
 
 extern "C" bpf_status bpf_analyse(const bpf_case *c, bpf_info *info,
@@ -737,7 +772,12 @@ extern "C" bpf_status bpf_analyse(const bpf_case *c, bpf_info *info,
                                   int32_t *pairL, int32_t *pairU, int32_t *lvB,
                                   int32_t *tB_dst, int32_t *tB_diag,
                                   int32_t *fw_ptr, int32_t *fw_rows,
-                                  int32_t *bw_ptr, int32_t *bw_rows) {
+                                  int32_t *bw_ptr, int32_t *bw_rows,
+                                  int32_t *tl_dense, int32_t *tl_pos,
+                                  int32_t *tl_ptr, int32_t *tl_pairL,
+                                  int32_t *tl_pairU, int32_t *tail_piv,
+                                  int32_t *ts_ptr, int32_t *ts_pos,
+                                  int32_t *ts_col) {
   if (!c || !info || c->nbus <= 0 || c->nbranch < 0 || !c->bus_type ||
       (c->nbranch && (!c->br_from || !c->br_to)))
     return BPF_ERR_INVALID_ARG;
@@ -756,6 +796,10 @@ extern "C" bpf_status bpf_analyse(const bpf_case *c, bpf_info *info,
   put(tB_dst, S.tB_dst); put(tB_diag, S.tB_diag);
   put(fw_ptr, S.fw_ptr); put(fw_rows, S.fw_rows);
   put(bw_ptr, S.bw_ptr); put(bw_rows, S.bw_rows);
+  put(tl_dense, S.tl_dense); put(tl_pos, S.tl_pos); put(tl_ptr, S.tl_pp);
+  put(tl_pairL, S.tl_pairL); put(tl_pairU, S.tl_pairU);
+  put(tail_piv, S.tail_piv); put(ts_ptr, S.ts_ptr); put(ts_pos, S.ts_pos);
+  put(ts_col, S.ts_col);
   return BPF_OK;
 }
 
@@ -802,8 +846,7 @@ extern "C" bpf_status bpf_get_ybus(bpf_engine *e, int32_t t, int32_t *rowptr,
 static bpf_status stage_eval(bpf_engine *e, const double *V) {
   if (!e->solved) { e->err = "call bpf_solve first (Ybus is assembled there)"; return BPF_ERR_STATE; }
   HIPCHK(hipMemcpy(e->d.V, V, sizeof(cd) * (size_t)e->T * e->n, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_mismatch, dim3(e->T), dim3(e->mis_threads), 0, e->stream,
-                     e->g, e->d, 2, e->opt.maxit_nr, e->opt.tol, 0, 0, 0.0, 0);
+  mismatch_pair(e, 2, e->opt.maxit_nr, 0, 0);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
   return BPF_OK;
@@ -974,18 +1017,16 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   hipLaunchKernelGGL(k_jacobian<double>, dim3(div_up(e->g.nnzY, 256), TT),
                      dim3(256), 0, e->stream, e->g, e->d, LU);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, 0, e->S.nlev, e->d_lvA,
-                     e->d_tA_dst, e->d_tA_pp, e->d_pairL, e->d_pairU,
-                     e->d_lvB, e->d_tB_dst, e->d_tB_diag, LU);
+                     e->stream, e->g, e->d.st, 0, e->fplan, LU,
+                     (B4<double> *)e->d_TL);
   hipLaunchKernelGGL(k_adj_rhs, dim3(div_up(e->S.nb, 256), TT), dim3(256), 0,
                      e->stream, e->g, e->d, (const double *)d_rate, w, d_G);
   size_t xs = sizeof(double) * nb2;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
   hipLaunchKernelGGL(k_adj_solve, dim3(TT), dim3(e->solve_threads),
-                     0, e->stream, e->g, e->d,
-                     (const B4<double> *)LU, (const double *)d_G, d_lam,
-                     e->S.nfw, e->d_fw_ptr, e->d_fw_rows, e->S.nbw,
-                     e->d_bw_ptr, e->d_bw_rows, use_lds, (double *)e->d_xglob);
+                     0, e->stream, e->g, e->d, (const B4<double> *)LU,
+                     (const B4<double> *)e->d_TL, (const double *)d_G, d_lam,
+                     e->splan, use_lds, (double *)e->d_xglob);
   hipLaunchKernelGGL(k_adj_grad, dim3(div_up(std::max(n, m), 256), TT),
                      dim3(256), 0, e->stream, e->g, e->d,
                      (const double *)d_lam, (const double *)d_rate, w, d_gP,

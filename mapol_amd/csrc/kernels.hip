@@ -174,59 +174,88 @@ __device__ int block_add(int v, int *sh) {
 
 // ---------------------------------------------------------------------------
 // Mismatch S = V o conj(Ybus V) - Sbus, inf-norm, verdicts and reactive-
-// limit switching; one workgroup per instance (newtonpf.h:18-95,394-396,
-// 518-520).  F is kept in block order: F[2k] = P, F[2k+1] = Q (0 for PV).
+// limit switching (newtonpf.h:18-95,394-396,518-520), as two launches:
+// k_mismatch_mb (one thread per block row, many workgroups per instance)
+// writes Ibus, F and one partial norm per workgroup; k_verdict (one
+// workgroup per instance) reduces the partials and decides.  The kernel
+// boundary orders the two: no device-scope fences, which on gfx950 write
+// back and invalidate the per-XCD L2.
+// F is kept in block order: F[2k] = P, F[2k+1] = Q (0 for PV).
 // mode: 0 initial check, 1 after a Newton step, 2 evaluation only,
 // 3 fast decoupled after the P half-step (counts the iteration; converged or
 // go on), 4 fast decoupled after the Q half-step (converged, iteration cap or
 // go on) -- the loop of fdpf.h:505-590.  fd_norm divides by |V| (fdpf.h:401).
 // ---------------------------------------------------------------------------
-__global__ void k_mismatch(DevTopo g, DevInst d, int mode, int maxit,
-                           double tol, int enforce_qlim, int max_rounds,
-                           double qtol, int fd_norm) {
+__device__ void mis_row(const DevTopo &g, const DevInst &d, int t, int k,
+                        int fd_norm, double &lmax, int &lnan) {
+  const cd *Y = d.Y + (size_t)t * g.nnzY;
+  const cd *V = d.V + (size_t)t * g.n;
+  int i = g.bus_of_blk[k];
+  cd I = mk(0.0, 0.0);
+  for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++)
+    I = cadd(I, cmul(Y[p], V[g.ycol[p]]));
+  size_t o = (size_t)t * g.n + i;
+  d.Ib[o] = I;
+  cd mis = csub(cmul(V[i], cconj(I)), mk(d.Pinj[o], d.Qinj[o]));
+  if (fd_norm) mis = cdiv(mis, mk(cabs_(V[i]), 0.0));
+  bool pq = d.type[o] == 1;
+  double *F = d.F + (size_t)t * 2 * g.nb;
+  F[2 * k] = mis.x;
+  F[2 * k + 1] = pq ? mis.y : 0.0;
+  lmax = fmax(lmax, fabs(mis.x));
+  lnan |= isnan(mis.x);
+  if (pq) { lmax = fmax(lmax, fabs(mis.y)); lnan |= isnan(mis.y); }
+}
+
+__global__ void k_mismatch_mb(DevTopo g, DevInst d, int mode, int fd_norm) {
+  __shared__ double shd[16];
+  __shared__ int shi[16];
+  const int t = blockIdx.y;
+  if (mode != 2 && d.st[t].done) return;
+  double lmax = 0.0;
+  int lnan = 0;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < g.nb) mis_row(g, d, t, k, fd_norm, lmax, lnan);
+  double bmax = block_max(lmax, shd);
+  int bnan = block_or(lnan, shi);
+  if (threadIdx.x == 0)
+    d.part[(size_t)t * gridDim.x + blockIdx.x] = bnan ? (double)NAN : bmax;
+}
+
+__global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
+                          double tol, int enforce_qlim, int max_rounds,
+                          double qtol, int fd_norm) {
   __shared__ double shd[16];
   __shared__ int shi[16];
   const int t = blockIdx.x;
   InstState *sp = d.st + t;
   if (mode != 2 && sp->done) return;
+  double pm = 0.0;
+  int pn = 0;
+  for (int q = threadIdx.x; q < nparts; q += blockDim.x) {
+    double v = d.part[(size_t)t * nparts + q];
+    pn |= isnan(v);
+    pm = fmax(pm, v);
+  }
+  double nrm = block_max(pm, shd);
+  int anynan = block_or(pn, shi);
   InstState s = *sp;
-  const cd *Y = d.Y + (size_t)t * g.nnzY;
-  const cd *V = d.V + (size_t)t * g.n;
-  cd *Ib = d.Ib + (size_t)t * g.n;
+  if (mode == 1 || mode == 3) { s.it_round++; s.it_total++; }
   int8_t *ty = d.type + (size_t)t * g.n;
-  double *Pinj = d.Pinj + (size_t)t * g.n;
   double *Qinj = d.Qinj + (size_t)t * g.n;
   double *F = d.F + (size_t)t * 2 * g.nb;
-  if (mode == 1 || mode == 3) { s.it_round++; s.it_total++; }
+  const cd *V = d.V + (size_t)t * g.n;
+  const cd *Ib = d.Ib + (size_t)t * g.n;
   bool active = false;
   for (int guard = 0; guard <= max_rounds + 1; guard++) {
-    double lmax = 0.0;
-    int lnan = 0;
-    for (int k = threadIdx.x; k < g.nb; k += blockDim.x) {
-      int i = g.bus_of_blk[k];
-      cd I = mk(0.0, 0.0);
-      for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++)
-        I = cadd(I, cmul(Y[p], V[g.ycol[p]]));
-      Ib[i] = I;
-      cd mis = csub(cmul(V[i], cconj(I)), mk(Pinj[i], Qinj[i]));
-      if (fd_norm) mis = cdiv(mis, mk(cabs_(V[i]), 0.0));
-      bool pq = ty[i] == 1;
-      F[2 * k] = mis.x;
-      F[2 * k + 1] = pq ? mis.y : 0.0;
-      lmax = fmax(lmax, fabs(mis.x));
-      lnan |= isnan(mis.x);
-      if (pq) { lmax = fmax(lmax, fabs(mis.y)); lnan |= isnan(mis.y); }
-    }
-    double nrm = block_max(lmax, shd);
-    int anynan = block_or(lnan, shi);
     s.norm = anynan ? NAN : nrm;
     if (mode == 2) break;
     if (anynan || isinf(nrm)) { s.verdict = BPF_V_NONFINITE; s.done = 1; break; }
     if (nrm < tol) {
       if (!enforce_qlim) { s.verdict = BPF_V_CONVERGED; s.done = 1; break; }
       int lv = 0;
-      for (int k = threadIdx.x; k < g.nb; k += blockDim.x) {
-        int i = g.bus_of_blk[k];
+      for (int b = threadIdx.x; b < g.nb; b += blockDim.x) {
+        int i = g.bus_of_blk[b];
         if (ty[i] != 2) continue;
         double qg = cmul(V[i], cconj(Ib[i])).y + g.Qd[i];
         size_t o = (size_t)t * g.n + i;
@@ -237,16 +266,31 @@ __global__ void k_mismatch(DevTopo g, DevInst d, int mode, int maxit,
       if (s.rounds >= max_rounds) { s.verdict = BPF_V_MAX_ITER; s.done = 1; break; }
       s.rounds++;
       s.it_round = 0;
-      for (int k = threadIdx.x; k < g.nb; k += blockDim.x) {
-        int i = g.bus_of_blk[k];
+      // switch every violating bus to PQ with its Q at the limit; only those
+      // buses' Q rows change (they were identity rows with F = 0)
+      double lm = 0.0;
+      int ln = 0;
+      for (int b = threadIdx.x; b < g.nb; b += blockDim.x) {
+        int i = g.bus_of_blk[b];
         if (ty[i] != 2) continue;
-        double qg = cmul(V[i], cconj(Ib[i])).y + g.Qd[i];
+        cd S = cmul(V[i], cconj(Ib[i]));
+        double qg = S.y + g.Qd[i];
         size_t o = (size_t)t * g.n + i;
-        if (qg > d.Qmax[o] + qtol) { ty[i] = 1; Qinj[i] = d.Qmax[o] - g.Qd[i]; }
-        else if (qg < d.Qmin[o] - qtol) { ty[i] = 1; Qinj[i] = d.Qmin[o] - g.Qd[i]; }
+        double q;
+        if (qg > d.Qmax[o] + qtol) q = d.Qmax[o] - g.Qd[i];
+        else if (qg < d.Qmin[o] - qtol) q = d.Qmin[o] - g.Qd[i];
+        else continue;
+        ty[i] = 1;
+        Qinj[i] = q;
+        double fq = S.y - q;
+        if (fd_norm) fq = fq / cabs_(V[i]);
+        F[2 * b + 1] = fq;
+        lm = fmax(lm, fabs(fq));
+        ln |= isnan(fq);
       }
-      __syncthreads();
-      continue;  // re-evaluate the mismatch with the new bus types
+      nrm = fmax(nrm, block_max(lm, shd));
+      anynan |= block_or(ln, shi);
+      continue;  // decide again with the new bus types
     }
     if (mode != 3 && s.it_round >= maxit) { s.verdict = BPF_V_MAX_ITER; s.done = 1; break; }
     active = true;
@@ -324,43 +368,175 @@ __device__ __forceinline__ B4<T> blk_mul(const B4<T> &x, const B4<T> &y) {
 // A(p) - sum L*U over its precomputed pair list (the pivot block is stored
 // inverted); phase B scales the level's L blocks by the pivot inverse.
 // ---------------------------------------------------------------------------
+// schedules of the factorization and of the sweeps (device arrays from the
+// symbolic analysis)
+struct FactorPlan {
+  int nlev;                            // Crout levels below the dense tail
+  const int *lvA, *lvA_max, *tA_dst, *tA_pp, *pairL, *pairU;
+  const int *lvB, *tB_dst, *tB_diag;
+  int tn, tnent, tmax;                 // dense tail: pivots, entries, longest list
+  const int *tdense, *tpos, *tpp, *tpairL, *tpairU;
+};
+
+struct SolvePlan {
+  int nfw, nbw;                        // levels over the non-tail rows
+  const int *fw_ptr, *fw_rows, *fw_max, *bw_ptr, *bw_rows, *bw_max;
+  int tn, tsmax;                       // tail pivots, longest tail-row list
+  const int *tpiv, *ts_ptr, *ts_pos, *ts_col, *ts_mir;
+};
+
+// Lanes that share one item (a task's pair list, a row's entries): a power of
+// two up to one wavefront, no more than the longest list of the level needs,
+// and few enough that every item of the level gets its group in one pass
+// when the workgroup is large enough.  Groups are aligned inside a wavefront,
+// so their partial sums combine with cross-lane shuffles.
+__device__ __forceinline__ int lane_group(int items, int maxlen) {
+  int gsz = 1;
+  while (gsz < 64 && gsz < maxlen && items * 2 * gsz <= (int)blockDim.x) gsz <<= 1;
+  return gsz;
+}
+
 template <typename T>
-__global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter, int nlev,
-                            const int *lvA, const int *tA_dst, const int *tA_pp,
-                            const int *pairL, const int *pairU, const int *lvB,
-                            const int *tB_dst, const int *tB_diag, B4<T> *LU) {
+__device__ __forceinline__ T group_sum(T v, int gsz) {
+  for (int off = gsz >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ void group_sum4(B4<T> &v, int gsz) {
+  for (int off = gsz >> 1; off > 0; off >>= 1) {
+    v.a += __shfl_xor(v.a, off, 64);
+    v.b += __shfl_xor(v.b, off, 64);
+    v.c += __shfl_xor(v.c, off, 64);
+    v.d += __shfl_xor(v.d, off, 64);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ B4<T> blk_zero() {
+  B4<T> z;
+  z.a = z.b = z.c = z.d = (T)0;
+  return z;
+}
+
+template <typename T>
+__device__ __forceinline__ bool blk_inv(const B4<T> &a, B4<T> &inv) {
+  T det = a.a * a.d - a.b * a.c;
+  bool bad = !(det != (T)0) || !isfinite((double)det);
+  T r = (T)1 / det;
+  inv.a = a.d * r; inv.b = -a.b * r; inv.c = -a.c * r; inv.d = a.a * r;
+  return bad;
+}
+
+// ---------------------------------------------------------------------------
+// Batched LU of the 2x2-block Jacobian, static pivot order from the symbolic
+// analysis.  One workgroup per instance.
+// Sparse part, level l of the elimination tree (levels below the tail):
+// phase A computes every output of the level's pivots as A(p) - sum L*U over
+// its precomputed pair list, a lane group per output (the pivot block is
+// stored inverted); phase B scales the level's L blocks by the pivot inverse.
+// Dense tail (the top pivots, ancestors of everything else): its Schur
+// complement S = A_SS - L_SN U_NS is assembled in LDS and inverted in place
+// by block Gauss-Jordan with the same pivot order (one barrier per pivot:
+// the next pivot row, column and inverse are captured into double buffers
+// while the current step updates); S^-1 goes to TL for the sweeps.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
+                            FactorPlan P, B4<T> *LU, B4<T> *TL) {
   const int t = blockIdx.x;
   if (st[t].done) return;
   __shared__ int sing;
   if (threadIdx.x == 0) sing = 0;
   B4<T> *A = LU + (size_t)t * g.nnzLU;
   __syncthreads();
-  for (int l = 0; l < nlev; l++) {
-    for (int k = lvA[l] + threadIdx.x; k < lvA[l + 1]; k += blockDim.x) {
-      int dst = tA_dst[k];
-      int p = dst & TASK_POS_MASK;
-      B4<T> acc;
-      if (dst & TASK_HAS_A) acc = A[p];
-      else { acc.a = acc.b = acc.c = acc.d = (T)0; }
-      int q1 = tA_pp[k + 1];
-      for (int q = tA_pp[k]; q < q1; q++) blk_fms(acc, A[pairL[q]], A[pairU[q]]);
-      if (dst & TASK_DIAG) {
-        T det = acc.a * acc.d - acc.b * acc.c;
-        if (!(det != (T)0) || !isfinite((double)det)) sing = 1;
-        T r = (T)1 / det;
-        B4<T> inv;
-        inv.a = acc.d * r; inv.b = -acc.b * r; inv.c = -acc.c * r; inv.d = acc.a * r;
-        A[p] = inv;
-      } else {
-        A[p] = acc;
+  for (int l = 0; l < P.nlev; l++) {
+    const int k0 = P.lvA[l], k1 = P.lvA[l + 1];
+    const int gsz = lane_group(k1 - k0, P.lvA_max[l]);
+    const int lane = threadIdx.x & (gsz - 1);
+    for (int k = k0 + (int)threadIdx.x / gsz; k < k1; k += (int)blockDim.x / gsz) {
+      const int dst = P.tA_dst[k];
+      const int p = dst & TASK_POS_MASK;
+      B4<T> acc = (lane == 0 && (dst & TASK_HAS_A)) ? A[p] : blk_zero<T>();
+      const int q1 = P.tA_pp[k + 1];
+      for (int q = P.tA_pp[k] + lane; q < q1; q += gsz)
+        blk_fms(acc, A[P.pairL[q]], A[P.pairU[q]]);
+      group_sum4(acc, gsz);
+      if (lane == 0) {
+        if (dst & TASK_DIAG) {
+          B4<T> inv;
+          if (blk_inv(acc, inv)) sing = 1;
+          A[p] = inv;
+        } else {
+          A[p] = acc;
+        }
       }
     }
     __syncthreads();
-    for (int k = lvB[l] + threadIdx.x; k < lvB[l + 1]; k += blockDim.x) {
-      int p = tB_dst[k];
-      A[p] = blk_mul(A[p], A[tB_diag[k]]);
+    for (int k = P.lvB[l] + threadIdx.x; k < P.lvB[l + 1]; k += blockDim.x) {
+      int p = P.tB_dst[k];
+      A[p] = blk_mul(A[p], A[P.tB_diag[k]]);
     }
     __syncthreads();
+  }
+  if (P.tn > 0) {
+    __shared__ B4<T> D[TAIL_MAX * TAIL_MAX];
+    __shared__ B4<T> colb[2][TAIL_MAX], rowb[2][TAIL_MAX], pvb[2];
+    const int s = P.tn, ss = s * s;
+    for (int q = threadIdx.x; q < ss; q += blockDim.x) D[q] = blk_zero<T>();
+    __syncthreads();
+    {
+      const int gsz = lane_group(P.tnent, P.tmax);
+      const int lane = threadIdx.x & (gsz - 1);
+      for (int k = (int)threadIdx.x / gsz; k < P.tnent; k += (int)blockDim.x / gsz) {
+        const int dd = P.tdense[k];
+        B4<T> acc = (lane == 0 && (dd & TASK_HAS_A)) ? A[P.tpos[k]] : blk_zero<T>();
+        const int q1 = P.tpp[k + 1];
+        for (int q = P.tpp[k] + lane; q < q1; q += gsz)
+          blk_fms(acc, A[P.tpairL[q]], A[P.tpairU[q]]);
+        group_sum4(acc, gsz);
+        if (lane == 0) D[dd & TASK_POS_MASK] = acc;
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < s; q += blockDim.x) {
+      colb[0][q] = D[q * s];
+      rowb[0][q] = D[q];
+    }
+    if (threadIdx.x == 0) {
+      B4<T> iv;
+      if (blk_inv(D[0], iv)) sing = 1;
+      pvb[0] = iv;
+    }
+    __syncthreads();
+    for (int k = 0; k < s; k++) {
+      const int cur = k & 1, nxt = cur ^ 1;
+      const B4<T> Pk = pvb[cur];
+      for (int q = threadIdx.x; q < ss; q += blockDim.x) {
+        const int i = q / s, j = q - i * s;
+        B4<T> v;
+        if (i == k) {
+          v = (j == k) ? Pk : blk_mul(Pk, rowb[cur][j]);
+        } else if (j == k) {
+          v = blk_zero<T>();
+          blk_fms(v, colb[cur][i], Pk);
+        } else {
+          v = D[q];
+          blk_fms(v, colb[cur][i], blk_mul(Pk, rowb[cur][j]));
+        }
+        D[q] = v;
+        if (j == k + 1) colb[nxt][i] = v;
+        if (i == k + 1) rowb[nxt][j] = v;
+        if (i == k + 1 && j == k + 1) {
+          B4<T> iv;
+          if (blk_inv(v, iv)) sing = 1;
+          pvb[nxt] = iv;
+        }
+      }
+      __syncthreads();
+    }
+    B4<T> *Ti = TL + (size_t)t * ss;
+    for (int q = threadIdx.x; q < ss; q += blockDim.x) Ti[q] = D[q];
   }
   if (threadIdx.x == 0 && sing) {
     InstState s = st[t];
@@ -372,43 +548,125 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter, int nlev,
   }
 }
 
-// forward (unit-lower L) and backward (U, inverted pivots) sweeps on a
-// block vector X (LDS or global, reached through a generic pointer)
+// Tail rows of a sweep: rs = b_S - L_SN y_N (trans: b_S - U_NS^T y_N), then
+// x_S = S^-1 rs (trans: S^-T rs), written into X.  rs: 2 * TAIL_MAX in LDS.
 template <typename T>
-__device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, T *X, int nfw,
-                          const int *fw_ptr, const int *fw_rows, int nbw,
-                          const int *bw_ptr, const int *bw_rows) {
-  for (int l = 0; l < nfw; l++) {
-    for (int r = fw_ptr[l] + threadIdx.x; r < fw_ptr[l + 1]; r += blockDim.x) {
-      int i = fw_rows[r];
-      T x0 = X[2 * i], x1 = X[2 * i + 1];
-      for (int p = g.lrow[i]; p < g.ldiag[i]; p++) {
-        int c = g.lcol[p];
-        B4<T> L = A[p];
-        T y0 = X[2 * c], y1 = X[2 * c + 1];
-        x0 -= L.a * y0 + L.b * y1;
-        x1 -= L.c * y0 + L.d * y1;
+__device__ void tail_rows(const B4<T> *A, const B4<T> *Ti, T *X, T *rs,
+                          const SolvePlan &P, bool trans) {
+  const int s = P.tn;
+  {
+    const int gsz = lane_group(s, P.tsmax);
+    const int lane = threadIdx.x & (gsz - 1);
+    for (int a = (int)threadIdx.x / gsz; a < s; a += (int)blockDim.x / gsz) {
+      T s0 = (T)0, s1 = (T)0;
+      const int q1 = P.ts_ptr[a + 1];
+      for (int q = P.ts_ptr[a] + lane; q < q1; q += gsz) {
+        const int c = P.ts_col[q];
+        const T y0 = X[2 * c], y1 = X[2 * c + 1];
+        if (trans) {
+          const B4<T> U = A[P.ts_mir[q]];
+          s0 += U.a * y0 + U.c * y1;
+          s1 += U.b * y0 + U.d * y1;
+        } else {
+          const B4<T> L = A[P.ts_pos[q]];
+          s0 += L.a * y0 + L.b * y1;
+          s1 += L.c * y0 + L.d * y1;
+        }
       }
-      X[2 * i] = x0;
-      X[2 * i + 1] = x1;
+      s0 = group_sum(s0, gsz);
+      s1 = group_sum(s1, gsz);
+      if (lane == 0) {
+        const int i = P.tpiv[a];
+        rs[2 * a] = X[2 * i] - s0;
+        rs[2 * a + 1] = X[2 * i + 1] - s1;
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const int gsz = lane_group(s, s);
+    const int lane = threadIdx.x & (gsz - 1);
+    for (int a = (int)threadIdx.x / gsz; a < s; a += (int)blockDim.x / gsz) {
+      T s0 = (T)0, s1 = (T)0;
+      for (int b = lane; b < s; b += gsz) {
+        const T r0 = rs[2 * b], r1 = rs[2 * b + 1];
+        if (trans) {
+          const B4<T> M = Ti[b * s + a];
+          s0 += M.a * r0 + M.c * r1;
+          s1 += M.b * r0 + M.d * r1;
+        } else {
+          const B4<T> M = Ti[a * s + b];
+          s0 += M.a * r0 + M.b * r1;
+          s1 += M.c * r0 + M.d * r1;
+        }
+      }
+      s0 = group_sum(s0, gsz);
+      s1 = group_sum(s1, gsz);
+      if (lane == 0) {
+        const int i = P.tpiv[a];
+        X[2 * i] = s0;
+        X[2 * i + 1] = s1;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// forward (unit-lower L) and backward (U, inverted pivots) sweeps on a block
+// vector X (LDS or global, reached through a generic pointer); non-tail rows
+// by levels with a lane group per row, tail rows through S^-1
+template <typename T>
+__device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, const B4<T> *Ti,
+                          T *X, T *rs, const SolvePlan &P) {
+  for (int l = 0; l < P.nfw; l++) {
+    const int r0 = P.fw_ptr[l], r1 = P.fw_ptr[l + 1];
+    const int gsz = lane_group(r1 - r0, P.fw_max[l]);
+    const int lane = threadIdx.x & (gsz - 1);
+    for (int r = r0 + (int)threadIdx.x / gsz; r < r1; r += (int)blockDim.x / gsz) {
+      const int i = P.fw_rows[r];
+      T s0 = (T)0, s1 = (T)0;
+      const int pe = g.ldiag[i];
+      for (int p = g.lrow[i] + lane; p < pe; p += gsz) {
+        const int c = g.lcol[p];
+        const B4<T> L = A[p];
+        const T y0 = X[2 * c], y1 = X[2 * c + 1];
+        s0 += L.a * y0 + L.b * y1;
+        s1 += L.c * y0 + L.d * y1;
+      }
+      s0 = group_sum(s0, gsz);
+      s1 = group_sum(s1, gsz);
+      if (lane == 0) {
+        X[2 * i] -= s0;
+        X[2 * i + 1] -= s1;
+      }
     }
     __syncthreads();
   }
-  for (int l = 0; l < nbw; l++) {
-    for (int r = bw_ptr[l] + threadIdx.x; r < bw_ptr[l + 1]; r += blockDim.x) {
-      int i = bw_rows[r];
-      T x0 = X[2 * i], x1 = X[2 * i + 1];
-      int dg = g.ldiag[i];
-      for (int p = dg + 1; p < g.lrow[i + 1]; p++) {
-        int c = g.lcol[p];
-        B4<T> U = A[p];
-        T y0 = X[2 * c], y1 = X[2 * c + 1];
-        x0 -= U.a * y0 + U.b * y1;
-        x1 -= U.c * y0 + U.d * y1;
+  if (P.tn > 0) tail_rows<T>(A, Ti, X, rs, P, false);
+  for (int l = 0; l < P.nbw; l++) {
+    const int r0 = P.bw_ptr[l], r1 = P.bw_ptr[l + 1];
+    const int gsz = lane_group(r1 - r0, P.bw_max[l]);
+    const int lane = threadIdx.x & (gsz - 1);
+    for (int r = r0 + (int)threadIdx.x / gsz; r < r1; r += (int)blockDim.x / gsz) {
+      const int i = P.bw_rows[r];
+      const int dg = g.ldiag[i];
+      const B4<T> Dg = (lane == 0) ? A[dg] : blk_zero<T>();
+      T s0 = (T)0, s1 = (T)0;
+      const int pe = g.lrow[i + 1];
+      for (int p = dg + 1 + lane; p < pe; p += gsz) {
+        const int c = g.lcol[p];
+        const B4<T> U = A[p];
+        const T y0 = X[2 * c], y1 = X[2 * c + 1];
+        s0 += U.a * y0 + U.b * y1;
+        s1 += U.c * y0 + U.d * y1;
       }
-      B4<T> D = A[dg];
-      X[2 * i] = D.a * x0 + D.b * x1;
-      X[2 * i + 1] = D.c * x0 + D.d * x1;
+      s0 = group_sum(s0, gsz);
+      s1 = group_sum(s1, gsz);
+      if (lane == 0) {
+        const T x0 = X[2 * i] - s0, x1 = X[2 * i + 1] - s1;
+        X[2 * i] = Dg.a * x0 + Dg.b * x1;
+        X[2 * i + 1] = Dg.c * x0 + Dg.d * x1;
+      }
     }
     __syncthreads();
   }
@@ -428,15 +686,16 @@ __device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, T *X, int nfw,
 constexpr int SOLVE_LDS_BYTES = 150 * 1024;
 
 template <typename T>
-__global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, int refine,
-                           int rhs_mode, int nfw, const int *fw_ptr, const int *fw_rows,
-                           int nbw, const int *bw_ptr, const int *bw_rows,
-                           int use_lds, T *xglob) {
+__global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
+                           const B4<T> *TL, int refine, int rhs_mode,
+                           SolvePlan P, int use_lds, T *xglob) {
   __shared__ T smem[SOLVE_LDS_BYTES / sizeof(T)];
+  __shared__ T rs[2 * TAIL_MAX];
   const int t = blockIdx.x;
   if (d.st[t].done) return;
   T *X = use_lds ? smem : xglob + (size_t)t * 2 * g.nb;
   const B4<T> *A = LU + (size_t)t * g.nnzLU;
+  const B4<T> *Ti = TL + (size_t)t * P.tn * P.tn;
   const double *F = d.F + (size_t)t * 2 * g.nb;
   double *dx = d.dx + (size_t)t * 2 * g.nb;
   const cd *Y = d.Y + (size_t)t * g.nnzY;
@@ -469,7 +728,7 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, int refine,
       }
     }
     __syncthreads();
-    lu_sweeps<T>(g, A, X, nfw, fw_ptr, fw_rows, nbw, bw_ptr, bw_rows);
+    lu_sweeps<T>(g, A, Ti, X, rs, P);
     for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
       dx[k] = (s == 0 ? 0.0 : dx[k]) + (double)X[k];
     __syncthreads();
@@ -655,21 +914,22 @@ __global__ void k_adj_rhs(DevTopo g, DevInst d, const double *rate, double w,
 // reached through the mirror map), then L^T lambda = z (backward levels).
 // One workgroup per instance.
 __global__ void k_adj_solve(DevTopo g, DevInst d, const B4<double> *LU,
-                            const double *G, double *lam, int nfw,
-                            const int *fw_ptr, const int *fw_rows, int nbw,
-                            const int *bw_ptr, const int *bw_rows, int use_lds,
-                            double *xglob) {
+                            const B4<double> *TL, const double *G, double *lam,
+                            SolvePlan P, int use_lds, double *xglob) {
   __shared__ double smem[SOLVE_LDS_BYTES / sizeof(double)];
+  __shared__ double rs[2 * TAIL_MAX];
   const int t = blockIdx.x;
   if (d.st[t].done) return;
   double *X = use_lds ? smem : xglob + (size_t)t * 2 * g.nb;
   const B4<double> *A = LU + (size_t)t * g.nnzLU;
+  const B4<double> *Ti = TL + (size_t)t * P.tn * P.tn;
   const double *Gt = G + (size_t)t * 2 * g.nb;
   for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) X[k] = Gt[k];
   __syncthreads();
-  for (int l = 0; l < nfw; l++) {
-    for (int r = fw_ptr[l] + threadIdx.x; r < fw_ptr[l + 1]; r += blockDim.x) {
-      int i = fw_rows[r];
+  // U^T z = g over the non-tail rows (forward levels)
+  for (int l = 0; l < P.nfw; l++) {
+    for (int r = P.fw_ptr[l] + threadIdx.x; r < P.fw_ptr[l + 1]; r += blockDim.x) {
+      int i = P.fw_rows[r];
       double x0 = X[2 * i], x1 = X[2 * i + 1];
       for (int p = g.lrow[i]; p < g.ldiag[i]; p++) {
         int c = g.lcol[p];
@@ -684,9 +944,12 @@ __global__ void k_adj_solve(DevTopo g, DevInst d, const B4<double> *LU,
     }
     __syncthreads();
   }
-  for (int l = 0; l < nbw; l++) {
-    for (int r = bw_ptr[l] + threadIdx.x; r < bw_ptr[l + 1]; r += blockDim.x) {
-      int i = bw_rows[r];
+  // tail: lambda_S = S^-T (z_S - U_NS^T z_N)
+  if (P.tn > 0) tail_rows<double>(A, Ti, X, rs, P, true);
+  // L^T lambda = z over the non-tail rows (backward levels)
+  for (int l = 0; l < P.nbw; l++) {
+    for (int r = P.bw_ptr[l] + threadIdx.x; r < P.bw_ptr[l + 1]; r += blockDim.x) {
+      int i = P.bw_rows[r];
       double x0 = X[2 * i], x1 = X[2 * i + 1];
       for (int p = g.ldiag[i] + 1; p < g.lrow[i + 1]; p++) {
         int c = g.lcol[p];
@@ -802,9 +1065,9 @@ __global__ void k_loss(DevTopo g, DevInst d, cd *Sf, cd *St, double *loss) {
 // explicit instantiations used by the engine
 template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *);
 template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *);
-template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, int, const int *, const int *, const int *, const int *, const int *, const int *, const int *, const int *, B4<float> *);
-template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, int, const int *, const int *, const int *, const int *, const int *, const int *, const int *, const int *, B4<double> *);
-template __global__ void k_lu_solve<float>(DevTopo, DevInst, const B4<float> *, int, int, int, const int *, const int *, int, const int *, const int *, int, float *);
-template __global__ void k_lu_solve<double>(DevTopo, DevInst, const B4<double> *, int, int, int, const int *, const int *, int, const int *, const int *, int, double *);
+template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *);
+template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *);
+template __global__ void k_lu_solve<float>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, float *);
+template __global__ void k_lu_solve<double>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, double *);
 
 }  // namespace bpf

@@ -57,6 +57,8 @@ struct DevInst {
   double *dx;              // [T][2nb]
   InstState *st;           // [T]
   int *active_count;       // [1]
+  // partial mismatch norms, one per mismatch workgroup (NaN: a NaN entry)
+  double *part;            // [T][nb / 64 rounded up]
 };
 
 }  // namespace bpf

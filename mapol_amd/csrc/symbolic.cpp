@@ -268,6 +268,7 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     std::stable_sort(tasks.begin(), tasks.end(), [](const Task &a, const Task &b) {
       return a.L.size() > b.L.size();
     });
+    S.lvA_max.push_back(tasks.empty() ? 0 : (int)tasks.front().L.size());
     for (auto &t : tasks) {
       int p = t.dst & TASK_POS_MASK;
       int flags = t.dst & ~TASK_POS_MASK;
@@ -281,25 +282,102 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     S.lvB.push_back((int)S.tB_dst.size());
   }
   S.npairs = (int64_t)S.pairL.size();
-  // ---- triangular-solve levels ----------------------------------------------
+  // ---- dense tail ------------------------------------------------------------
+  {
+    std::vector<int> cnt(S.nlev + 1, 0);  // pivots at levels >= l
+    for (int k = 0; k < nb; k++) cnt[lev[k]]++;
+    for (int l = S.nlev - 1; l >= 0; l--) cnt[l] += cnt[l + 1];
+    int Ls = S.nlev;
+    while (Ls > 0 && cnt[Ls - 1] <= TAIL_MAX) Ls--;
+    // worth it only when it removes a few levels
+    if (S.nlev - Ls < 4) Ls = S.nlev;
+    S.tail_level = Ls;
+    std::vector<int> loc(nb, -1);
+    for (int k = 0; k < nb; k++)
+      if (lev[k] >= Ls) { loc[k] = (int)S.tail_piv.size(); S.tail_piv.push_back(k); }
+    const int s = S.tail_n = (int)S.tail_piv.size();
+    S.tl_pp.assign(1, 0);
+    for (int k : S.tail_piv) {
+      const auto &sk = st[k];
+      // outputs of tail pivot k: (k,k), (k,j), (j,k) for j in st[k] (all in
+      // the tail: ancestors of a tail pivot are tail pivots)
+      std::vector<std::pair<int, int>> outs;  // (row, col)
+      outs.push_back({k, k});
+      for (int j : sk) { outs.push_back({k, j}); outs.push_back({j, k}); }
+      for (auto &rc : outs) {
+        int r = rc.first, c = rc.second;
+        int p = pos(r, c);
+        int flags = inJ[p] ? TASK_HAS_A : 0;
+        S.tl_dense.push_back((loc[r] * s + loc[c]) | flags);
+        S.tl_pos.push_back(p);
+        // pairs over non-tail pivots m common to Lrow(r) and Lrow(c)
+        int lo = std::min(r, c);
+        for (int mm : lrowv[lo]) {
+          if (loc[mm] >= 0) continue;
+          int hi = std::max(r, c);
+          // m in Lrow(hi) <=> hi in st[m]
+          if (!std::binary_search(st[mm].begin(), st[mm].end(), hi)) continue;
+          S.tl_pairL.push_back(pos(r, mm));
+          S.tl_pairU.push_back(pos(mm, c));
+        }
+        S.tl_pp.push_back((int)S.tl_pairL.size());
+      }
+    }
+  }
+  for (int q = 0; q + 1 < (int)S.tl_pp.size(); q++)
+    S.tl_max = std::max(S.tl_max, S.tl_pp[q + 1] - S.tl_pp[q]);
+  // ---- tail rows in the sweeps ----------------------------------------------
+  std::vector<char> intail(nb, 0);
+  for (int k : S.tail_piv) intail[k] = 1;
+  S.ts_ptr.assign(1, 0);
+  for (int k : S.tail_piv) {
+    for (int p = S.lrow[k]; p < S.ldiag[k]; p++) {
+      int c = S.lcol[p];
+      if (intail[c]) continue;
+      S.ts_pos.push_back(p);
+      S.ts_col.push_back(c);
+      S.ts_mir.push_back(S.mirror[p]);
+    }
+    S.ts_ptr.push_back((int)S.ts_pos.size());
+    S.ts_max = std::max(S.ts_max, S.ts_ptr.back() - S.ts_ptr[S.ts_ptr.size() - 2]);
+  }
+  // ---- triangular-solve levels (non-tail rows) -------------------------------
+  // a tail row is never a dependency of a non-tail row in the forward sweep
+  // (its descendants are non-tail) and is solved before the backward sweep
   std::vector<int> lf(nb, 0), lb(nb, 0);
   for (int i = 0; i < nb; i++)
     for (int k : lrowv[i]) lf[i] = std::max(lf[i], lf[k] + 1);
-  for (int i = nb - 1; i >= 0; i--)
+  for (int i = nb - 1; i >= 0; i--) {
+    if (intail[i]) { lb[i] = -1; continue; }
     for (int j : st[i]) lb[i] = std::max(lb[i], lb[j] + 1);
-  S.nfw = 1 + *std::max_element(lf.begin(), lf.end());
-  S.nbw = 1 + *std::max_element(lb.begin(), lb.end());
+  }
+  S.nfw = S.nbw = 0;
+  for (int i = 0; i < nb; i++)
+    if (!intail[i]) { S.nfw = std::max(S.nfw, lf[i] + 1); S.nbw = std::max(S.nbw, lb[i] + 1); }
   auto group = [&](const std::vector<int> &lv, int nl, std::vector<int> &ptr,
-                   std::vector<int> &rows) {
+                   std::vector<int> &rows, std::vector<int> &mx,
+                   const std::vector<int> &len) {
     ptr.assign(nl + 1, 0);
-    for (int i = 0; i < nb; i++) ptr[lv[i] + 1]++;
+    for (int i = 0; i < nb; i++) if (!intail[i]) ptr[lv[i] + 1]++;
     for (int l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
-    rows.resize(nb);
+    rows.resize(ptr[nl]);
     std::vector<int> w(ptr.begin(), ptr.end() - 1);
-    for (int i = 0; i < nb; i++) rows[w[lv[i]]++] = i;
+    for (int i = 0; i < nb; i++) if (!intail[i]) rows[w[lv[i]]++] = i;
+    mx.assign(nl, 0);
+    for (int l = 0; l < nl; l++) {
+      // longest rows first: they lead the lane groups of the level
+      std::stable_sort(rows.begin() + ptr[l], rows.begin() + ptr[l + 1],
+                       [&](int a, int b) { return len[a] > len[b]; });
+      if (ptr[l + 1] > ptr[l]) mx[l] = len[rows[ptr[l]]];
+    }
   };
-  group(lf, S.nfw, S.fw_ptr, S.fw_rows);
-  group(lb, S.nbw, S.bw_ptr, S.bw_rows);
+  std::vector<int> lenL(nb), lenU(nb);
+  for (int i = 0; i < nb; i++) {
+    lenL[i] = S.ldiag[i] - S.lrow[i];
+    lenU[i] = S.lrow[i + 1] - S.ldiag[i] - 1;
+  }
+  group(lf, S.nfw, S.fw_ptr, S.fw_rows, S.fw_max, lenL);
+  group(lb, S.nbw, S.bw_ptr, S.bw_rows, S.bw_max, lenU);
   return true;
 }
 

@@ -15,6 +15,7 @@ enum { KFF = 0, KFT = 1, KTF = 2, KTT = 3 };
 constexpr int TASK_HAS_A = 1 << 30;  // the position holds a Jacobian entry
 constexpr int TASK_DIAG = 1 << 29;   // pivot block: store its inverse
 constexpr int TASK_POS_MASK = (1 << 29) - 1;
+constexpr int TAIL_MAX = 64;         // dense tail: at most 64 blocks square
 
 struct Symbolic {
   int n = 0, m = 0, nb = 0;
@@ -36,11 +37,30 @@ struct Symbolic {
   // Crout factorization schedule (one level = one elimination-tree height)
   int nlev = 0;
   std::vector<int> lvA, tA_dst, tA_pp, pairL, pairU;  // phase A: dot products
+  std::vector<int> lvA_max;                           // longest pair list per level
   std::vector<int> lvB, tB_dst, tB_diag;              // phase B: L scaling
   int64_t npairs = 0;
-  // triangular solves: rows grouped by dependency level
+  // dense tail: the pivots of the top levels (>= tail_level, at most
+  // TAIL_MAX of them, ascending = topological) are factored as one dense
+  // block in LDS.  Its entries start from A minus the contributions of the
+  // non-tail pivots (pairs below), then a right-looking dense LU finishes.
+  int tail_level = 0;     // sparse levels 0 .. tail_level-1
+  int tail_n = 0;         // pivots in the tail
+  std::vector<int> tail_piv;                 // tail pivots, ascending
+  std::vector<int> tl_dense, tl_pos, tl_pp;  // per tail entry (flags in tl_dense)
+  std::vector<int> tl_pairL, tl_pairU;
+  int tl_max = 0;                            // longest tail pair list
+  // The factorization leaves the inverse of the tail's Schur complement
+  // (dense, tail_n x tail_n blocks); the sweeps solve the non-tail rows by
+  // levels and the tail rows with that inverse.  Tail row a (block
+  // tail_piv[a]) first subtracts its L entries in non-tail columns:
+  std::vector<int> ts_ptr, ts_pos, ts_col, ts_mir;  // position, column, mirror
+  int ts_max = 0;
+  // triangular solves: non-tail rows grouped by dependency level (tail rows
+  // count as solved before the backward sweep), longest rows first
   int nfw = 0, nbw = 0;
   std::vector<int> fw_ptr, fw_rows, bw_ptr, bw_rows;
+  std::vector<int> fw_max, bw_max;           // longest row per level
 };
 
 // bus_type: MATPOWER codes; buses of type PV or PQ are unknowns, every other

@@ -64,7 +64,9 @@ class BpfInfo(C.Structure):
                 ("bw_levels", C.c_int32), ("lu_pairs", C.c_int64),
                 ("instances", C.c_int32), ("lds_solve", C.c_int32),
                 ("crout_tasks", C.c_int32), ("crout_scale_tasks", C.c_int32),
-                ("graph_used", C.c_int32)]
+                ("graph_used", C.c_int32), ("tail_level", C.c_int32),
+                ("tail_n", C.c_int32), ("tail_entries", C.c_int32),
+                ("tail_pairs", C.c_int32), ("tail_row_entries", C.c_int32)]
 
 
 _LIB = None
@@ -107,7 +109,7 @@ def load_library(path: str = LIB_PATH):
     L.bpf_set_timing.argtypes = [V, C.c_int32]
     L.bpf_get_kernel_time.argtypes = [V, C.c_char_p, _pi, _pd]
     L.bpf_set_launch.argtypes = [V, C.c_int32, C.c_int32, C.c_int32]
-    L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 15
+    L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 24
     for name in EXPORTS:
         getattr(L, name)
     _LIB = L
@@ -146,7 +148,7 @@ def analyse(case):
     keep = []
     s = _case_struct(case, keep)
     info = BpfInfo()
-    st = L.bpf_analyse(C.byref(s), C.byref(info), *([None] * 15))
+    st = L.bpf_analyse(C.byref(s), C.byref(info), *([None] * 24))
     if st != 0:
         raise EngineError(f"bpf_analyse failed ({st})")
     i = {f: getattr(info, f) for f, _ in BpfInfo._fields_}
@@ -160,11 +162,23 @@ def analyse(case):
         lvB=np.zeros(nl + 1, np.int32),
         tB_dst=np.zeros(i["crout_scale_tasks"], np.int32),
         tB_diag=np.zeros(i["crout_scale_tasks"], np.int32),
-        fw_ptr=np.zeros(i["fw_levels"] + 1, np.int32), fw_rows=np.zeros(nb, np.int32),
-        bw_ptr=np.zeros(i["bw_levels"] + 1, np.int32), bw_rows=np.zeros(nb, np.int32))
+        fw_ptr=np.zeros(i["fw_levels"] + 1, np.int32),
+        fw_rows=np.zeros(nb - i["tail_n"], np.int32),
+        bw_ptr=np.zeros(i["bw_levels"] + 1, np.int32),
+        bw_rows=np.zeros(nb - i["tail_n"], np.int32),
+        tl_dense=np.zeros(i["tail_entries"], np.int32),
+        tl_pos=np.zeros(i["tail_entries"], np.int32),
+        tl_ptr=np.zeros(i["tail_entries"] + 1, np.int32),
+        tl_pairL=np.zeros(i["tail_pairs"], np.int32),
+        tl_pairU=np.zeros(i["tail_pairs"], np.int32),
+        tail_piv=np.zeros(i["tail_n"], np.int32),
+        ts_ptr=np.zeros(i["tail_n"] + 1, np.int32),
+        ts_pos=np.zeros(i["tail_row_entries"], np.int32),
+        ts_col=np.zeros(i["tail_row_entries"], np.int32))
     order = ["bus_of_blk", "lu_rowptr", "lu_colind", "lvA", "tA_dst", "tA_ptr",
              "pairL", "pairU", "lvB", "tB_dst", "tB_diag", "fw_ptr", "fw_rows",
-             "bw_ptr", "bw_rows"]
+             "bw_ptr", "bw_rows", "tl_dense", "tl_pos", "tl_ptr", "tl_pairL",
+             "tl_pairU", "tail_piv", "ts_ptr", "ts_pos", "ts_col"]
     st = L.bpf_analyse(C.byref(s), C.byref(info),
                        *[_ptr(out[k], _pi) for k in order])
     if st != 0:

@@ -18,6 +18,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBS = {}
 REF_SO = os.path.join(_HERE, "_ref", "libpfref.so")
+_SO = {"own": "liborcpf.so", "x": "liborcpf_x.so"}
 
 PQ, PV, SLACK = 1, 2, 3
 NR, FDXB, FDBX = 0, 1, 2
@@ -60,7 +61,7 @@ class OrcResult(C.Structure):
 
 def build(force: bool = False) -> str:
     so = os.path.join(_HERE, "liborcpf.so")
-    if force or not os.path.exists(so):
+    if force or not all(os.path.exists(os.path.join(_HERE, f)) for f in _SO.values()):
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"],
                        check=True)
     return so
@@ -73,14 +74,15 @@ def have_ref() -> bool:
 
 def lib(which: str = "own"):
     """``own``: the oracle with its own sparse LU; ``ref``: the build whose
-    Newton steps go through the reference's Eigen SparseLU."""
+    Newton steps go through the reference's Eigen SparseLU; ``x``: the own
+    oracle compiled in 80-bit long double (FP64 arrays at the interface)."""
     if which not in _LIBS:
         if which == "ref":
             so = REF_SO
             if not os.path.exists(so):
                 raise FileNotFoundError(so)
         else:
-            so = os.path.join(_HERE, "liborcpf.so")
+            so = os.path.join(_HERE, _SO[which])
             if not os.path.exists(so):
                 build()
         L = C.CDLL(so)
@@ -100,6 +102,10 @@ def lib(which: str = "own"):
         L.orc_sparse_solve.argtypes = [C.c_int, _pi, _pi, _pd, _pd, _pd,
                                        C.c_double]
         L.orc_min_degree.argtypes = [C.c_int, _pi, _pi, _pi]
+        L.orc_solve_objective.argtypes = [C.POINTER(OrcCase), C.POINTER(OrcInst),
+                                          C.POINTER(OrcOpts), _pd, C.c_double,
+                                          C.POINTER(OrcResult), _pd, _pd]
+        L.orc_precision_bits.restype = C.c_int
         _LIBS[which] = L
     return _LIBS[which]
 
@@ -175,6 +181,24 @@ class Oracle:
                     iterations=r.iterations, verdict=r.verdict,
                     qlim_rounds=r.qlim_rounds, converged=bool(r.converged),
                     norm=r.norm, loss=r.loss)
+
+    def solve_objective(self, opts=None, rate=None, w=0.0, **inst):
+        """solve, then the objective at the solution in the working precision
+        (returned unrounded as the pair (hi, lo), objective = hi + lo)"""
+        o = opts or self.opts()
+        n = self.case.nbus
+        V = np.zeros(2 * n)
+        bt = np.zeros(n, dtype=np.int32)
+        r = OrcResult(_p(V, _pd), _p(bt, _pi), 0, 0, 0, 0, 0.0, 0.0)
+        hi, lo = np.zeros(1), np.zeros(1)
+        rt = None if rate is None else np.ascontiguousarray(rate, np.float64)
+        i, keep = self._inst(**inst)
+        lib(self.which).orc_solve_objective(C.byref(self.s), C.byref(i), C.byref(o),
+                                            _p(rt, _pd), w, C.byref(r),
+                                            _p(hi, _pd), _p(lo, _pd))
+        return dict(V=V[0::2] + 1j * V[1::2], iterations=r.iterations,
+                    verdict=r.verdict, converged=bool(r.converged), norm=r.norm,
+                    f_hi=float(hi[0]), f_lo=float(lo[0]))
 
     def ybus(self, **inst):
         n = self.case.nbus

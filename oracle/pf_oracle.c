@@ -1,8 +1,9 @@
 /*
  * CPU oracle for the batched AC power-flow path (TEST INFRASTRUCTURE ONLY;
  * see pf_oracle.h for the scope rule and the reference functions restated).
- * FP64 throughout; compiled with -ffp-contract=off so every expression is
- * evaluated exactly as written.
+ * FP64 throughout (an 80-bit build of the same source exists for gradient
+ * checks, see ORC_EXTENDED below); compiled with -ffp-contract=off so every
+ * expression is evaluated exactly as written.
  */
 #include "pf_oracle.h"
 
@@ -11,32 +12,59 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Working precision.  The default build is FP64 like the reference; the
+ * ORC_EXTENDED build (liborcpf_x.so) runs the same code in x87 80-bit long
+ * double, used only where the tests need a re-solve whose rounding is far
+ * below FP64 (finite-difference checks of gradients).  The public entry
+ * points keep FP64 arrays in both builds. */
+#ifdef ORC_EXTENDED
+typedef long double real;
+#define R_hypot hypotl
+#define R_fabs fabsl
+#define R_cos cosl
+#define R_sin sinl
+#define R_atan atanl
+#define R_sqrt sqrtl
+#define R_fmax fmaxl
+#define R_PI M_PIl
+#else
+typedef double real;
+#define R_hypot hypot
+#define R_fabs fabs
+#define R_cos cos
+#define R_sin sin
+#define R_atan atan
+#define R_sqrt sqrt
+#define R_fmax fmax
+#define R_PI M_PI
+#endif
+
 /* ------------------------------------------------------------------------- */
 /* complex helpers (the reference uses cuComplex; these are written from the
  * definitions; division is Smith's 1962 scaled algorithm)                     */
 /* ------------------------------------------------------------------------- */
-typedef struct { double re, im; } cpx;
+typedef struct { real re, im; } cpx;
 
-static cpx c_make(double re, double im) { cpx z; z.re = re; z.im = im; return z; }
+static cpx c_make(real re, real im) { cpx z; z.re = re; z.im = im; return z; }
 static cpx c_add(cpx a, cpx b) { return c_make(a.re + b.re, a.im + b.im); }
 static cpx c_sub(cpx a, cpx b) { return c_make(a.re - b.re, a.im - b.im); }
 static cpx c_mul(cpx a, cpx b) {
   return c_make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 static cpx c_conj(cpx a) { return c_make(a.re, -a.im); }
-static double c_abs(cpx a) { return hypot(a.re, a.im); }
+static real c_abs(cpx a) { return R_hypot(a.re, a.im); }
 static cpx c_div(cpx n, cpx d) {
-  if (fabs(d.re) >= fabs(d.im)) {
-    double r = d.im / d.re, den = d.re + d.im * r;
+  if (R_fabs(d.re) >= R_fabs(d.im)) {
+    real r = d.im / d.re, den = d.re + d.im * r;
     return c_make((n.re + n.im * r) / den, (n.im - n.re * r) / den);
   } else {
-    double r = d.re / d.im, den = d.re * r + d.im;
+    real r = d.re / d.im, den = d.re * r + d.im;
     return c_make((n.re * r + n.im) / den, (n.im * r - n.re) / den);
   }
 }
-static cpx c_expj(double th) { return c_make(cos(th), sin(th)); }
-/* the reference's angle is atan(y/x), not atan2 (complexUtils.h:21-23) */
-static double c_angle(cpx a) { return atan(a.im / a.re); }
+static cpx c_expj(real th) { return c_make(R_cos(th), R_sin(th)); }
+/* the reference's angle is R_atan(y/x), not atan2 (complexUtils.h:21-23) */
+static real c_angle(cpx a) { return R_atan(a.im / a.re); }
 
 /* ------------------------------------------------------------------------- */
 /* per-instance resolved data                                                 */
@@ -44,15 +72,15 @@ static double c_angle(cpx a) { return atan(a.im / a.re); }
 typedef struct {
   int n, m, g;
   int *type;        /* effective bus type */
-  double *P, *Q;    /* net injections (pu) */
-  double *vset;     /* set-point magnitude used for V0 */
+  real *P, *Q;    /* net injections (pu) */
+  real *vset;     /* set-point magnitude used for V0 */
   unsigned char *on;/* effective branch status */
-  double *tap;      /* effective tap ratio (tap field, 0 => 1) */
-  double *shift;    /* degrees */
-  double *R, *X, *B;
-  double *Gs, *Bs;
-  double *Qmax, *Qmin; /* per bus, summed over in-service generators */
-  double *Qd;
+  real *tap;      /* effective tap ratio (tap field, 0 => 1) */
+  real *shift;    /* degrees */
+  real *R, *X, *B;
+  real *Gs, *Bs;
+  real *Qmax, *Qmin; /* per bus, summed over in-service generators */
+  real *Qd;
 } inst_t;
 
 static void inst_free(inst_t *I) {
@@ -65,23 +93,23 @@ static void inst_build(const orc_case *c, const orc_inst *in, inst_t *I) {
   int n = c->nbus, m = c->nbranch, g = c->ngen;
   I->n = n; I->m = m; I->g = g;
   I->type = (int *)malloc(sizeof(int) * n);
-  I->P = (double *)calloc(n, sizeof(double));
-  I->Q = (double *)calloc(n, sizeof(double));
-  I->vset = (double *)malloc(sizeof(double) * n);
+  I->P = (real *)calloc(n, sizeof(real));
+  I->Q = (real *)calloc(n, sizeof(real));
+  I->vset = (real *)malloc(sizeof(real) * n);
   I->on = (unsigned char *)malloc(m);
-  I->tap = (double *)malloc(sizeof(double) * m);
-  I->shift = (double *)malloc(sizeof(double) * m);
-  I->R = (double *)malloc(sizeof(double) * m);
-  I->X = (double *)malloc(sizeof(double) * m);
-  I->B = (double *)malloc(sizeof(double) * m);
-  I->Gs = (double *)malloc(sizeof(double) * n);
-  I->Bs = (double *)malloc(sizeof(double) * n);
-  I->Qmax = (double *)calloc(n, sizeof(double));
-  I->Qmin = (double *)calloc(n, sizeof(double));
-  I->Qd = (double *)malloc(sizeof(double) * n);
+  I->tap = (real *)malloc(sizeof(real) * m);
+  I->shift = (real *)malloc(sizeof(real) * m);
+  I->R = (real *)malloc(sizeof(real) * m);
+  I->X = (real *)malloc(sizeof(real) * m);
+  I->B = (real *)malloc(sizeof(real) * m);
+  I->Gs = (real *)malloc(sizeof(real) * n);
+  I->Bs = (real *)malloc(sizeof(real) * n);
+  I->Qmax = (real *)calloc(n, sizeof(real));
+  I->Qmin = (real *)calloc(n, sizeof(real));
+  I->Qd = (real *)malloc(sizeof(real) * n);
   int *ngen_on = (int *)calloc(n, sizeof(int));
   int *ngen_any = (int *)calloc(n, sizeof(int));
-  double *vg = (double *)malloc(sizeof(double) * n);
+  real *vg = (real *)malloc(sizeof(real) * n);
   for (int i = 0; i < n; i++) {
     I->type[i] = c->bus_type[i];
     I->P[i] = -c->Pd[i];
@@ -108,14 +136,14 @@ static void inst_build(const orc_case *c, const orc_inst *in, inst_t *I) {
   for (int i = 0; i < n; i++) {
     /* a PV bus whose generators are all out becomes PQ */
     if (I->type[i] == ORC_PV && ngen_on[i] == 0) I->type[i] = ORC_PQ;
-    double v = c->Vm[i];
+    real v = c->Vm[i];
     if (c->bus_type[i] == ORC_PV && !isnan(vg[i])) v = vg[i];
     if (in && in->vset && !isnan(in->vset[i])) v = in->vset[i];
     I->vset[i] = v;
   }
   for (int b = 0; b < m; b++) {
     I->on[b] = c->br_status[b] && (!in || !in->br_on || in->br_on[b]);
-    double t = c->tap[b];
+    real t = c->tap[b];
     if (t != 0.0 && in && in->tap && !isnan(in->tap[b])) t = in->tap[b];
     I->tap[b] = t;
     I->shift[b] = c->shift[b];
@@ -143,11 +171,11 @@ static void ybus_free(ybus_t *Y) {
 /* pi-model of one branch (makeYbus.h:63-80) */
 static void branch_pi(const inst_t *I, int b, cpx *Yff, cpx *Yft, cpx *Ytf,
                       cpx *Ytt) {
-  double stat = I->on[b] ? 1.0 : 0.0;
+  real stat = I->on[b] ? 1.0 : 0.0;
   cpx Ys = c_div(c_make(stat, 0.0), c_make(I->R[b], I->X[b]));
   cpx Bc = c_mul(c_make(stat, 0.0), c_make(0.0, I->B[b]));
   cpx tap = (I->tap[b] != 0.0) ? c_make(I->tap[b], 0.0) : c_make(1.0, 0.0);
-  tap = c_mul(tap, c_expj(M_PI / 180.0 * I->shift[b]));
+  tap = c_mul(tap, c_expj(R_PI / 180.0 * I->shift[b]));
   *Ytt = c_add(Ys, c_div(Bc, c_make(2.0, 0.0)));
   *Yff = c_div(*Ytt, c_mul(tap, c_conj(tap)));
   *Yft = c_div(c_mul(Ys, c_make(-1.0, 0.0)), c_conj(tap));
@@ -264,7 +292,7 @@ static void order_free(order_t *O) { free(O->pv); free(O->pq); free(O->idx); }
 /* ------------------------------------------------------------------------- */
 static void initial_voltage(const orc_case *c, const inst_t *I, cpx *V) {
   for (int i = 0; i < I->n; i++) {
-    double Vbus = I->vset[i];
+    real Vbus = I->vset[i];
     V[i] = c_mul(c_make(Vbus, 0.0), c_expj(c->Va[i]));
     if (c->bus_type[i] == ORC_PV || c->bus_type[i] == ORC_SLACK)
       V[i] = c_mul(c_make(Vbus / c_abs(V[i]), 0.0), V[i]);
@@ -279,9 +307,9 @@ static cpx row_current(const ybus_t *Y, const cpx *V, int i) {
 }
 
 /* NR mismatch; returns inf-norm; NaN-aware (a NaN entry yields NaN norm) */
-static double mismatch_nr(const ybus_t *Y, const inst_t *I, const order_t *O,
-                          const cpx *V, double *F) {
-  double err = 0.0;
+static real mismatch_nr(const ybus_t *Y, const inst_t *I, const order_t *O,
+                          const cpx *V, real *F) {
+  real err = 0.0;
   int npv = O->npv, npq = O->npq;
   int nan = 0;
   for (int id = 0; id < npv + npq; id++) {
@@ -291,13 +319,13 @@ static double mismatch_nr(const ybus_t *Y, const inst_t *I, const order_t *O,
     cpx miss = c_sub(c_mul(V[bus], c_conj(cc)), c_make(I->P[bus], I->Q[bus]));
     if (id < npv) {
       F[i] = miss.re;
-      err = fmax(err, fabs(miss.re));
+      err = R_fmax(err, R_fabs(miss.re));
       nan |= isnan(miss.re);
     } else {
       F[npv + i] = miss.re;
       F[npv + npq + i] = miss.im;
-      err = fmax(err, fabs(miss.re));
-      err = fmax(err, fabs(miss.im));
+      err = R_fmax(err, R_fabs(miss.re));
+      err = R_fmax(err, R_fabs(miss.im));
       nan |= isnan(miss.re) | isnan(miss.im);
     }
   }
@@ -310,7 +338,7 @@ static double mismatch_nr(const ybus_t *Y, const inst_t *I, const order_t *O,
 typedef struct {
   int nJ, nnz;
   int *rowptr, *colind, *ybpos; /* ybpos: Ybus position of each entry */
-  double *val;
+  real *val;
 } jac_t;
 
 static void jac_free(jac_t *J) {
@@ -358,7 +386,7 @@ static void jac_pattern(const ybus_t *Y, const int *type, const order_t *O,
   J->rowptr = (int *)calloc(nJ + 1, sizeof(int));
   J->colind = (int *)malloc(sizeof(int) * (cnt > 0 ? cnt : 1));
   J->ybpos = (int *)malloc(sizeof(int) * (cnt > 0 ? cnt : 1));
-  J->val = (double *)malloc(sizeof(double) * (cnt > 0 ? cnt : 1));
+  J->val = (real *)malloc(sizeof(real) * (cnt > 0 ? cnt : 1));
   for (int q = 0; q < cnt; q++) {
     J->rowptr[t[q].r + 1]++;
     J->colind[q] = t[q].c;
@@ -379,32 +407,32 @@ static void jac_values(const ybus_t *Y, const order_t *O, const cpx *V,
       int jj = (cidx < length) ? ((cidx < npv) ? O->pv[cidx] : O->pq[cidx - npv])
                                : O->pq[cidx - length];
       cpx adm = Y->val[J->ybpos[q]];
-      double yr = adm.re, yi = adm.im;
-      double vrj = V[jj].re, vij = V[jj].im;
-      double ibr = (ii == jj) ? Ibus[ii].re : 0.0;
-      double ibi = (ii == jj) ? Ibus[ii].im : 0.0;
-      double vri = V[ii].re, vii = V[ii].im;
-      double v;
+      real yr = adm.re, yi = adm.im;
+      real vrj = V[jj].re, vij = V[jj].im;
+      real ibr = (ii == jj) ? Ibus[ii].re : 0.0;
+      real ibi = (ii == jj) ? Ibus[ii].im : 0.0;
+      real vri = V[ii].re, vii = V[ii].im;
+      real v;
       if (r < length) {
         if (cidx < length) {
-          double re = yr * vrj - yi * vij;
-          double im = yr * vij + yi * vrj;
+          real re = yr * vrj - yi * vij;
+          real im = yr * vij + yi * vrj;
           v = -vii * (ibr - re) - vri * (-ibi + im);
         } else {
-          double ab = sqrt(vrj * vrj + vij * vij);
-          double re = yr * vrj / ab - yi * vij / ab;
-          double im = yr * vij / ab + yi * vrj / ab;
+          real ab = R_sqrt(vrj * vrj + vij * vij);
+          real re = yr * vrj / ab - yi * vij / ab;
+          real im = yr * vij / ab + yi * vrj / ab;
           v = vri * re - vii * -im + ibr * vrj / ab + ibi * vij / ab;
         }
       } else {
         if (cidx < length) {
-          double re = yr * vrj - yi * vij;
-          double im = yr * vij + yi * vrj;
+          real re = yr * vrj - yi * vij;
+          real im = yr * vij + yi * vrj;
           v = -vii * (-ibi + im) + vri * (ibr - re);
         } else {
-          double ab = sqrt(vrj * vrj + vij * vij);
-          double re = yr * vrj / ab - yi * vij / ab;
-          double im = yr * vij / ab + yi * vrj / ab;
+          real ab = R_sqrt(vrj * vrj + vij * vij);
+          real re = yr * vrj / ab - yi * vij / ab;
+          real im = yr * vij / ab + yi * vrj / ab;
           v = vri * -im + vii * re + ibr * vij / ab + -ibi * vrj / ab;
         }
       }
@@ -503,8 +531,8 @@ void orc_min_degree(int n, const int *rowptr, const int *colind, int *perm) {
 typedef struct {
   int n;
   int *q, *pinv;
-  int *Lp, *Li; double *Lx; int lcap;
-  int *Up, *Ui; double *Ux; int ucap;
+  int *Lp, *Li; real *Lx; int lcap;
+  int *Up, *Ui; real *Ux; int ucap;
 } lu_t;
 
 static void lu_free(lu_t *F) {
@@ -514,8 +542,8 @@ static void lu_free(lu_t *F) {
 
 /* Factor A (CSC: Ap, Ai, Ax) with column order q.  Returns 0 or ORC_SINGULAR /
  * ORC_NONFINITE. */
-static int lu_factor(int n, const int *Ap, const int *Ai, const double *Ax,
-                     const int *q, double thresh, lu_t *F) {
+static int lu_factor(int n, const int *Ap, const int *Ai, const real *Ax,
+                     const int *q, real thresh, lu_t *F) {
   F->n = n;
   F->q = (int *)malloc(sizeof(int) * n);
   memcpy(F->q, q, sizeof(int) * n);
@@ -524,11 +552,11 @@ static int lu_factor(int n, const int *Ap, const int *Ai, const double *Ax,
   F->lcap = 4 * Ap[n] + n; F->ucap = 4 * Ap[n] + n;
   F->Lp = (int *)malloc(sizeof(int) * (n + 1));
   F->Li = (int *)malloc(sizeof(int) * F->lcap);
-  F->Lx = (double *)malloc(sizeof(double) * F->lcap);
+  F->Lx = (real *)malloc(sizeof(real) * F->lcap);
   F->Up = (int *)malloc(sizeof(int) * (n + 1));
   F->Ui = (int *)malloc(sizeof(int) * F->ucap);
-  F->Ux = (double *)malloc(sizeof(double) * F->ucap);
-  double *x = (double *)calloc(n, sizeof(double));
+  F->Ux = (real *)malloc(sizeof(real) * F->ucap);
+  real *x = (real *)calloc(n, sizeof(real));
   int *xi = (int *)malloc(sizeof(int) * n);
   int *stk = (int *)malloc(sizeof(int) * n);
   int *pos = (int *)malloc(sizeof(int) * n);
@@ -570,26 +598,26 @@ static int lu_factor(int n, const int *Ap, const int *Ai, const double *Ax,
       int r = xi[p];
       int j = F->pinv[r];
       if (j < 0) continue;
-      double xr = x[r];
+      real xr = x[r];
       for (int t = F->Lp[j]; t < F->Lp[j + 1]; t++) x[F->Li[t]] -= F->Lx[t] * xr;
     }
     /* pivot selection */
-    int ipiv = -1; double amax = -1.0;
+    int ipiv = -1; real amax = -1.0;
     int need = n - top + 1;
     if (unz + need > F->ucap) {
       F->ucap = 2 * (unz + need);
       F->Ui = (int *)realloc(F->Ui, sizeof(int) * F->ucap);
-      F->Ux = (double *)realloc(F->Ux, sizeof(double) * F->ucap);
+      F->Ux = (real *)realloc(F->Ux, sizeof(real) * F->ucap);
     }
     if (lnz + need > F->lcap) {
       F->lcap = 2 * (lnz + need);
       F->Li = (int *)realloc(F->Li, sizeof(int) * F->lcap);
-      F->Lx = (double *)realloc(F->Lx, sizeof(double) * F->lcap);
+      F->Lx = (real *)realloc(F->Lx, sizeof(real) * F->lcap);
     }
     for (int p = top; p < n; p++) {
       int r = xi[p];
       if (F->pinv[r] < 0) {
-        double a = fabs(x[r]);
+        real a = R_fabs(x[r]);
         if (isnan(a)) { status = ORC_NONFINITE; }
         if (a > amax) { amax = a; ipiv = r; }
       } else {
@@ -598,13 +626,13 @@ static int lu_factor(int n, const int *Ap, const int *Ai, const double *Ax,
       }
     }
     if (status) break;
-    if (F->pinv[col] < 0 && mark[col] == k + 1 && fabs(x[col]) >= thresh * amax)
+    if (F->pinv[col] < 0 && mark[col] == k + 1 && R_fabs(x[col]) >= thresh * amax)
       ipiv = col;
     if (ipiv < 0 || amax <= 0.0 || !isfinite(amax)) {
       status = (ipiv >= 0 && !isfinite(amax)) ? ORC_NONFINITE : ORC_SINGULAR;
       break;
     }
-    double piv = x[ipiv];
+    real piv = x[ipiv];
     F->Ui[unz] = k; F->Ux[unz++] = piv;
     F->pinv[ipiv] = k;
     for (int p = top; p < n; p++) {
@@ -624,19 +652,19 @@ static int lu_factor(int n, const int *Ap, const int *Ai, const double *Ax,
   return status;
 }
 
-static void lu_solve(const lu_t *F, const double *b, double *xout) {
+static void lu_solve(const lu_t *F, const real *b, real *xout) {
   int n = F->n;
-  double *c = (double *)malloc(sizeof(double) * n);
+  real *c = (real *)malloc(sizeof(real) * n);
   for (int r = 0; r < n; r++) c[F->pinv[r]] = b[r];
   for (int k = 0; k < n; k++) {
-    double ck = c[k];
+    real ck = c[k];
     for (int t = F->Lp[k]; t < F->Lp[k + 1]; t++) c[F->Li[t]] -= F->Lx[t] * ck;
   }
   for (int k = n - 1; k >= 0; k--) {
     /* diagonal is the last entry of U(:,k) */
     int last = F->Up[k + 1] - 1;
     c[k] /= F->Ux[last];
-    double ck = c[k];
+    real ck = c[k];
     for (int t = F->Up[k]; t < last; t++) c[F->Ui[t]] -= F->Ux[t] * ck;
   }
   for (int k = 0; k < n; k++) xout[F->q[k]] = c[k];
@@ -644,8 +672,8 @@ static void lu_solve(const lu_t *F, const double *b, double *xout) {
 }
 
 /* CSR -> CSC (values follow) */
-static void csr_to_csc(int n, const int *rp, const int *ci, const double *v,
-                       int *Ap, int *Ai, double *Ax) {
+static void csr_to_csc(int n, const int *rp, const int *ci, const real *v,
+                       int *Ap, int *Ai, real *Ax) {
   int nnz = rp[n];
   for (int j = 0; j <= n; j++) Ap[j] = 0;
   for (int k = 0; k < nnz; k++) Ap[ci[k] + 1]++;
@@ -662,12 +690,12 @@ static void csr_to_csc(int n, const int *rp, const int *ci, const double *v,
 }
 
 /* factor a CSR matrix with a given column order; returns status */
-static int csr_factor(int n, const int *rp, const int *ci, const double *v,
-                      const int *q, double thresh, lu_t *F) {
+static int csr_factor(int n, const int *rp, const int *ci, const real *v,
+                      const int *q, real thresh, lu_t *F) {
   int nnz = rp[n];
   int *Ap = (int *)malloc(sizeof(int) * (n + 1));
   int *Ai = (int *)malloc(sizeof(int) * (nnz > 0 ? nnz : 1));
-  double *Ax = (double *)malloc(sizeof(double) * (nnz > 0 ? nnz : 1));
+  real *Ax = (real *)malloc(sizeof(real) * (nnz > 0 ? nnz : 1));
   csr_to_csc(n, rp, ci, v, Ap, Ai, Ax);
   int st = lu_factor(n, Ap, Ai, Ax, q, thresh, F);
   free(Ap); free(Ai); free(Ax);
@@ -679,12 +707,21 @@ int orc_sparse_solve(int n, const int *rowptr, const int *colind,
                      double pivot_thresh) {
   int *q = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
   orc_min_degree(n, rowptr, colind, q);
+  int nnz = rowptr[n];
+  real *v = (real *)malloc(sizeof(real) * (nnz > 0 ? nnz : 1));
+  real *bb = (real *)malloc(sizeof(real) * (n > 0 ? n : 1));
+  real *xx = (real *)malloc(sizeof(real) * (n > 0 ? n : 1));
+  for (int k = 0; k < nnz; k++) v[k] = val[k];
+  for (int k = 0; k < n; k++) bb[k] = b[k];
   lu_t F;
   memset(&F, 0, sizeof(F));
-  int st = csr_factor(n, rowptr, colind, val, q, pivot_thresh, &F);
-  if (!st) lu_solve(&F, b, x);
+  int st = csr_factor(n, rowptr, colind, v, q, pivot_thresh, &F);
+  if (!st) {
+    lu_solve(&F, bb, xx);
+    for (int k = 0; k < n; k++) x[k] = (double)xx[k];
+  }
   lu_free(&F);
-  free(q);
+  free(q); free(v); free(bb); free(xx);
   return st;
 }
 
@@ -698,7 +735,7 @@ int orc_ref_linsolve(int n, const int *rowptr, const int *colind,
 /* ------------------------------------------------------------------------- */
 /* voltage update (newtonpf.h:188-213)                                         */
 /* ------------------------------------------------------------------------- */
-static void update_voltage(const order_t *O, cpx *V, const double *dx) {
+static void update_voltage(const order_t *O, cpx *V, const real *dx) {
   int npv = O->npv, npq = O->npq;
   for (int id = 0; id < npv + npq; id++) {
     if (id < npv) {
@@ -718,17 +755,17 @@ static void update_voltage(const order_t *O, cpx *V, const double *dx) {
 /* Newton-Raphson (newtonpf.h:365-523)                                        */
 /* ------------------------------------------------------------------------- */
 static int newtonpf(const ybus_t *Y, const inst_t *I, const int *type, cpx *V,
-                    const orc_opts *o, int *iters, double *norm_out) {
+                    const orc_opts *o, int *iters, real *norm_out) {
   order_t O;
   order_build(type, I->n, &O);
   int nJ = O.npv + 2 * O.npq;
-  double *F = (double *)malloc(sizeof(double) * (nJ > 0 ? nJ : 1));
-  double *dx = (double *)malloc(sizeof(double) * (nJ > 0 ? nJ : 1));
+  real *F = (real *)malloc(sizeof(real) * (nJ > 0 ? nJ : 1));
+  real *dx = (real *)malloc(sizeof(real) * (nJ > 0 ? nJ : 1));
   cpx *Ibus = (cpx *)malloc(sizeof(cpx) * I->n);
   jac_t J;
   memset(&J, 0, sizeof(J));
   int *q = NULL;
-  double err = mismatch_nr(Y, I, &O, V, F);
+  real err = mismatch_nr(Y, I, &O, V, F);
   int iter = 0, verdict = ORC_MAX_ITER;
   if (err < o->tol) verdict = ORC_CONVERGED;
   else if (isnan(err) || isinf(err)) verdict = ORC_NONFINITE;
@@ -741,6 +778,9 @@ static int newtonpf(const ybus_t *Y, const inst_t *I, const int *type, cpx *V,
       orc_min_degree(nJ, J.rowptr, J.colind, q);
     }
     jac_values(Y, &O, V, Ibus, &J);
+#if defined(ORC_LINSOLVE_EIGEN) && defined(ORC_EXTENDED)
+#error "the Eigen-linked build is FP64 only"
+#endif
 #ifdef ORC_LINSOLVE_EIGEN
     int st = orc_ref_linsolve(nJ, J.rowptr, J.colind, J.val, F, dx);
     if (st) { verdict = st; break; }
@@ -768,9 +808,9 @@ static int newtonpf(const ybus_t *Y, const inst_t *I, const int *type, cpx *V,
 /* ------------------------------------------------------------------------- */
 /* fast decoupled (makeB.h:311-501, fdpf.h:369-596)                           */
 /* ------------------------------------------------------------------------- */
-static double mismatch_fd(const ybus_t *Y, const inst_t *I, const order_t *O,
-                          const cpx *V, double *P, double *Q) {
-  double errP = -1.0, errQ = -1.0;
+static real mismatch_fd(const ybus_t *Y, const inst_t *I, const order_t *O,
+                          const cpx *V, real *P, real *Q) {
+  real errP = -1.0, errQ = -1.0;
   int npv = O->npv, npq = O->npq, nan = 0;
   for (int id = 0; id < npv + npq; id++) {
     int i = (id < npv) ? id : id - npv;
@@ -780,22 +820,22 @@ static double mismatch_fd(const ybus_t *Y, const inst_t *I, const order_t *O,
     miss = c_div(miss, c_make(c_abs(V[bus]), 0.0));
     if (id < npv) {
       P[i] = miss.re;
-      errP = fmax(errP, fabs(P[i]));
+      errP = R_fmax(errP, R_fabs(P[i]));
       nan |= isnan(miss.re);
     } else {
       P[npv + i] = miss.re;
-      errP = fmax(errP, fabs(P[npv + i]));
+      errP = R_fmax(errP, R_fabs(P[npv + i]));
       Q[i] = miss.im;
-      errQ = fmax(errQ, fabs(Q[i]));
+      errQ = R_fmax(errQ, R_fabs(Q[i]));
       nan |= isnan(miss.re) | isnan(miss.im);
     }
   }
-  return nan ? NAN : fmax(errP, errQ);
+  return nan ? NAN : R_fmax(errP, errQ);
 }
 
 /* B matrix over an index list (idx[bus] >= 0 selects) from a Ybus build */
 static void make_bmat(const ybus_t *Y, const int *sel, int nsel, const int *idx,
-                      int **rp_o, int **ci_o, double **v_o) {
+                      int **rp_o, int **ci_o, real **v_o) {
   int *rp = (int *)calloc(nsel + 1, sizeof(int));
   int cnt = 0;
   for (int r = 0; r < nsel; r++) {
@@ -805,7 +845,7 @@ static void make_bmat(const ybus_t *Y, const int *sel, int nsel, const int *idx,
     rp[r + 1] = cnt;
   }
   int *ci = (int *)malloc(sizeof(int) * (cnt > 0 ? cnt : 1));
-  double *v = (double *)malloc(sizeof(double) * (cnt > 0 ? cnt : 1));
+  real *v = (real *)malloc(sizeof(real) * (cnt > 0 ? cnt : 1));
   trip_t *row = (trip_t *)malloc(sizeof(trip_t) * (Y->n + 1));
   for (int r = 0; r < nsel; r++) {
     int i = sel[r], len = 0;
@@ -826,17 +866,17 @@ static void make_bmat(const ybus_t *Y, const int *sel, int nsel, const int *idx,
 
 static int fdpf(const ybus_t *Y, const inst_t *I, const int *type,
                 const int *from, const int *to, cpx *V, const orc_opts *o,
-                int *iters, double *norm_out) {
+                int *iters, real *norm_out) {
   int n = I->n, m = I->m;
   order_t O;
   order_build(type, n, &O);
   int npv = O.npv, npq = O.npq, np = npv + npq;
   /* B': zero line charging, taps -> 1, bus shunts -> 0; XB also R -> 0 */
   inst_t T = *I;
-  double *zB = (double *)calloc(m, sizeof(double));
-  double *zT = (double *)calloc(m, sizeof(double));
-  double *zR = (double *)calloc(m, sizeof(double));
-  double *zBs = (double *)calloc(n, sizeof(double));
+  real *zB = (real *)calloc(m, sizeof(real));
+  real *zT = (real *)calloc(m, sizeof(real));
+  real *zR = (real *)calloc(m, sizeof(real));
+  real *zBs = (real *)calloc(n, sizeof(real));
   T.B = zB; T.tap = zT; T.Bs = zBs;
   if (o->alg == ORC_FDXB) T.R = zR;
   ybus_t Yp;
@@ -845,11 +885,11 @@ static int fdpf(const ybus_t *Y, const inst_t *I, const int *type,
   int *sel_p = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
   for (int k = 0; k < npv; k++) sel_p[k] = O.pv[k];
   for (int k = 0; k < npq; k++) sel_p[npv + k] = O.pq[k];
-  int *rpP, *ciP; double *vP;
+  int *rpP, *ciP; real *vP;
   make_bmat(&Yp, sel_p, np, O.idx, &rpP, &ciP, &vP);
   /* B'': zero phase shifters; BX also R -> 0 (taps, charging, shunts kept) */
   inst_t T2 = *I;
-  double *zS = (double *)calloc(m, sizeof(double));
+  real *zS = (real *)calloc(m, sizeof(real));
   T2.shift = zS;
   if (o->alg == ORC_FDBX) T2.R = zR;
   ybus_t Ypp;
@@ -858,15 +898,15 @@ static int fdpf(const ybus_t *Y, const inst_t *I, const int *type,
   int *pqidx = (int *)malloc(sizeof(int) * n);
   for (int i = 0; i < n; i++) pqidx[i] = -1;
   for (int k = 0; k < npq; k++) pqidx[O.pq[k]] = k;
-  int *rpQ, *ciQ; double *vQ;
+  int *rpQ, *ciQ; real *vQ;
   make_bmat(&Ypp, O.pq, npq, pqidx, &rpQ, &ciQ, &vQ);
 
-  double *P = (double *)malloc(sizeof(double) * (np > 0 ? np : 1));
-  double *Q = (double *)malloc(sizeof(double) * (npq > 0 ? npq : 1));
-  double *dVa = (double *)malloc(sizeof(double) * (np > 0 ? np : 1));
-  double *dVm = (double *)malloc(sizeof(double) * (npq > 0 ? npq : 1));
+  real *P = (real *)malloc(sizeof(real) * (np > 0 ? np : 1));
+  real *Q = (real *)malloc(sizeof(real) * (npq > 0 ? npq : 1));
+  real *dVa = (real *)malloc(sizeof(real) * (np > 0 ? np : 1));
+  real *dVm = (real *)malloc(sizeof(real) * (npq > 0 ? npq : 1));
   int verdict = ORC_MAX_ITER, it = 0;
-  double err = mismatch_fd(Y, I, &O, V, P, Q);
+  real err = mismatch_fd(Y, I, &O, V, P, Q);
   if (err < o->tol) verdict = ORC_CONVERGED;
   else if (isnan(err)) verdict = ORC_NONFINITE;
   /* B' and B'' are constant: factor once (the reference refactors the same
@@ -919,9 +959,9 @@ static int fdpf(const ybus_t *Y, const inst_t *I, const int *type,
 /* ------------------------------------------------------------------------- */
 /* loss and branch flows (runpf.h:228-257)                                     */
 /* ------------------------------------------------------------------------- */
-static double branch_loss(const ybus_t *Y, const int *from, const int *to,
+static real branch_loss(const ybus_t *Y, const int *from, const int *to,
                           int m, const cpx *V, cpx *Sf, cpx *St) {
-  double sum = 0.0;
+  real sum = 0.0;
   for (int b = 0; b < m; b++) {
     int f = from[b], t = to[b];
     /* Yf/Yt rows hold two entries with sorted columns (makeYbus.h:84-98) */
@@ -943,12 +983,27 @@ static double branch_loss(const ybus_t *Y, const int *from, const int *to,
 }
 
 /* ------------------------------------------------------------------------- */
-/* public entry points                                                         */
+/* public entry points (FP64 arrays; converted at the boundary)               */
 /* ------------------------------------------------------------------------- */
+static cpx *cpx_in(const double *v, int n) {
+  cpx *z = (cpx *)malloc(sizeof(cpx) * (n > 0 ? n : 1));
+  for (int i = 0; i < n; i++) z[i] = c_make(v[2 * i], v[2 * i + 1]);
+  return z;
+}
+static void cpx_out(const cpx *z, int n, double *v) {
+  for (int i = 0; i < n; i++) {
+    v[2 * i] = (double)z[i].re;
+    v[2 * i + 1] = (double)z[i].im;
+  }
+}
+
 void orc_initial_voltage(const orc_case *c, const orc_inst *inst, double *Vout) {
   inst_t I;
   inst_build(c, inst, &I);
-  initial_voltage(c, &I, (cpx *)Vout);
+  cpx *V = (cpx *)malloc(sizeof(cpx) * c->nbus);
+  initial_voltage(c, &I, V);
+  cpx_out(V, c->nbus, Vout);
+  free(V);
   inst_free(&I);
 }
 
@@ -963,7 +1018,7 @@ int orc_make_ybus(const orc_case *c, const orc_inst *inst, int *rowptr,
   if (nnz <= cap) {
     memcpy(rowptr, Y.rowptr, sizeof(int) * (c->nbus + 1));
     memcpy(colind, Y.colind, sizeof(int) * nnz);
-    memcpy(val, Y.val, sizeof(cpx) * nnz);
+    cpx_out(Y.val, nnz, val);
   } else {
     nnz = -nnz;
   }
@@ -972,7 +1027,7 @@ int orc_make_ybus(const orc_case *c, const orc_inst *inst, int *rowptr,
   return nnz;
 }
 
-int orc_mismatch(const orc_case *c, const orc_inst *inst, const double *V,
+int orc_mismatch(const orc_case *c, const orc_inst *inst, const double *Vin,
                  double *F, double *norm) {
   inst_t I;
   inst_build(c, inst, &I);
@@ -981,15 +1036,19 @@ int orc_mismatch(const orc_case *c, const orc_inst *inst, const double *V,
   ybus_build(&I, c->br_from, c->br_to, &Y);
   order_t O;
   order_build(I.type, I.n, &O);
-  *norm = mismatch_nr(&Y, &I, &O, (const cpx *)V, F);
   int nJ = O.npv + 2 * O.npq;
+  cpx *V = cpx_in(Vin, I.n);
+  real *Fr = (real *)malloc(sizeof(real) * (nJ > 0 ? nJ : 1));
+  *norm = (double)mismatch_nr(&Y, &I, &O, V, Fr);
+  for (int k = 0; k < nJ; k++) F[k] = (double)Fr[k];
+  free(Fr); free(V);
   order_free(&O);
   ybus_free(&Y);
   inst_free(&I);
   return nJ;
 }
 
-int orc_jacobian(const orc_case *c, const orc_inst *inst, const double *V,
+int orc_jacobian(const orc_case *c, const orc_inst *inst, const double *Vin,
                  int *rowptr, int *colind, double *val, int cap) {
   inst_t I;
   inst_build(c, inst, &I);
@@ -1001,18 +1060,19 @@ int orc_jacobian(const orc_case *c, const orc_inst *inst, const double *V,
   jac_t J;
   memset(&J, 0, sizeof(J));
   jac_pattern(&Y, I.type, &O, &J);
+  cpx *V = cpx_in(Vin, I.n);
   cpx *Ib = (cpx *)malloc(sizeof(cpx) * I.n);
-  for (int i = 0; i < I.n; i++) Ib[i] = row_current(&Y, (const cpx *)V, i);
-  jac_values(&Y, &O, (const cpx *)V, Ib, &J);
+  for (int i = 0; i < I.n; i++) Ib[i] = row_current(&Y, V, i);
+  jac_values(&Y, &O, V, Ib, &J);
   int nnz = J.nnz;
   if (nnz <= cap) {
     memcpy(rowptr, J.rowptr, sizeof(int) * (J.nJ + 1));
     memcpy(colind, J.colind, sizeof(int) * nnz);
-    memcpy(val, J.val, sizeof(double) * nnz);
+    for (int k = 0; k < nnz; k++) val[k] = (double)J.val[k];
   } else {
     nnz = -nnz;
   }
-  free(Ib);
+  free(Ib); free(V);
   jac_free(&J);
   order_free(&O);
   ybus_free(&Y);
@@ -1020,45 +1080,59 @@ int orc_jacobian(const orc_case *c, const orc_inst *inst, const double *V,
   return nnz;
 }
 
-double orc_objective(const orc_case *c, const orc_inst *inst, const double *V,
+/* losses + w * sum over both ends of max(0, |S| - rate)^2 at V, in the
+ * working precision; optional Sf/St out */
+static real objective_at(const orc_case *c, const ybus_t *Y, const cpx *V,
+                         const double *rate, double w, double *Sf, double *St) {
+  int m = c->nbranch;
+  cpx *sf = (cpx *)malloc(sizeof(cpx) * (m > 0 ? m : 1));
+  cpx *st = (cpx *)malloc(sizeof(cpx) * (m > 0 ? m : 1));
+  real f = branch_loss(Y, c->br_from, c->br_to, m, V, sf, st);
+  if (rate) {
+    for (int b = 0; b < m; b++) {
+      real ef = c_abs(sf[b]) - rate[b], et = c_abs(st[b]) - rate[b];
+      if (ef > 0) f += w * ef * ef;
+      if (et > 0) f += w * et * et;
+    }
+  }
+  if (Sf) cpx_out(sf, m, Sf);
+  if (St) cpx_out(st, m, St);
+  free(sf); free(st);
+  return f;
+}
+
+double orc_objective(const orc_case *c, const orc_inst *inst, const double *Vin,
                      const double *rate, double w, double *Sf, double *St) {
   inst_t I;
   inst_build(c, inst, &I);
   ybus_t Y;
   memset(&Y, 0, sizeof(Y));
   ybus_build(&I, c->br_from, c->br_to, &Y);
-  int m = c->nbranch;
-  cpx *sf = (cpx *)malloc(sizeof(cpx) * m);
-  cpx *st = (cpx *)malloc(sizeof(cpx) * m);
-  double f = branch_loss(&Y, c->br_from, c->br_to, m, (const cpx *)V, sf, st);
-  if (rate) {
-    for (int b = 0; b < m; b++) {
-      double ef = c_abs(sf[b]) - rate[b], et = c_abs(st[b]) - rate[b];
-      if (ef > 0) f += w * ef * ef;
-      if (et > 0) f += w * et * et;
-    }
-  }
-  if (Sf) memcpy(Sf, sf, sizeof(cpx) * m);
-  if (St) memcpy(St, st, sizeof(cpx) * m);
-  free(sf); free(st);
+  cpx *V = cpx_in(Vin, c->nbus);
+  real f = objective_at(c, &Y, V, rate, w, Sf, St);
+  free(V);
   ybus_free(&Y);
   inst_free(&I);
-  return f;
+  return (double)f;
 }
 
-int orc_solve(const orc_case *c, const orc_inst *inst, const orc_opts *o,
-              orc_result *r) {
+/* solve, then (when f_hi is given) the objective at the final V before it is
+ * rounded to FP64: f = f_hi + f_lo (whatever the verdict: floor-precision
+ * re-solves run a fixed iteration count and check r->norm themselves) */
+static int solve_impl(const orc_case *c, const orc_inst *inst, const orc_opts *o,
+                      orc_result *r, const double *rate, double w,
+                      double *f_hi, double *f_lo) {
   inst_t I;
   inst_build(c, inst, &I);
   ybus_t Y;
   memset(&Y, 0, sizeof(Y));
   ybus_build(&I, c->br_from, c->br_to, &Y);
   int n = c->nbus;
-  cpx *V = (cpx *)malloc(sizeof(cpx) * n);
-  if (inst && inst->V0) memcpy(V, inst->V0, sizeof(cpx) * n);
-  else initial_voltage(c, &I, V);
+  cpx *V;
+  if (inst && inst->V0) V = cpx_in(inst->V0, n);
+  else { V = (cpx *)malloc(sizeof(cpx) * n); initial_voltage(c, &I, V); }
   int total = 0, rounds = 0, verdict = ORC_MAX_ITER;
-  double norm = NAN;
+  real norm = NAN;
   for (;;) {
     int it = 0;
     if (o->alg == ORC_NR)
@@ -1073,7 +1147,7 @@ int orc_solve(const orc_case *c, const orc_inst *inst, const orc_opts *o,
     for (int i = 0; i < n; i++) {
       if (I.type[i] != ORC_PV) continue;
       cpx S = c_mul(V[i], c_conj(row_current(&Y, V, i)));
-      double qg = S.im + I.Qd[i];
+      real qg = S.im + I.Qd[i];
       if (qg > I.Qmax[i] + o->qlim_tol) { viol[i] = 1; nviol++; }
       else if (qg < I.Qmin[i] - o->qlim_tol) { viol[i] = -1; nviol++; }
     }
@@ -1095,14 +1169,39 @@ int orc_solve(const orc_case *c, const orc_inst *inst, const orc_opts *o,
   r->qlim_rounds = rounds;
   r->verdict = verdict;
   r->converged = verdict == ORC_CONVERGED;
-  r->norm = norm;
+  r->norm = (double)norm;
   r->loss = r->converged
-                ? branch_loss(&Y, c->br_from, c->br_to, c->nbranch, V, NULL, NULL)
+                ? (double)branch_loss(&Y, c->br_from, c->br_to, c->nbranch, V,
+                                      NULL, NULL)
                 : DBL_MAX;
-  if (r->V) memcpy(r->V, V, sizeof(cpx) * n);
+  if (f_hi) {
+    real f = objective_at(c, &Y, V, rate, w, NULL, NULL);
+    *f_hi = (double)f;
+    if (f_lo) *f_lo = (double)(f - (real)*f_hi);
+  }
+  if (r->V) cpx_out(V, n, r->V);
   if (r->bus_type) memcpy(r->bus_type, I.type, sizeof(int) * n);
   free(V);
   ybus_free(&Y);
   inst_free(&I);
   return 0;
+}
+
+int orc_solve(const orc_case *c, const orc_inst *inst, const orc_opts *o,
+              orc_result *r) {
+  return solve_impl(c, inst, o, r, NULL, 0.0, NULL, NULL);
+}
+
+int orc_solve_objective(const orc_case *c, const orc_inst *inst,
+                        const orc_opts *o, const double *rate, double w,
+                        orc_result *r, double *f_hi, double *f_lo) {
+  return solve_impl(c, inst, o, r, rate, w, f_hi, f_lo);
+}
+
+int orc_precision_bits(void) {
+#ifdef ORC_EXTENDED
+  return LDBL_MANT_DIG;
+#else
+  return DBL_MANT_DIG;
+#endif
 }

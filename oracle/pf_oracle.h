@@ -4,7 +4,8 @@
  * library as the checker.  Nothing in the product (mapol_amd/, include/)
  * links, loads or calls it.
  *
- * Plain C, FP64 throughout.  Every routine restates one function of the
+ * Plain C, FP64 (an 80-bit build of the same source serves gradient
+ * checks, see orc_solve_objective).  Every routine restates one function of the
  * reference's CPU path (MAPoL, exec types S/O):
  *   V0            src/powerflow/runpf.h:211-226      (mkl_computeVoltage)
  *   Yf/Yt/Ybus    src/powerflow/makeYbus.h:17-274    (mkl_computeCfCt/YfYt/makeYbus)
@@ -111,6 +112,17 @@ void orc_initial_voltage(const orc_case *c, const orc_inst *inst, double *V);
  * total_loss + w * sum(max(0,|S|-rate)^2) over both ends. */
 double orc_objective(const orc_case *c, const orc_inst *inst, const double *V,
                      const double *rate, double w, double *Sf, double *St);
+/* Solve, then evaluate the objective of orc_objective at the solution in the
+ * working precision at the final V, before rounding: objective = *f_hi +
+ * *f_lo, whatever the verdict (check r->norm).  In the FP64 build f_lo is 0; the extended
+ * build (-DORC_EXTENDED, liborcpf_x.so: the same code in 80-bit long double,
+ * FP64 arrays at this interface) resolves the objective to ~1e-19 relative,
+ * which finite-difference gradient checks need. */
+int orc_solve_objective(const orc_case *c, const orc_inst *inst,
+                        const orc_opts *o, const double *rate, double w,
+                        orc_result *r, double *f_hi, double *f_lo);
+/* significand bits of the working precision (53, or 64 extended) */
+int orc_precision_bits(void);
 /* Solve A x = b with the oracle's sparse LU; A in CSR. Returns 0 / verdict. */
 int orc_sparse_solve(int n, const int *rowptr, const int *colind,
                      const double *val, const double *b, double *x,

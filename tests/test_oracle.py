@@ -269,3 +269,32 @@ def test_own_lu_matches_reference_eigen_live():
         ra, rb = a.solve(**kw), b.solve(**kw)
         assert ra["iterations"] == rb["iterations"]
         assert np.abs(ra["V"] - rb["V"]).max() < 1e-10
+
+
+def test_extended_build_matches_golden_and_reaches_its_floor(golden):
+    """The 80-bit build (same source, long double) that the adjoint checks
+    re-solve with: 64 significand bits, the golden iteration counts and
+    verdicts, voltages within FP64 rounding of the golden ones, a Newton
+    floor three orders below FP64's, and the unrounded objective (hi + lo)
+    equal to the FP64 objective at the same point to FP64 rounding."""
+    assert oracle.lib("x").orc_precision_bits() == 64
+    assert oracle.lib("own").orc_precision_bits() == 53
+    case, cont = synth.workload("pf118x200")
+    ox, o64 = O(case, which="x"), O(case)
+    for k, t in enumerate(golden["pf118x200/nr/idx"][:40]):
+        inst = dict(br_on=cont.br_on[t], gen_on=cont.gen_on[t])
+        r = ox.solve(**inst)
+        assert r["iterations"] == golden["pf118x200/nr/its"][k]
+        assert r["verdict"] == golden["pf118x200/nr/verdict"][k]
+        if r["converged"]:
+            assert np.abs(r["V"] - golden["pf118x200/nr/V"][k]).max() <= 1e-10
+    floor = O.opts(tol=1e-300, maxit_nr=12)
+    inst = dict(br_on=cont.br_on[0], gen_on=cont.gen_on[0])
+    rx = ox.solve_objective(floor, w=0.0, **inst)
+    r64 = o64.solve_objective(floor, w=0.0, **inst)
+    assert rx["norm"] < 1e-16 < r64["norm"] * 10
+    assert r64["f_lo"] == 0.0
+    f64 = o64.objective(rx["V"], **inst)[0]
+    assert abs((rx["f_hi"] + rx["f_lo"]) - f64) <= 1e-13 * abs(f64)
+    # the FP64 and 80-bit solutions agree to FP64 rounding
+    assert np.abs(rx["V"] - r64["V"]).max() <= 1e-13

@@ -1,9 +1,11 @@
 """Host-side symbolic analysis of the engine (no GPU): the L+U block pattern,
 the Crout level schedule and the sweep schedules exported by ``bpf_analyse``
 are replayed in numpy exactly as the kernels execute them (phase A dot
-products, inverted pivots, phase B scaling; forward/backward sweeps) on the
-oracle's Jacobian, and the resulting Newton step must equal a sparse direct
-solve of the same system."""
+products, inverted pivots, phase B scaling, the dense tail's Gauss-Jordan
+inverse; forward/backward sweeps over the non-tail rows and the tail rows
+through that inverse) on the oracle's Jacobian: the resulting Newton step
+must equal a sparse direct solve of the same system, and the transposed
+(adjoint) schedule a dense solve of the transposed system."""
 import numpy as np
 import pytest
 import scipy.sparse.linalg as sla
@@ -50,8 +52,12 @@ def _blocks(case, a, J, F):
 
 
 def _crout(a, A):
+    """the kernel's factorization: Crout levels below the dense tail (phase A
+    dot products, inverted pivots, phase B scaling), then the tail's Schur
+    complement assembled from its pair lists and inverted by block
+    Gauss-Jordan in the kernel's pivot order.  Returns (LU blocks, S^-1)."""
     A = A.copy()
-    for l in range(a["info"]["lu_levels"]):
+    for l in range(a["info"]["tail_level"]):
         for k in range(a["lvA"][l], a["lvA"][l + 1]):
             d = int(a["tA_dst"][k])
             p = d & MASK
@@ -61,44 +67,143 @@ def _crout(a, A):
             A[p] = np.linalg.inv(acc) if d & DIAG else acc
         for k in range(a["lvB"][l], a["lvB"][l + 1]):
             A[a["tB_dst"][k]] = A[a["tB_dst"][k]] @ A[a["tB_diag"][k]]
-    return A
+    s = a["info"]["tail_n"]
+    D = np.zeros((s, s, 2, 2))
+    for k in range(a["info"]["tail_entries"]):
+        dd = int(a["tl_dense"][k])
+        acc = A[a["tl_pos"][k]].copy() if dd & HAS_A else np.zeros((2, 2))
+        for q in range(a["tl_ptr"][k], a["tl_ptr"][k + 1]):
+            acc -= A[a["tl_pairL"][q]] @ A[a["tl_pairU"][q]]
+        D[divmod(dd & MASK, s)] = acc
+    for k in range(s):
+        P = np.linalg.inv(D[k, k])
+        col, row = D[:, k].copy(), D[k, :].copy()
+        for i in range(s):
+            for j in range(s):
+                if i == k:
+                    D[i, j] = P if j == k else P @ row[j]
+                elif j == k:
+                    D[i, j] = -(col[i] @ P)
+                else:
+                    D[i, j] = D[i, j] - col[i] @ (P @ row[j])
+    return A, D
 
 
-def _sweeps(a, A, b):
+def _diag(a):
+    rp, ci = a["lu_rowptr"], a["lu_colind"]
+    return np.array([rp[i] + np.searchsorted(ci[rp[i]:rp[i + 1]], i)
+                     for i in range(len(rp) - 1)])
+
+
+def _mirror(a):
+    rp, ci = a["lu_rowptr"], a["lu_colind"]
+    m = np.empty(len(ci), int)
+    for r in range(len(rp) - 1):
+        for p in range(rp[r], rp[r + 1]):
+            c = ci[p]
+            m[p] = rp[c] + np.searchsorted(ci[rp[c]:rp[c + 1]], r)
+    return m
+
+
+def _sweeps(a, A, Sinv, b, trans=False):
+    """the kernels' sweeps: non-tail rows by the forward levels, the tail rows
+    through S^-1 after subtracting their non-tail L entries, non-tail rows by
+    the backward levels.  trans: the adjoint's transposed solve (U^T then
+    L^T through the mirror map, S^-T)."""
     rp, ci = a["lu_rowptr"], a["lu_colind"]
     x = b.reshape(-1, 2).copy()
-    diag = np.array([rp[i] + np.searchsorted(ci[rp[i]:rp[i + 1]], i)
-                     for i in range(len(rp) - 1)])
+    diag = _diag(a)
+    mir = _mirror(a) if trans else None
+    T = (lambda M: M.T) if trans else (lambda M: M)
     for l in range(a["info"]["fw_levels"]):
         for i in a["fw_rows"][a["fw_ptr"][l]:a["fw_ptr"][l + 1]]:
             for p in range(rp[i], diag[i]):
-                x[i] -= A[p] @ x[ci[p]]
+                x[i] -= (A[mir[p]].T if trans else A[p]) @ x[ci[p]]
+            if trans:
+                x[i] = A[diag[i]].T @ x[i]
+    s = a["info"]["tail_n"]
+    piv = a["tail_piv"]
+    rs = np.zeros((s, 2))
+    for k in range(s):
+        rs[k] = x[piv[k]]
+        for q in range(a["ts_ptr"][k], a["ts_ptr"][k + 1]):
+            p = a["ts_pos"][q]
+            rs[k] -= (A[mir[p]].T if trans else A[p]) @ x[a["ts_col"][q]]
+    for k in range(s):
+        x[piv[k]] = sum(T(Sinv[j, k] if trans else Sinv[k, j]) @ rs[j]
+                        for j in range(s))
     for l in range(a["info"]["bw_levels"]):
         for i in a["bw_rows"][a["bw_ptr"][l]:a["bw_ptr"][l + 1]]:
             for p in range(diag[i] + 1, rp[i + 1]):
-                x[i] -= A[p] @ x[ci[p]]
-            x[i] = A[diag[i]] @ x[i]
+                x[i] -= (A[mir[p]].T if trans else A[p]) @ x[ci[p]]
+            if not trans:
+                x[i] = A[diag[i]] @ x[i]
     return x.ravel()
 
 
-@pytest.mark.parametrize("name", ["pf14", "pf118x200", "pf1354x200"])
-def test_schedule_replay_gives_the_newton_step(name):
+def _system(name, seed=0):
     case, cont = synth.workload(name)
     a = engine.analyse(case)
     orc = oracle.Oracle(case)
-    V = orc.initial_voltage() * (1 + 0.01 * np.random.default_rng(0)
+    V = orc.initial_voltage() * (1 + 0.01 * np.random.default_rng(seed)
                                  .normal(size=case.nbus))
     J = orc.jacobian(V)
     F, _ = orc.mismatch(V)
     A, rhs, rowmap = _blocks(case, a, J, F)
-    LU = _crout(a, A)
-    x = _sweeps(a, LU, rhs)
+    return case, a, J, F, A, rhs, rowmap
+
+
+@pytest.mark.parametrize("name", ["pf14", "pf118x200", "pf1354x200"])
+def test_schedule_replay_gives_the_newton_step(name):
+    case, a, J, F, A, rhs, rowmap = _system(name)
+    assert a["info"]["tail_n"] > 0
+    LU, Sinv = _crout(a, A)
+    x = _sweeps(a, LU, Sinv, rhs)
     ref = sla.spsolve(J.tocsc(), F)
     got = np.array([x[2 * b + c] for b, c in rowmap])
     assert np.abs(got - ref).max() <= 1e-10 * max(1, np.abs(ref).max())
     # the PV magnitude unknowns of the superset system come out exactly zero
     pv_blocks = [r[0] for r in rowmap[:int((case.bus_type == 2).sum())]]
     assert np.all(x[2 * np.array(pv_blocks, int) + 1] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["pf118x200", "pf1354x200"])
+def test_transposed_replay_solves_the_adjoint_system(name):
+    """k_adj_solve's schedule: (superset block matrix)^T lambda = g against a
+    dense solve of the same block matrix"""
+    case, a, J, F, A, rhs, rowmap = _system(name, seed=1)
+    nb = a["info"]["nblocks"]
+    rp, ci = a["lu_rowptr"], a["lu_colind"]
+    M = np.zeros((2 * nb, 2 * nb))
+    for r in range(nb):
+        for p in range(rp[r], rp[r + 1]):
+            M[2 * r:2 * r + 2, 2 * ci[p]:2 * ci[p] + 2] = A[p]
+    g = np.random.default_rng(2).normal(size=2 * nb)
+    LU, Sinv = _crout(a, A)
+    lam = _sweeps(a, LU, Sinv, g, trans=True)
+    ref = np.linalg.solve(M.T, g)
+    assert np.abs(lam - ref).max() <= 1e-10 * max(1, np.abs(ref).max())
+
+
+def test_tail_inverse_is_the_schur_complement_inverse():
+    """S^-1 from the Gauss-Jordan replay equals the inverse of
+    A_SS - A_SN A_NN^-1 A_NS formed densely"""
+    case, a, J, F, A, rhs, rowmap = _system("pf118x200", seed=3)
+    nb = a["info"]["nblocks"]
+    rp, ci = a["lu_rowptr"], a["lu_colind"]
+    M = np.zeros((2 * nb, 2 * nb))
+    for r in range(nb):
+        for p in range(rp[r], rp[r + 1]):
+            M[2 * r:2 * r + 2, 2 * ci[p]:2 * ci[p] + 2] = A[p]
+    S_ = np.concatenate([[2 * k, 2 * k + 1] for k in a["tail_piv"]])
+    N_ = np.setdiff1d(np.arange(2 * nb), S_)
+    schur = M[np.ix_(S_, S_)] - M[np.ix_(S_, N_)] @ np.linalg.solve(
+        M[np.ix_(N_, N_)], M[np.ix_(N_, S_)])
+    _, Sinv = _crout(a, A)
+    s = a["info"]["tail_n"]
+    got = Sinv.transpose(0, 2, 1, 3).reshape(2 * s, 2 * s)
+    ref = np.linalg.inv(schur)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
 def test_schedule_invariants():
