@@ -105,8 +105,8 @@ typedef struct {
   int64_t lu_pairs;                /* 2x2 block products per factorization */
   int32_t instances;               /* T set by bpf_set_instances */
   int32_t lds_solve;               /* 1: solve vector held in LDS */
-  int32_t crout_tasks;             /* phase-A outputs (one per L+U block) */
-  int32_t crout_scale_tasks;       /* phase-B L-block scalings */
+  int32_t crout_tasks;             /* outputs below the tail (one per block) */
+  int32_t chunks;                  /* factorization chunks below the tail */
   int32_t graph_used;              /* last solve replayed the iteration graph */
   int32_t tail_level;              /* levels >= tail_level: dense LDS block */
   int32_t tail_n;                  /* pivots in the dense tail (<= 64) */
@@ -160,29 +160,33 @@ const char *bpf_last_error(const bpf_engine *e);
 bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info);
 
 /* Host-only symbolic analysis (no device needed): the block order, the L+U
- * block pattern and the level schedules bpf_create builds (the J pattern of
+ * block pattern and the schedules bpf_create builds (the J pattern of
  * newtonpf.h:215-310 extended by the fill of the factorization).  Sizes come
  * back in *info; every array may be NULL, otherwise it must hold:
  * bus_of_blk[nblocks], lu_rowptr[nblocks+1], lu_colind[nnz_lu_blocks],
- * lvA[lu_levels+1], tA_dst/tA_ptr[crout_tasks(+1)], pairL/pairU[lu_pairs],
- * lvB[lu_levels+1], tB_dst/tB_diag[crout_scale_tasks],
+ * ch_task/ch_pair[chunks+1], tk_a/tk_b[4*crout_tasks], pairs[2*lu_pairs],
  * fw_ptr[fw_levels+1], fw_rows[nblocks-tail_n], bw_ptr[bw_levels+1],
  * bw_rows[nblocks-tail_n], tl_dense/tl_pos[tail_entries],
  * tl_ptr[tail_entries+1], tl_pairL/tl_pairU[tail_pairs], tail_piv[tail_n],
- * ts_ptr[tail_n+1], ts_pos/ts_col[tail_row_entries].  tA_dst carries flags:
- * bit 30 = position holds a Jacobian entry, bit 29 = pivot block (stored
- * inverted).  The factorization kernel runs the Crout levels below
- * tail_level and then inverts the Schur complement of the top tail_n pivots
- * (blocks tail_piv) as one dense block by Gauss-Jordan (tl_dense = row *
- * tail_n + col of the dense block, same flag bits; its pairs are the
- * non-tail contributions).  The sweeps solve the non-tail rows by levels
+ * ts_ptr[tail_n+1], ts_pos/ts_col[tail_row_entries].
+ * Factorization below the tail, chunk by chunk: chunk c owns the outputs
+ * ch_task[c]..ch_task[c+1]-1 and the products ch_pair[c]..ch_pair[c+1]-1;
+ * output k has tk_a[4k..] = {position | flags, first product, end product,
+ * 0} with flags bit 30 = the position holds a Jacobian entry, bit 29 = pivot
+ * block (stored inverted), bit 28 = L block (scaled by the pivot inverse,
+ * the pivot's product range in tk_b[4k+1], tk_b[4k+2]); product q is
+ * pairs[2q], pairs[2q+1] = L and U positions, U = -1 for an init entry (the
+ * pivot's own Jacobian block).  An output is its init/Jacobian value minus
+ * the sum of its L*U products.  The top tail_n pivots (blocks tail_piv) are
+ * then finished as one dense block: its Schur complement (tl_dense = row *
+ * tail_n + col, flag bit 30; its pairs are the non-tail contributions) is
+ * inverted by Gauss-Jordan.  The sweeps solve the non-tail rows by levels
  * (fw/bw lists) and the tail rows with that inverse after subtracting their
  * non-tail L entries (ts lists: L position and column per tail row). */
 bpf_status bpf_analyse(const bpf_case *c, bpf_info *info, int32_t *bus_of_blk,
-                       int32_t *lu_rowptr, int32_t *lu_colind, int32_t *lvA,
-                       int32_t *tA_dst, int32_t *tA_ptr, int32_t *pairL,
-                       int32_t *pairU, int32_t *lvB, int32_t *tB_dst,
-                       int32_t *tB_diag, int32_t *fw_ptr, int32_t *fw_rows,
+                       int32_t *lu_rowptr, int32_t *lu_colind, int32_t *ch_task,
+                       int32_t *ch_pair, int32_t *tk_a, int32_t *tk_b,
+                       int32_t *pairs, int32_t *fw_ptr, int32_t *fw_rows,
                        int32_t *bw_ptr, int32_t *bw_rows, int32_t *tl_dense,
                        int32_t *tl_pos, int32_t *tl_ptr, int32_t *tl_pairL,
                        int32_t *tl_pairU, int32_t *tail_piv, int32_t *ts_ptr,

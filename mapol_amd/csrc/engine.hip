@@ -249,10 +249,15 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(g.Gs, e->Gs); UP(g.Qd, e->Qd); UP(g.Va0, e->Va);
   UP(g.base_type, e->bus_type);
   FactorPlan &F = e->fplan;
-  F.nlev = S.tail_level;
-  UP(F.lvA, S.lvA); UP(F.lvA_max, S.lvA_max); UP(F.tA_dst, S.tA_dst);
-  UP(F.tA_pp, S.tA_pp); UP(F.pairL, S.pairL); UP(F.pairU, S.pairU);
-  UP(F.lvB, S.lvB); UP(F.tB_dst, S.tB_dst); UP(F.tB_diag, S.tB_diag);
+  F.nchunk = S.nchunk;
+  UP(F.ch_task, S.ch_task); UP(F.ch_pair, S.ch_pair); UP(F.ch_max, S.ch_max);
+  {
+    const int *ta, *tb, *pr;
+    UP(ta, S.tk_a); UP(tb, S.tk_b); UP(pr, S.pairs);
+    F.tk_a = (const int4 *)ta;
+    F.tk_b = (const int4 *)tb;
+    F.pairs = (const int2 *)pr;
+  }
   F.tn = S.tail_n;
   F.tnent = (int)S.tl_pos.size();
   F.tmax = S.tl_max;
@@ -756,8 +761,8 @@ static void fill_info(const Symbolic &S, bpf_info *info) {
   info->fw_levels = S.nfw;
   info->bw_levels = S.nbw;
   info->lu_pairs = S.npairs;
-  info->crout_tasks = (int)S.tA_dst.size();
-  info->crout_scale_tasks = (int)S.tB_dst.size();
+  info->crout_tasks = (int)S.tk_a.size() / 4;
+  info->chunks = S.nchunk;
   info->tail_level = S.tail_level;
   info->tail_n = S.tail_n;
   info->tail_entries = (int)S.tl_pos.size();
@@ -767,17 +772,15 @@ static void fill_info(const Symbolic &S, bpf_info *info) {
 
 extern "C" bpf_status bpf_analyse(const bpf_case *c, bpf_info *info,
                                   int32_t *bus_of_blk, int32_t *lu_rowptr,
-                                  int32_t *lu_colind, int32_t *lvA,
-                                  int32_t *tA_dst, int32_t *tA_ptr,
-                                  int32_t *pairL, int32_t *pairU, int32_t *lvB,
-                                  int32_t *tB_dst, int32_t *tB_diag,
-                                  int32_t *fw_ptr, int32_t *fw_rows,
-                                  int32_t *bw_ptr, int32_t *bw_rows,
-                                  int32_t *tl_dense, int32_t *tl_pos,
-                                  int32_t *tl_ptr, int32_t *tl_pairL,
-                                  int32_t *tl_pairU, int32_t *tail_piv,
-                                  int32_t *ts_ptr, int32_t *ts_pos,
-                                  int32_t *ts_col) {
+                                  int32_t *lu_colind, int32_t *ch_task,
+                                  int32_t *ch_pair, int32_t *tk_a, int32_t *tk_b,
+                                  int32_t *pairs, int32_t *fw_ptr,
+                                  int32_t *fw_rows, int32_t *bw_ptr,
+                                  int32_t *bw_rows, int32_t *tl_dense,
+                                  int32_t *tl_pos, int32_t *tl_ptr,
+                                  int32_t *tl_pairL, int32_t *tl_pairU,
+                                  int32_t *tail_piv, int32_t *ts_ptr,
+                                  int32_t *ts_pos, int32_t *ts_col) {
   if (!c || !info || c->nbus <= 0 || c->nbranch < 0 || !c->bus_type ||
       (c->nbranch && (!c->br_from || !c->br_to)))
     return BPF_ERR_INVALID_ARG;
@@ -791,9 +794,8 @@ extern "C" bpf_status bpf_analyse(const bpf_case *c, bpf_info *info,
     if (dst && !v.empty()) memcpy(dst, v.data(), sizeof(int32_t) * v.size());
   };
   put(bus_of_blk, S.bus_of_blk); put(lu_rowptr, S.lrow); put(lu_colind, S.lcol);
-  put(lvA, S.lvA); put(tA_dst, S.tA_dst); put(tA_ptr, S.tA_pp);
-  put(pairL, S.pairL); put(pairU, S.pairU); put(lvB, S.lvB);
-  put(tB_dst, S.tB_dst); put(tB_diag, S.tB_diag);
+  put(ch_task, S.ch_task); put(ch_pair, S.ch_pair); put(tk_a, S.tk_a);
+  put(tk_b, S.tk_b); put(pairs, S.pairs);
   put(fw_ptr, S.fw_ptr); put(fw_rows, S.fw_rows);
   put(bw_ptr, S.bw_ptr); put(bw_rows, S.bw_rows);
   put(tl_dense, S.tl_dense); put(tl_pos, S.tl_pos); put(tl_ptr, S.tl_pp);

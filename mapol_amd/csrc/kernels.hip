@@ -371,9 +371,10 @@ __device__ __forceinline__ B4<T> blk_mul(const B4<T> &x, const B4<T> &y) {
 // schedules of the factorization and of the sweeps (device arrays from the
 // symbolic analysis)
 struct FactorPlan {
-  int nlev;                            // Crout levels below the dense tail
-  const int *lvA, *lvA_max, *tA_dst, *tA_pp, *pairL, *pairU;
-  const int *lvB, *tB_dst, *tB_diag;
+  int nchunk;                          // chunks of the levels below the tail
+  const int *ch_task, *ch_pair, *ch_max;
+  const int4 *tk_a, *tk_b;             // per output: {dst, p0, p1}, pivot range
+  const int2 *pairs;                   // {L, U} positions (U < 0: init entry)
   int tn, tnent, tmax;                 // dense tail: pivots, entries, longest list
   const int *tdense, *tpos, *tpp, *tpairL, *tpairU;
 };
@@ -413,6 +414,11 @@ __device__ __forceinline__ void group_sum4(B4<T> &v, int gsz) {
 }
 
 template <typename T>
+__device__ __forceinline__ void blk_acc(B4<T> &acc, const B4<T> &x) {
+  acc.a += x.a; acc.b += x.b; acc.c += x.c; acc.d += x.d;
+}
+
+template <typename T>
 __device__ __forceinline__ B4<T> blk_zero() {
   B4<T> z;
   z.a = z.b = z.c = z.d = (T)0;
@@ -431,10 +437,15 @@ __device__ __forceinline__ bool blk_inv(const B4<T> &a, B4<T> &inv) {
 // ---------------------------------------------------------------------------
 // Batched LU of the 2x2-block Jacobian, static pivot order from the symbolic
 // analysis.  One workgroup per instance.
-// Sparse part, level l of the elimination tree (levels below the tail):
-// phase A computes every output of the level's pivots as A(p) - sum L*U over
-// its precomputed pair list, a lane group per output (the pivot block is
-// stored inverted); phase B scales the level's L blocks by the pivot inverse.
+// Sparse part: the outputs of the levels below the tail (each pivot's block,
+// U row and L column, A(p) - sum L(r,m) U(m,c) over its descendants m) come
+// in chunks of whole pivot groups.  Per chunk, every product is formed by
+// its own thread into LDS (all of them independent loads of finished
+// blocks), then each output reduces its product range with a lane group;
+// a pivot stores its inverse, an L output reduces its pivot's range too and
+// scales by the pivot inverse.  The pivot's own Jacobian block reaches LDS
+// as an init entry of its range, so nothing reads a position another
+// output of the same chunk writes.
 // Dense tail (the top pivots, ancestors of everything else): its Schur
 // complement S = A_SS - L_SN U_NS is assembled in LDS and inverted in place
 // by block Gauss-Jordan with the same pivot order (one barrier per pivot:
@@ -444,43 +455,62 @@ __device__ __forceinline__ bool blk_inv(const B4<T> &a, B4<T> &inv) {
 template <typename T>
 __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
                             FactorPlan P, B4<T> *LU, B4<T> *TL) {
+  static_assert(TAIL_MAX * TAIL_MAX == CHUNK_PAIRS,
+                "the chunk products and the dense tail share one LDS buffer");
   const int t = blockIdx.x;
   if (st[t].done) return;
+  __shared__ B4<T> buf[CHUNK_PAIRS];
   __shared__ int sing;
   if (threadIdx.x == 0) sing = 0;
   B4<T> *A = LU + (size_t)t * g.nnzLU;
   __syncthreads();
-  for (int l = 0; l < P.nlev; l++) {
-    const int k0 = P.lvA[l], k1 = P.lvA[l + 1];
-    const int gsz = lane_group(k1 - k0, P.lvA_max[l]);
+  for (int c = 0; c < P.nchunk; c++) {
+    const int q0 = P.ch_pair[c], q1 = P.ch_pair[c + 1];
+    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+      const int2 pr = P.pairs[q];
+      B4<T> v;
+      if (pr.y < 0) {
+        v = A[pr.x];
+      } else {
+        v = blk_zero<T>();
+        blk_fms(v, A[pr.x], A[pr.y]);
+      }
+      buf[q - q0] = v;
+    }
+    __syncthreads();
+    const int k0 = P.ch_task[c], k1 = P.ch_task[c + 1];
+    const int gsz = lane_group(k1 - k0, P.ch_max[c]);
     const int lane = threadIdx.x & (gsz - 1);
     for (int k = k0 + (int)threadIdx.x / gsz; k < k1; k += (int)blockDim.x / gsz) {
-      const int dst = P.tA_dst[k];
-      const int p = dst & TASK_POS_MASK;
-      B4<T> acc = (lane == 0 && (dst & TASK_HAS_A)) ? A[p] : blk_zero<T>();
-      const int q1 = P.tA_pp[k + 1];
-      for (int q = P.tA_pp[k] + lane; q < q1; q += gsz)
-        blk_fms(acc, A[P.pairL[q]], A[P.pairU[q]]);
-      group_sum4(acc, gsz);
-      if (lane == 0) {
-        if (dst & TASK_DIAG) {
+      const int4 ta = P.tk_a[k];
+      const int p = ta.x & TASK_POS_MASK;
+      B4<T> e = (lane == 0 && (ta.x & TASK_HAS_A)) ? A[p] : blk_zero<T>();
+      for (int q = ta.y + lane; q < ta.z; q += gsz) blk_acc(e, buf[q - q0]);
+      group_sum4(e, gsz);
+      if (ta.x & TASK_LCOL) {
+        const int4 tb = P.tk_b[k];
+        B4<T> pv = blk_zero<T>();
+        for (int q = tb.y + lane; q < tb.z; q += gsz) blk_acc(pv, buf[q - q0]);
+        group_sum4(pv, gsz);
+        if (lane == 0) {
           B4<T> inv;
-          if (blk_inv(acc, inv)) sing = 1;
+          blk_inv(pv, inv);   // a singular pivot is flagged by its own output
+          A[p] = blk_mul(e, inv);
+        }
+      } else if (lane == 0) {
+        if (ta.x & TASK_DIAG) {
+          B4<T> inv;
+          if (blk_inv(e, inv)) sing = 1;
           A[p] = inv;
         } else {
-          A[p] = acc;
+          A[p] = e;
         }
       }
     }
     __syncthreads();
-    for (int k = P.lvB[l] + threadIdx.x; k < P.lvB[l + 1]; k += blockDim.x) {
-      int p = P.tB_dst[k];
-      A[p] = blk_mul(A[p], A[P.tB_diag[k]]);
-    }
-    __syncthreads();
   }
   if (P.tn > 0) {
-    __shared__ B4<T> D[TAIL_MAX * TAIL_MAX];
+    B4<T> *D = buf;
     __shared__ B4<T> colb[2][TAIL_MAX], rowb[2][TAIL_MAX], pvb[2];
     const int s = P.tn, ss = s * s;
     for (int q = threadIdx.x; q < ss; q += blockDim.x) D[q] = blk_zero<T>();

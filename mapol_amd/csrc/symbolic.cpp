@@ -1,6 +1,7 @@
 // Symbolic analysis: Ybus pattern, bus ordering, L+U block pattern and the
 // level schedules of the batched factorization and triangular solves.
 #include <algorithm>
+#include <map>
 #include <numeric>
 #include <string>
 
@@ -228,24 +229,51 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   S.nlev = nb ? 1 + *std::max_element(lev.begin(), lev.end()) : 0;
   std::vector<std::vector<int>> piv(S.nlev);
   for (int k = 0; k < nb; k++) piv[lev[k]].push_back(k);
+  // dense tail: the top levels holding at most TAIL_MAX pivots together
+  // (their pivots are ancestors of everything else, so they can go last)
+  {
+    std::vector<int> cnt(S.nlev + 1, 0);  // pivots at levels >= l
+    for (int k = 0; k < nb; k++) cnt[lev[k]]++;
+    for (int l = S.nlev - 1; l >= 0; l--) cnt[l] += cnt[l + 1];
+    int Ls = S.nlev;
+    while (Ls > 0 && cnt[Ls - 1] <= TAIL_MAX) Ls--;
+    // worth it only when it removes a few levels
+    if (S.nlev - Ls < 4) Ls = S.nlev;
+    S.tail_level = Ls;
+  }
+  // Each output of a pivot (its block, U row, L column) is A(p) minus a dot
+  // product over the pivot's descendants m: sum L(r,m) U(m,c).  The outputs
+  // of the levels below the tail are packed, whole pivot groups at a time,
+  // into chunks of at most CHUNK_PAIRS products: the kernel forms a chunk's
+  // products in parallel into LDS, then reduces them per output.  An L
+  // output also reduces its pivot's products (same chunk) and scales by the
+  // pivot inverse, so no separate scaling pass is needed.
   struct Task { int dst; std::vector<int> L, U; };
-  S.lvA.assign(1, 0);
-  S.lvB.assign(1, 0);
-  S.tA_pp.assign(1, 0);
+  S.ch_task.assign(1, 0);
+  S.ch_pair.assign(1, 0);
+  S.lv_chunk.assign(1, 0);
+  int64_t ntask_total = 0;
   for (int l = 0; l < S.nlev; l++) {
-    std::vector<Task> tasks;
+    if (l >= S.tail_level) { S.lv_chunk.push_back((int)S.ch_max.size()); continue; }
+    // pivot groups of the level
+    std::vector<std::vector<Task>> groups;
     for (int k : piv[l]) {
-      // outputs of pivot k: diag, U row k, L column k (rows = st[k])
       const auto &sk = st[k];
       std::vector<Task> out(1 + 2 * sk.size());
-      out[0].dst = S.ldiag[k] | TASK_DIAG;
+      // the pivot's own value enters its product range as an init entry
+      // (U position -1): the L outputs of the group read the pivot's sum
+      // from LDS while the pivot output overwrites its position
+      const int pk = S.ldiag[k];
+      out[0].dst = pk | TASK_DIAG;
+      if (inJ[pk]) { out[0].L.push_back(pk); out[0].U.push_back(-1); }
       for (size_t a = 0; a < sk.size(); a++) {
-        out[1 + a].dst = pos(k, sk[a]);
-        out[1 + sk.size() + a].dst = pos(sk[a], k);
+        int pu = pos(k, sk[a]), pl = pos(sk[a], k);
+        out[1 + a].dst = pu | (inJ[pu] ? TASK_HAS_A : 0);
+        out[1 + sk.size() + a].dst = pl | (inJ[pl] ? TASK_HAS_A : 0) | TASK_LCOL;
       }
       for (int mm : lrowv[k]) {          // descendants m with k in st[m]
         int pkm = pos(k, mm), pmk = pos(mm, k);
-        // U_kk += L_km U_mk
+        // U_kk -= L_km U_mk
         out[0].L.push_back(pkm);
         out[0].U.push_back(pmk);
         for (int j : st[mm]) {
@@ -258,40 +286,80 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
           out[1 + sk.size() + a].U.push_back(pmk);
         }
       }
-      for (size_t a = 0; a < sk.size(); a++) {
-        S.tB_dst.push_back(out[1 + sk.size() + a].dst);
-        S.tB_diag.push_back(S.ldiag[k]);
+      groups.push_back(std::move(out));
+    }
+    auto gpairs = [](const std::vector<Task> &g) {
+      size_t n = 0;
+      for (auto &t : g) n += t.L.size();
+      return n;
+    };
+    std::stable_sort(groups.begin(), groups.end(),
+                     [&](const std::vector<Task> &x, const std::vector<Task> &y) {
+                       return gpairs(x) > gpairs(y);
+                     });
+    // pack groups into chunks (first fit in order of decreasing size)
+    std::vector<std::vector<int>> chunks;   // group indices
+    std::vector<size_t> fill;
+    for (int gi = 0; gi < (int)groups.size(); gi++) {
+      size_t np = gpairs(groups[gi]);
+      int c = -1;
+      for (int q = 0; q < (int)chunks.size(); q++)
+        if (fill[q] + np <= (size_t)CHUNK_PAIRS) { c = q; break; }
+      if (c < 0) { chunks.push_back({}); fill.push_back(0); c = (int)chunks.size() - 1; }
+      chunks[c].push_back(gi);
+      fill[c] += np;
+    }
+    for (auto &cg : chunks) {
+      // tasks of the chunk, heaviest first (an L output also reduces its
+      // pivot's products)
+      std::vector<std::pair<int, int>> ord;  // (group, index in group)
+      for (int gi : cg)
+        for (int q = 0; q < (int)groups[gi].size(); q++) ord.push_back({gi, q});
+      auto work = [&](const std::pair<int, int> &x) {
+        const Task &t = groups[x.first][x.second];
+        size_t w = t.L.size();
+        if (t.dst & TASK_LCOL) w = std::max(w, groups[x.first][0].L.size());
+        return w;
+      };
+      std::stable_sort(ord.begin(), ord.end(),
+                       [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                         return work(x) > work(y);
+                       });
+      // pair offsets of every task (in chunk order)
+      const int base = (int)S.pairs.size() / 2;
+      std::map<std::pair<int, int>, int> start;
+      int off = base;
+      for (auto &x : ord) { start[x] = off; off += (int)groups[x.first][x.second].L.size(); }
+      int wmax = 0;
+      for (auto &x : ord) {
+        const Task &t = groups[x.first][x.second];
+        const int p0 = start[x], p1 = p0 + (int)t.L.size();
+        S.tk_a.insert(S.tk_a.end(), {t.dst, p0, p1, 0});
+        if (t.dst & TASK_LCOL) {
+          const std::pair<int, int> pv{x.first, 0};
+          const int v0 = start[pv], v1 = v0 + (int)groups[x.first][0].L.size();
+          S.tk_b.insert(S.tk_b.end(), {0, v0, v1, 0});
+        } else {
+          S.tk_b.insert(S.tk_b.end(), {0, 0, 0, 0});
+        }
+        for (size_t q = 0; q < t.L.size(); q++) {
+          S.pairs.push_back(t.L[q]);
+          S.pairs.push_back(t.U[q]);
+        }
+        wmax = std::max(wmax, (int)work(x));
+        ntask_total++;
       }
-      for (auto &t : out) tasks.push_back(std::move(t));
+      S.ch_task.push_back((int)ntask_total);
+      S.ch_pair.push_back((int)S.pairs.size() / 2);
+      S.ch_max.push_back(wmax);
     }
-    // longest dot products first: better balance across the workgroup
-    std::stable_sort(tasks.begin(), tasks.end(), [](const Task &a, const Task &b) {
-      return a.L.size() > b.L.size();
-    });
-    S.lvA_max.push_back(tasks.empty() ? 0 : (int)tasks.front().L.size());
-    for (auto &t : tasks) {
-      int p = t.dst & TASK_POS_MASK;
-      int flags = t.dst & ~TASK_POS_MASK;
-      if (inJ[p]) flags |= TASK_HAS_A;
-      S.tA_dst.push_back(p | flags);
-      S.pairL.insert(S.pairL.end(), t.L.begin(), t.L.end());
-      S.pairU.insert(S.pairU.end(), t.U.begin(), t.U.end());
-      S.tA_pp.push_back((int)S.pairL.size());
-    }
-    S.lvA.push_back((int)S.tA_dst.size());
-    S.lvB.push_back((int)S.tB_dst.size());
+    S.lv_chunk.push_back((int)S.ch_max.size());
   }
-  S.npairs = (int64_t)S.pairL.size();
+  S.nchunk = (int)S.ch_max.size();
+  S.npairs = (int64_t)S.pairs.size() / 2;
   // ---- dense tail ------------------------------------------------------------
   {
-    std::vector<int> cnt(S.nlev + 1, 0);  // pivots at levels >= l
-    for (int k = 0; k < nb; k++) cnt[lev[k]]++;
-    for (int l = S.nlev - 1; l >= 0; l--) cnt[l] += cnt[l + 1];
-    int Ls = S.nlev;
-    while (Ls > 0 && cnt[Ls - 1] <= TAIL_MAX) Ls--;
-    // worth it only when it removes a few levels
-    if (S.nlev - Ls < 4) Ls = S.nlev;
-    S.tail_level = Ls;
+    const int Ls = S.tail_level;
     std::vector<int> loc(nb, -1);
     for (int k = 0; k < nb; k++)
       if (lev[k] >= Ls) { loc[k] = (int)S.tail_piv.size(); S.tail_piv.push_back(k); }

@@ -14,7 +14,9 @@ enum { KFF = 0, KFT = 1, KTF = 2, KTT = 3 };
 // Crout task destination flags (bits above the block position)
 constexpr int TASK_HAS_A = 1 << 30;  // the position holds a Jacobian entry
 constexpr int TASK_DIAG = 1 << 29;   // pivot block: store its inverse
-constexpr int TASK_POS_MASK = (1 << 29) - 1;
+constexpr int TASK_LCOL = 1 << 28;   // L block: scale by the pivot inverse
+constexpr int TASK_POS_MASK = (1 << 28) - 1;
+constexpr int CHUNK_PAIRS = 4096;    // products per factorization chunk (LDS)
 constexpr int TAIL_MAX = 64;         // dense tail: at most 64 blocks square
 
 struct Symbolic {
@@ -36,9 +38,18 @@ struct Symbolic {
   int nnz_jac_blocks = 0;
   // Crout factorization schedule (one level = one elimination-tree height)
   int nlev = 0;
-  std::vector<int> lvA, tA_dst, tA_pp, pairL, pairU;  // phase A: dot products
-  std::vector<int> lvA_max;                           // longest pair list per level
-  std::vector<int> lvB, tB_dst, tB_diag;              // phase B: L scaling
+  // chunks of the levels below the tail, in execution order; level l owns
+  // chunks lv_chunk[l] .. lv_chunk[l+1]-1
+  int nchunk = 0;
+  std::vector<int> lv_chunk;
+  std::vector<int> ch_task, ch_pair;  // task / pair offsets [nchunk+1]
+  std::vector<int> ch_max;            // longest reduction of the chunk
+  // per task: tk_a = {dst | flags, first pair, end pair, 0};
+  // tk_b = {0, the pivot task's first pair, end pair, 0} (L tasks)
+  std::vector<int> tk_a, tk_b;
+  // {L position, U position} per product; U position -1: init entry, the
+  // value at the L position itself (a pivot's Jacobian block)
+  std::vector<int> pairs;
   int64_t npairs = 0;
   // dense tail: the pivots of the top levels (>= tail_level, at most
   // TAIL_MAX of them, ascending = topological) are factored as one dense

@@ -63,7 +63,7 @@ class BpfInfo(C.Structure):
                 ("lu_levels", C.c_int32), ("fw_levels", C.c_int32),
                 ("bw_levels", C.c_int32), ("lu_pairs", C.c_int64),
                 ("instances", C.c_int32), ("lds_solve", C.c_int32),
-                ("crout_tasks", C.c_int32), ("crout_scale_tasks", C.c_int32),
+                ("crout_tasks", C.c_int32), ("chunks", C.c_int32),
                 ("graph_used", C.c_int32), ("tail_level", C.c_int32),
                 ("tail_n", C.c_int32), ("tail_entries", C.c_int32),
                 ("tail_pairs", C.c_int32), ("tail_row_entries", C.c_int32)]
@@ -109,7 +109,7 @@ def load_library(path: str = LIB_PATH):
     L.bpf_set_timing.argtypes = [V, C.c_int32]
     L.bpf_get_kernel_time.argtypes = [V, C.c_char_p, _pi, _pd]
     L.bpf_set_launch.argtypes = [V, C.c_int32, C.c_int32, C.c_int32]
-    L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 24
+    L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 21
     for name in EXPORTS:
         getattr(L, name)
     _LIB = L
@@ -148,7 +148,7 @@ def analyse(case):
     keep = []
     s = _case_struct(case, keep)
     info = BpfInfo()
-    st = L.bpf_analyse(C.byref(s), C.byref(info), *([None] * 24))
+    st = L.bpf_analyse(C.byref(s), C.byref(info), *([None] * 21))
     if st != 0:
         raise EngineError(f"bpf_analyse failed ({st})")
     i = {f: getattr(info, f) for f, _ in BpfInfo._fields_}
@@ -156,12 +156,11 @@ def analyse(case):
     out = dict(
         bus_of_blk=np.zeros(nb, np.int32), lu_rowptr=np.zeros(nb + 1, np.int32),
         lu_colind=np.zeros(i["nnz_lu_blocks"], np.int32),
-        lvA=np.zeros(nl + 1, np.int32), tA_dst=np.zeros(i["crout_tasks"], np.int32),
-        tA_ptr=np.zeros(i["crout_tasks"] + 1, np.int32),
-        pairL=np.zeros(i["lu_pairs"], np.int32), pairU=np.zeros(i["lu_pairs"], np.int32),
-        lvB=np.zeros(nl + 1, np.int32),
-        tB_dst=np.zeros(i["crout_scale_tasks"], np.int32),
-        tB_diag=np.zeros(i["crout_scale_tasks"], np.int32),
+        ch_task=np.zeros(i["chunks"] + 1, np.int32),
+        ch_pair=np.zeros(i["chunks"] + 1, np.int32),
+        tk_a=np.zeros((i["crout_tasks"], 4), np.int32),
+        tk_b=np.zeros((i["crout_tasks"], 4), np.int32),
+        pairs=np.zeros((i["lu_pairs"], 2), np.int32),
         fw_ptr=np.zeros(i["fw_levels"] + 1, np.int32),
         fw_rows=np.zeros(nb - i["tail_n"], np.int32),
         bw_ptr=np.zeros(i["bw_levels"] + 1, np.int32),
@@ -175,8 +174,8 @@ def analyse(case):
         ts_ptr=np.zeros(i["tail_n"] + 1, np.int32),
         ts_pos=np.zeros(i["tail_row_entries"], np.int32),
         ts_col=np.zeros(i["tail_row_entries"], np.int32))
-    order = ["bus_of_blk", "lu_rowptr", "lu_colind", "lvA", "tA_dst", "tA_ptr",
-             "pairL", "pairU", "lvB", "tB_dst", "tB_diag", "fw_ptr", "fw_rows",
+    order = ["bus_of_blk", "lu_rowptr", "lu_colind", "ch_task", "ch_pair",
+             "tk_a", "tk_b", "pairs", "fw_ptr", "fw_rows",
              "bw_ptr", "bw_rows", "tl_dense", "tl_pos", "tl_ptr", "tl_pairL",
              "tl_pairU", "tail_piv", "ts_ptr", "ts_pos", "ts_col"]
     st = L.bpf_analyse(C.byref(s), C.byref(info),

@@ -13,7 +13,7 @@ import scipy.sparse.linalg as sla
 import oracle
 from mapol_amd import engine, synth
 
-HAS_A, DIAG, MASK = 1 << 30, 1 << 29, (1 << 29) - 1
+HAS_A, DIAG, LCOL, MASK = 1 << 30, 1 << 29, 1 << 28, (1 << 28) - 1
 
 pytestmark = pytest.mark.skipif(not __import__("os").path.exists(engine.LIB_PATH),
                                 reason="libbatchpf.so not built")
@@ -52,21 +52,30 @@ def _blocks(case, a, J, F):
 
 
 def _crout(a, A):
-    """the kernel's factorization: Crout levels below the dense tail (phase A
-    dot products, inverted pivots, phase B scaling), then the tail's Schur
-    complement assembled from its pair lists and inverted by block
-    Gauss-Jordan in the kernel's pivot order.  Returns (LU blocks, S^-1)."""
+    """the kernel's factorization: chunks of the levels below the dense tail
+    (each product L*U of a chunk formed from finished blocks, then every
+    output = its Jacobian value or init entry minus its products; a pivot
+    stores its inverse, an L output is scaled by its pivot's inverse), then
+    the tail's Schur complement assembled from its pair lists and inverted by
+    block Gauss-Jordan in the kernel's pivot order.  Returns (LU, S^-1)."""
     A = A.copy()
-    for l in range(a["info"]["tail_level"]):
-        for k in range(a["lvA"][l], a["lvA"][l + 1]):
-            d = int(a["tA_dst"][k])
+    for c in range(a["info"]["chunks"]):
+        q0, q1 = a["ch_pair"][c], a["ch_pair"][c + 1]
+        prod = np.zeros((q1 - q0, 2, 2))
+        for q in range(q0, q1):
+            l, u = a["pairs"][q]
+            prod[q - q0] = A[l] if u < 0 else -(A[l] @ A[u])
+        for k in range(a["ch_task"][c], a["ch_task"][c + 1]):
+            d, p0, p1, _ = a["tk_a"][k]
             p = d & MASK
-            acc = A[p].copy() if d & HAS_A else np.zeros((2, 2))
-            for q in range(a["tA_ptr"][k], a["tA_ptr"][k + 1]):
-                acc -= A[a["pairL"][q]] @ A[a["pairU"][q]]
-            A[p] = np.linalg.inv(acc) if d & DIAG else acc
-        for k in range(a["lvB"][l], a["lvB"][l + 1]):
-            A[a["tB_dst"][k]] = A[a["tB_dst"][k]] @ A[a["tB_diag"][k]]
+            e = (A[p].copy() if d & HAS_A else np.zeros((2, 2))) \
+                + prod[p0 - q0:p1 - q0].sum(0)
+            if d & LCOL:
+                _, v0, v1, _ = a["tk_b"][k]
+                e = e @ np.linalg.inv(prod[v0 - q0:v1 - q0].sum(0))
+            elif d & DIAG:
+                e = np.linalg.inv(e)
+            A[p] = e
     s = a["info"]["tail_n"]
     D = np.zeros((s, s, 2, 2))
     for k in range(a["info"]["tail_entries"]):
@@ -216,17 +225,35 @@ def test_schedule_invariants():
     pat = {(r, c) for r in range(nb) for c in ci[rp[r]:rp[r + 1]]}
     assert all((c, r) in pat for r, c in pat)
     assert all((r, r) in pat for r in range(nb))
-    # every position is written exactly once by phase A
-    dst = a["tA_dst"] & MASK
-    assert np.array_equal(np.sort(dst), np.arange(nnz))
-    # a pair only reads positions finished at an earlier level
-    done_at = np.empty(nnz, int)
-    for l in range(i["lu_levels"]):
-        done_at[dst[a["lvA"][l]:a["lvA"][l + 1]]] = l
-    for l in range(i["lu_levels"]):
-        for k in range(a["lvA"][l], a["lvA"][l + 1]):
-            q0, q1 = a["tA_ptr"][k], a["tA_ptr"][k + 1]
-            assert np.all(done_at[a["pairL"][q0:q1]] < l)
-            assert np.all(done_at[a["pairU"][q0:q1]] < l)
-    # every block of the Jacobian pattern holds a Jacobian value
-    assert i["nnz_jac_blocks"] == int((a["tA_dst"] & HAS_A > 0).sum())
+    # every position is written exactly once: by a chunk output or, for the
+    # blocks of the dense tail, not at all (the tail keeps its inverse)
+    dst = a["tk_a"][:, 0] & MASK
+    tail = set(int(k) for k in a["tail_piv"])
+    in_tail = np.array([(r in tail and ci[p] in tail)
+                        for r in range(nb) for p in range(rp[r], rp[r + 1])])
+    assert len(np.unique(dst)) == len(dst)
+    assert np.array_equal(np.sort(dst), np.flatnonzero(~in_tail))
+    # a product only reads positions finished in an earlier chunk (or its
+    # own pivot's block, as an init entry); an L output's pivot range lies
+    # in its own chunk
+    done_at = np.full(nnz, 10 ** 9)
+    for c in range(i["chunks"]):
+        done_at[dst[a["ch_task"][c]:a["ch_task"][c + 1]]] = c
+    for c in range(i["chunks"]):
+        q0, q1 = a["ch_pair"][c], a["ch_pair"][c + 1]
+        assert q1 - q0 <= 4096
+        for l, u in a["pairs"][q0:q1]:
+            if u < 0:
+                assert done_at[l] == c
+            else:
+                assert done_at[l] < c and done_at[u] < c
+        for k in range(a["ch_task"][c], a["ch_task"][c + 1]):
+            d, p0, p1, _ = a["tk_a"][k]
+            assert q0 <= p0 <= p1 <= q1
+            if d & LCOL:
+                assert q0 <= a["tk_b"][k][1] <= a["tk_b"][k][2] <= q1
+    # every block of the Jacobian pattern outside the tail holds a Jacobian
+    # value: directly (HAS_A) or as its pivot's init entry
+    n_init = int((a["pairs"][:, 1] < 0).sum())
+    assert n_init + int((a["tk_a"][:, 0] & HAS_A > 0).sum()) \
+        + int((a["tl_dense"] & HAS_A > 0).sum()) == i["nnz_jac_blocks"]
