@@ -112,6 +112,8 @@ typedef struct {
   int32_t tail_n;                  /* pivots in the dense tail (<= 64) */
   int32_t tail_entries, tail_pairs;
   int32_t tail_row_entries;        /* non-tail L entries of the tail rows */
+  int32_t sweep_fw_chunks, sweep_bw_chunks;  /* entry-parallel sweep chunks */
+  int32_t sweep_rows, sweep_entries;
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */
@@ -191,6 +193,19 @@ bpf_status bpf_analyse(const bpf_case *c, bpf_info *info, int32_t *bus_of_blk,
                        int32_t *tl_pos, int32_t *tl_ptr, int32_t *tl_pairL,
                        int32_t *tl_pairU, int32_t *tail_piv, int32_t *ts_ptr,
                        int32_t *ts_pos, int32_t *ts_col);
+
+/* Host-only: the entry-parallel form of the sweeps (used when the solve
+ * vector and a product buffer fit in LDS).  The non-tail rows of every
+ * level, in the level order of fw/bw_rows, are packed into chunks of at most
+ * 2048 entries; chunks 0 .. sweep_fw_chunks-1 are the forward sweep, the
+ * rest the backward sweep, and the tail rows are solved between them.
+ * sw_chunk[4*(fw+bw chunks)] = {first row, end row, first entry, end entry};
+ * sw_row[4*sweep_rows] = {block row, first entry, end entry, pivot position
+ * (backward) or -1}; sw_ent[2*sweep_entries] = {L or U position, column
+ * block}.  Arrays may be NULL. */
+bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
+                              int32_t *sw_chunk, int32_t *sw_row,
+                              int32_t *sw_ent);
 
 /* ---- stage-level entry points (tests, benchmarks) ---------------------- */
 

@@ -33,6 +33,19 @@ struct TimedEvent {
 
 static unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
 
+// entry-parallel sweeps: X and the product buffer both in the solve LDS
+static int use_entry_sweeps(const Symbolic &S, size_t scalar, int use_lds,
+                            int forced_rows) {
+  if (!use_lds || !S.sw_ok || forced_rows) return 0;
+  return ((size_t)ep_offset(S.nb) + 2 * (size_t)SWEEP_CHUNK) * scalar <=
+         (size_t)SOLVE_LDS_BYTES;
+}
+// 1-D grid of the kernels that keep an instance's workgroups on one XCD
+// (xcd_slot): 8 * ceil(T / 8) * parts workgroups
+static dim3 xgrid(int T, size_t count, unsigned threads) {
+  return dim3(8u * div_up((size_t)T, 8) * div_up(count, threads));
+}
+
 struct bpf_engine {
   bpf_options opt;
   std::string err;
@@ -78,6 +91,7 @@ struct bpf_engine {
   std::string launch_err;            // first failed launch (kernel name)
   const char *cur_kernel = "";
   int debug_sync = 0;                // BPF_DEBUG_SYNC: sync after launches
+  int sweep_rows = 0;                // BPF_SWEEP_ROWS: level-row sweeps only
   bool capturing = false;            // recording the iteration graph
   int cap_slot = 0;
   std::vector<TimedEvent> fixed_ev;  // event nodes of the timed graph
@@ -272,6 +286,18 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   SP.tsmax = S.ts_max;
   UP(SP.tpiv, S.tail_piv); UP(SP.ts_ptr, S.ts_ptr); UP(SP.ts_pos, S.ts_pos);
   UP(SP.ts_col, S.ts_col); UP(SP.ts_mir, S.ts_mir);
+  SP.sw_nfw = S.sw_nfw;
+  SP.sw_nbw = S.sw_nbw;
+  UP(SP.sw_cmax, S.sw_cmax);
+  {
+    const int *ch, *rw, *en;
+    UP(ch, S.sw_chunk); UP(rw, S.sw_row); UP(en, S.sw_ent);
+    SP.sw_chunk = (const int4 *)ch;
+    SP.sw_row = (const int4 *)rw;
+    SP.sw_ent = (const int2 *)en;
+  }
+  const char *swr = getenv("BPF_SWEEP_ROWS");
+  e->sweep_rows = (swr && swr[0] == '1') ? 1 : 0;
 #undef UP
   return BPF_OK;
 }
@@ -396,7 +422,7 @@ extern "C" bpf_status bpf_set_instances(bpf_engine *e, int32_t T,
   AL(e->d_Sf, cd, Tm);
   AL(e->d_St, cd, Tm);
   AL(e->d_loss, double, T);
-  AL(d.part, double, (size_t)T * div_up(nb, 64));
+  AL(d.part, double, (size_t)T * div_up(e->n, 64));
 #undef AL
   return BPF_OK;
 }
@@ -500,9 +526,10 @@ static void tcollect(bpf_engine *e) {
 // mismatch rows over many workgroups, then one verdict workgroup per instance
 static void mismatch_pair(bpf_engine *e, int mode, int maxit, int qlim,
                           int fd_norm) {
-  const unsigned nparts = div_up(e->S.nb, e->mis_threads);
-  hipLaunchKernelGGL(k_mismatch_mb, dim3(nparts, e->T), dim3(e->mis_threads), 0,
-                     e->stream, e->g, e->d, mode, fd_norm);
+  const unsigned nparts = div_up(e->n, e->mis_threads);
+  hipLaunchKernelGGL(k_mismatch_mb, xgrid(e->T, e->n, e->mis_threads),
+                     dim3(e->mis_threads), 0, e->stream, e->g, e->d, mode,
+                     fd_norm);
   hipLaunchKernelGGL(k_verdict, dim3(e->T), dim3(256), 0, e->stream, e->g, e->d,
                      (int)nparts, mode, maxit, e->opt.tol, qlim,
                      e->opt.max_qlim_rounds, e->opt.qlim_tol, fd_norm);
@@ -520,7 +547,7 @@ template <typename T>
 static void launch_newton_step(bpf_engine *e) {
   B4<T> *LU = (B4<T> *)e->d_LU;
   tbegin(e, "jacobian");
-  hipLaunchKernelGGL(k_jacobian<T>, dim3(div_up(e->g.nnzY, 256), e->T),
+  hipLaunchKernelGGL(k_jacobian<T>, xgrid(e->T, e->g.nnzY, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU);
   tend(e);
   tbegin(e, "lu_factor");
@@ -535,7 +562,9 @@ static void launch_newton_step(bpf_engine *e) {
   hipLaunchKernelGGL(k_lu_solve<T>, dim3(e->T), dim3(e->solve_threads),
                      0, e->stream, e->g, e->d,
                      (const B4<T> *)LU, (const B4<T> *)e->d_TL, refine, 0,
-                     e->splan, use_lds, (T *)e->d_xglob);
+                     e->splan, use_lds,
+                     use_entry_sweeps(e->S, sizeof(T), use_lds, e->sweep_rows),
+                     (T *)e->d_xglob);
   tend(e);
 }
 
@@ -543,7 +572,7 @@ static void launch_iteration(bpf_engine *e) {
   if (e->lu_fp64) launch_newton_step<double>(e);
   else launch_newton_step<float>(e);
   tbegin(e, "update");
-  hipLaunchKernelGGL(k_update, dim3(div_up(e->S.nb, 256), e->T), dim3(256), 0,
+  hipLaunchKernelGGL(k_update, xgrid(e->T, e->n, 256), dim3(256), 0,
                      e->stream, e->g, e->d);
   tend(e);
   launch_mismatch(e, 1);
@@ -644,10 +673,13 @@ static bpf_status run_fd(bpf_engine *e) {
       hipLaunchKernelGGL(k_lu_solve<double>, dim3(T), dim3(e->solve_threads),
                          0, e->stream, e->g, e->d, (const B4<double> *)LU,
                          (const B4<double> *)e->d_TL, 0, half, e->splan,
-                         use_lds, (double *)e->d_xglob);
+                         use_lds,
+                         use_entry_sweeps(e->S, sizeof(double), use_lds,
+                                          e->sweep_rows),
+                         (double *)e->d_xglob);
       tend(e);
       tbegin(e, "update");
-      hipLaunchKernelGGL(k_fd_update, dim3(div_up(e->S.nb, 256), T), dim3(256),
+      hipLaunchKernelGGL(k_fd_update, xgrid(T, e->n, 256), dim3(256),
                          0, e->stream, e->g, e->d, half);
       tend(e);
       tbegin(e, "mismatch");
@@ -668,11 +700,11 @@ static bpf_status solve_once(bpf_engine *e, bool *graph_failed) {
   const bool fd = e->opt.alg != BPF_NR;
   const unsigned T = (unsigned)e->T;
   tbegin(e, "ybus");
-  hipLaunchKernelGGL(k_ybus, dim3(div_up(e->g.nnzY, 256), T), dim3(256), 0,
+  hipLaunchKernelGGL(k_ybus, xgrid(T, e->g.nnzY, 256), dim3(256), 0,
                      e->stream, e->g, e->d);
   tend(e);
   tbegin(e, "init");
-  hipLaunchKernelGGL(k_init, dim3(div_up(e->n, 256), T), dim3(256), 0,
+  hipLaunchKernelGGL(k_init, xgrid(T, e->n, 256), dim3(256), 0,
                      e->stream, e->g, e->d, (const int8_t *)e->d_type0,
                      (const double *)e->d_Qinj0);
   tend(e);
@@ -768,6 +800,29 @@ static void fill_info(const Symbolic &S, bpf_info *info) {
   info->tail_entries = (int)S.tl_pos.size();
   info->tail_pairs = (int)S.tl_pairL.size();
   info->tail_row_entries = (int)S.ts_pos.size();
+  info->sweep_fw_chunks = S.sw_nfw;
+  info->sweep_bw_chunks = S.sw_nbw;
+  info->sweep_rows = (int)S.sw_row.size() / 4;
+  info->sweep_entries = (int)S.sw_ent.size() / 2;
+}
+
+extern "C" bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
+                                         int32_t *sw_chunk, int32_t *sw_row,
+                                         int32_t *sw_ent) {
+  if (!c || !info || c->nbus <= 0 || c->nbranch < 0 || !c->bus_type ||
+      (c->nbranch && (!c->br_from || !c->br_to)))
+    return BPF_ERR_INVALID_ARG;
+  Symbolic S;
+  std::string err;
+  if (!analyse(c->nbus, c->nbranch, c->br_from, c->br_to, c->bus_type, S, &err))
+    return BPF_ERR_INVALID_ARG;
+  memset(info, 0, sizeof(*info));
+  fill_info(S, info);
+  auto put = [](int32_t *dst, const std::vector<int> &v) {
+    if (dst && !v.empty()) memcpy(dst, v.data(), sizeof(int32_t) * v.size());
+  };
+  put(sw_chunk, S.sw_chunk); put(sw_row, S.sw_row); put(sw_ent, S.sw_ent);
+  return BPF_OK;
 }
 
 extern "C" bpf_status bpf_analyse(const bpf_case *c, bpf_info *info,
@@ -906,7 +961,7 @@ extern "C" bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V,
   HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
   B4<double> *LU = (B4<double> *)e->d_LU;
   HIPCHK(hipMemset(LU, 0, sizeof(B4<double>) * (size_t)e->T * e->g.nnzLU));
-  hipLaunchKernelGGL(k_jacobian<double>, dim3(div_up(e->g.nnzY, 256), e->T),
+  hipLaunchKernelGGL(k_jacobian<double>, xgrid(e->T, e->g.nnzY, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
@@ -1016,7 +1071,7 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   B4<double> *LU = (B4<double> *)e->d_LU;
   const unsigned TT = (unsigned)T;
   tbegin(e, "adjoint");
-  hipLaunchKernelGGL(k_jacobian<double>, dim3(div_up(e->g.nnzY, 256), TT),
+  hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, e->g.nnzY, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
                      e->stream, e->g, e->d.st, 0, e->fplan, LU,

@@ -63,14 +63,37 @@ __device__ cd branch_entry(const DevTopo &g, const DevInst &d, int t, int b,
 }
 
 // ---------------------------------------------------------------------------
+// Placement of kernels that give one instance several workgroups: every
+// workgroup of an instance runs on the same XCD, so the instance's vectors
+// (V, Ibus, ...) are fetched into one L2 only.  Workgroups are dispatched to
+// the 8 XCDs round-robin (w mod 8), so instance t takes the workgroups
+// w = 8 * (j * parts + part) + t mod 8 with j = t / 8; the 1-D grid holds
+// 8 * ceil(T / 8) * parts workgroups and the surplus ones return.  Only the
+// placement depends on the dispatch order, never the result.
+// ---------------------------------------------------------------------------
+struct XSlot {
+  int t, part;
+};
+__device__ __forceinline__ XSlot xcd_slot(int parts) {
+  const int w = blockIdx.x, s = w >> 3;
+  XSlot r;
+  r.t = (w & 7) + 8 * (s / parts);
+  r.part = s - (s / parts) * parts;
+  return r;
+}
+__device__ __forceinline__ int parts_of(int count) {
+  return (count + (int)blockDim.x - 1) / (int)blockDim.x;
+}
+
+// ---------------------------------------------------------------------------
 // Ybus assembly with outage masks: one thread per (position, instance).
 // Sum order = the reference's: from-side terms, then to-side terms, then the
 // bus shunt (Cf*Yf + Ct*Yt + Ysh, makeYbus.h:196-266).
 // ---------------------------------------------------------------------------
 __global__ void k_ybus(DevTopo g, DevInst d) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  int t = blockIdx.y;
-  if (p >= g.nnzY) return;
+  const XSlot sl = xcd_slot(parts_of(g.nnzY));
+  const int t = sl.t, p = sl.part * blockDim.x + threadIdx.x;
+  if (t >= d.T || p >= g.nnzY) return;
   cd af = mk(0.0, 0.0), at = mk(0.0, 0.0);
   for (int q = g.yf_ptr[p]; q < g.yf_ptr[p + 1]; q++) {
     int it = g.yf_item[q];
@@ -91,8 +114,9 @@ __global__ void k_ybus(DevTopo g, DevInst d) {
 // ---------------------------------------------------------------------------
 __global__ void k_init(DevTopo g, DevInst d, const int8_t *type0,
                        const double *Qinj0) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int t = blockIdx.y;
+  const XSlot sl = xcd_slot(parts_of(g.n));
+  const int t = sl.t, i = sl.part * blockDim.x + threadIdx.x;
+  if (t >= d.T) return;
   if (i == 0 && t == 0) *d.active_count = 0;
   if (i == 0) {
     InstState s;
@@ -186,16 +210,19 @@ __device__ int block_add(int v, int *sh) {
 // go on), 4 fast decoupled after the Q half-step (converged, iteration cap or
 // go on) -- the loop of fdpf.h:505-590.  fd_norm divides by |V| (fdpf.h:401).
 // ---------------------------------------------------------------------------
-__device__ void mis_row(const DevTopo &g, const DevInst &d, int t, int k,
+// one bus: Ibus_i = sum_j Y_ij V_j, and for a non-slack bus its P (and, PQ,
+// Q) mismatch at its block position
+__device__ void mis_bus(const DevTopo &g, const DevInst &d, int t, int i,
                         int fd_norm, double &lmax, int &lnan) {
   const cd *Y = d.Y + (size_t)t * g.nnzY;
   const cd *V = d.V + (size_t)t * g.n;
-  int i = g.bus_of_blk[k];
   cd I = mk(0.0, 0.0);
   for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++)
     I = cadd(I, cmul(Y[p], V[g.ycol[p]]));
   size_t o = (size_t)t * g.n + i;
   d.Ib[o] = I;
+  const int k = g.blk_of_bus[i];
+  if (k < 0) return;
   cd mis = csub(cmul(V[i], cconj(I)), mk(d.Pinj[o], d.Qinj[o]));
   if (fd_norm) mis = cdiv(mis, mk(cabs_(V[i]), 0.0));
   bool pq = d.type[o] == 1;
@@ -210,16 +237,19 @@ __device__ void mis_row(const DevTopo &g, const DevInst &d, int t, int k,
 __global__ void k_mismatch_mb(DevTopo g, DevInst d, int mode, int fd_norm) {
   __shared__ double shd[16];
   __shared__ int shi[16];
-  const int t = blockIdx.y;
+  const int parts = parts_of(g.n);
+  const XSlot sl = xcd_slot(parts);
+  const int t = sl.t;
+  if (t >= d.T) return;
   if (mode != 2 && d.st[t].done) return;
   double lmax = 0.0;
   int lnan = 0;
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < g.nb) mis_row(g, d, t, k, fd_norm, lmax, lnan);
+  const int i = sl.part * blockDim.x + threadIdx.x;
+  if (i < g.n) mis_bus(g, d, t, i, fd_norm, lmax, lnan);
   double bmax = block_max(lmax, shd);
   int bnan = block_or(lnan, shi);
   if (threadIdx.x == 0)
-    d.part[(size_t)t * gridDim.x + blockIdx.x] = bnan ? (double)NAN : bmax;
+    d.part[(size_t)t * parts + sl.part] = bnan ? (double)NAN : bmax;
 }
 
 __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
@@ -254,8 +284,7 @@ __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
     if (nrm < tol) {
       if (!enforce_qlim) { s.verdict = BPF_V_CONVERGED; s.done = 1; break; }
       int lv = 0;
-      for (int b = threadIdx.x; b < g.nb; b += blockDim.x) {
-        int i = g.bus_of_blk[b];
+      for (int i = threadIdx.x; i < g.n; i += blockDim.x) {
         if (ty[i] != 2) continue;
         double qg = cmul(V[i], cconj(Ib[i])).y + g.Qd[i];
         size_t o = (size_t)t * g.n + i;
@@ -270,9 +299,9 @@ __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
       // buses' Q rows change (they were identity rows with F = 0)
       double lm = 0.0;
       int ln = 0;
-      for (int b = threadIdx.x; b < g.nb; b += blockDim.x) {
-        int i = g.bus_of_blk[b];
+      for (int i = threadIdx.x; i < g.n; i += blockDim.x) {
         if (ty[i] != 2) continue;
+        const int b = g.blk_of_bus[i];
         cd S = cmul(V[i], cconj(Ib[i]));
         double qg = S.y + g.Qd[i];
         size_t o = (size_t)t * g.n + i;
@@ -310,10 +339,10 @@ __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU) {
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *d.active_count = 0;
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  int t = blockIdx.y;
-  if (p >= g.nnzY) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.active_count = 0;
+  const XSlot sl = xcd_slot(parts_of(g.nnzY));
+  const int t = sl.t, p = sl.part * blockDim.x + threadIdx.x;
+  if (t >= d.T || p >= g.nnzY) return;
   if (d.st[t].done) return;
   int q = g.y2lu[p];
   if (q < 0) return;
@@ -384,6 +413,12 @@ struct SolvePlan {
   const int *fw_ptr, *fw_rows, *fw_max, *bw_ptr, *bw_rows, *bw_max;
   int tn, tsmax;                       // tail pivots, longest tail-row list
   const int *tpiv, *ts_ptr, *ts_pos, *ts_col, *ts_mir;
+  // entry-parallel chunks: forward ones first, then backward
+  int sw_nfw, sw_nbw;
+  const int4 *sw_chunk;                // {row0, row1, entry0, entry1}
+  const int *sw_cmax;                  // longest row of the chunk
+  const int4 *sw_row;                  // {block row, entry0, entry1, pivot position}
+  const int2 *sw_ent;                  // {block position, column block}
 };
 
 // Lanes that share one item (a task's pair list, a row's entries): a power of
@@ -702,6 +737,101 @@ __device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, const B4<T> *Ti,
   }
 }
 
+// Entry-parallel sweeps (X and a product buffer in LDS).  Per chunk of a
+// level: every entry's product (L or U block) * x_col is formed by its own
+// thread into LDS, then each row reduces its products with a lane group
+// and updates x.  The factor blocks and columns of the next chunk (static
+// for the whole solve) are loaded into registers while the current chunk
+// reduces; the sweeps write only LDS, so those loads stay in flight across
+// the barriers and a chunk's entry phase touches LDS only.
+template <typename T>
+struct EntPref {
+  B4<T> a0, a1;
+  int c0, c1;
+};
+
+template <typename T>
+__device__ __forceinline__ void ep_prefetch(const SolvePlan &P, const B4<T> *A,
+                                            int c, int4 &meta, EntPref<T> &pf) {
+  meta = P.sw_chunk[c];
+  const int e0 = meta.z + threadIdx.x, e1 = e0 + blockDim.x;
+  pf.c0 = pf.c1 = -1;
+  if (e0 < meta.w) {
+    const int2 en = P.sw_ent[e0];
+    pf.a0 = A[en.x];
+    pf.c0 = en.y;
+  }
+  if (e1 < meta.w) {
+    const int2 en = P.sw_ent[e1];
+    pf.a1 = A[en.x];
+    pf.c1 = en.y;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void ep_product(T *prod, int e, const B4<T> &a,
+                                           const T *X, int c) {
+  const T y0 = X[2 * c], y1 = X[2 * c + 1];
+  prod[2 * e] = a.a * y0 + a.b * y1;
+  prod[2 * e + 1] = a.c * y0 + a.d * y1;
+}
+
+template <typename T>
+__device__ void ep_chunk(const SolvePlan &P, const B4<T> *A, T *X, T *prod,
+                         int c, const int4 &meta, const EntPref<T> &pf,
+                         int cnext, int4 &mnext, EntPref<T> &pnext, bool bw) {
+  if (pf.c0 >= 0) ep_product(prod, threadIdx.x, pf.a0, X, pf.c0);
+  if (pf.c1 >= 0) ep_product(prod, threadIdx.x + blockDim.x, pf.a1, X, pf.c1);
+  for (int e = meta.z + threadIdx.x + 2 * blockDim.x; e < meta.w; e += blockDim.x) {
+    const int2 en = P.sw_ent[e];
+    ep_product(prod, e - meta.z, A[en.x], X, en.y);
+  }
+  __syncthreads();
+  if (cnext >= 0) ep_prefetch(P, A, cnext, mnext, pnext);
+  const int gsz = lane_group(meta.y - meta.x, P.sw_cmax[c]);
+  const int lane = threadIdx.x & (gsz - 1);
+  for (int r = meta.x + (int)threadIdx.x / gsz; r < meta.y; r += (int)blockDim.x / gsz) {
+    const int4 rw = P.sw_row[r];
+    const B4<T> Dg = (bw && lane == 0) ? A[rw.w] : blk_zero<T>();
+    T s0 = (T)0, s1 = (T)0;
+    for (int e = rw.y + lane; e < rw.z; e += gsz) {
+      s0 += prod[2 * (e - meta.z)];
+      s1 += prod[2 * (e - meta.z) + 1];
+    }
+    s0 = group_sum(s0, gsz);
+    s1 = group_sum(s1, gsz);
+    if (lane == 0) {
+      const int i = rw.x;
+      const T x0 = X[2 * i] - s0, x1 = X[2 * i + 1] - s1;
+      if (bw) {
+        X[2 * i] = Dg.a * x0 + Dg.b * x1;
+        X[2 * i + 1] = Dg.c * x0 + Dg.d * x1;
+      } else {
+        X[2 * i] = x0;
+        X[2 * i + 1] = x1;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti, T *X, T *prod,
+                             T *rs, const SolvePlan &P) {
+  const int nc = P.sw_nfw + P.sw_nbw;
+  int4 meta, mnext;
+  EntPref<T> pf, pn;
+  if (nc > 0) ep_prefetch(P, A, 0, meta, pf);
+  for (int c = 0; c < nc; c++) {
+    if (c == P.sw_nfw && P.tn > 0) tail_rows<T>(A, Ti, X, rs, P, false);
+    ep_chunk(P, A, X, prod, c, meta, pf, c + 1 < nc ? c + 1 : -1, mnext, pn,
+             c >= P.sw_nfw);
+    meta = mnext;
+    pf = pn;
+  }
+  if (P.sw_nbw == 0 && P.tn > 0) tail_rows<T>(A, Ti, X, rs, P, false);
+}
+
 // ---------------------------------------------------------------------------
 // Newton step: dx = J^-1 F with the factors in precision T, then
 // `refine` FP64 iterative-refinement steps whose residual r = F - J dx is
@@ -714,11 +844,14 @@ __device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, const B4<T> *Ti,
 // 160 KB, i.e. 19200 blocks in FP32 or 9600 in FP64; larger systems keep the
 // vector in global memory (use_lds = 0)
 constexpr int SOLVE_LDS_BYTES = 150 * 1024;
+// the entry-parallel sweeps keep their product buffer (2 * SWEEP_CHUNK
+// scalars) after X in the same LDS block
+__host__ __device__ constexpr int ep_offset(int nb) { return (2 * nb + 3) & ~3; }
 
 template <typename T>
 __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
                            const B4<T> *TL, int refine, int rhs_mode,
-                           SolvePlan P, int use_lds, T *xglob) {
+                           SolvePlan P, int use_lds, int ep, T *xglob) {
   __shared__ T smem[SOLVE_LDS_BYTES / sizeof(T)];
   __shared__ T rs[2 * TAIL_MAX];
   const int t = blockIdx.x;
@@ -738,8 +871,9 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
       for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
         X[k] = (rhs_mode == 0 || (rhs_mode == 1) == ((k & 1) == 0)) ? (T)F[k] : (T)0;
     } else {
-      for (int k = threadIdx.x; k < g.nb; k += blockDim.x) {
-        int i = g.bus_of_blk[k];
+      for (int i = threadIdx.x; i < g.n; i += blockDim.x) {
+        const int k = g.blk_of_bus[i];
+        if (k < 0) continue;
         cd acc = mk(0.0, 0.0);
         for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++) {
           int j = g.ycol[p];
@@ -758,7 +892,8 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
       }
     }
     __syncthreads();
-    lu_sweeps<T>(g, A, Ti, X, rs, P);
+    if (ep) lu_sweeps_ep<T>(A, Ti, X, smem + ep_offset(g.nb), rs, P);
+    else lu_sweeps<T>(g, A, Ti, X, rs, P);
     for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
       dx[k] = (s == 0 ? 0.0 : dx[k]) + (double)X[k];
     __syncthreads();
@@ -770,11 +905,12 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
 // the reference's angle is atan(y/x), complexUtils.h:21-23).
 // ---------------------------------------------------------------------------
 __global__ void k_update(DevTopo g, DevInst d) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int t = blockIdx.y;
-  if (k >= g.nb) return;
+  const XSlot sl = xcd_slot(parts_of(g.n));
+  const int t = sl.t, i = sl.part * blockDim.x + threadIdx.x;
+  if (t >= d.T || i >= g.n) return;
   if (d.st[t].done) return;
-  int i = g.bus_of_blk[k];
+  const int k = g.blk_of_bus[i];
+  if (k < 0) return;
   size_t o = (size_t)t * g.n + i;
   const double *dx = d.dx + (size_t)t * 2 * g.nb;
   cd v = d.V[o];
@@ -791,11 +927,12 @@ __global__ void k_zero_count(DevInst d) { *d.active_count = 0; }
 // unknown buses, part 2: magnitudes of PQ buses.
 // ---------------------------------------------------------------------------
 __global__ void k_fd_update(DevTopo g, DevInst d, int part) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  int t = blockIdx.y;
-  if (k >= g.nb) return;
+  const XSlot sl = xcd_slot(parts_of(g.n));
+  const int t = sl.t, i = sl.part * blockDim.x + threadIdx.x;
+  if (t >= d.T || i >= g.n) return;
   if (d.st[t].done) return;
-  int i = g.bus_of_blk[k];
+  const int k = g.blk_of_bus[i];
+  if (k < 0) return;
   size_t o = (size_t)t * g.n + i;
   const double *dx = d.dx + (size_t)t * 2 * g.nb;
   cd v = d.V[o];
@@ -1097,7 +1234,7 @@ template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *);
 template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *);
 template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *);
 template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *);
-template __global__ void k_lu_solve<float>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, float *);
-template __global__ void k_lu_solve<double>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, double *);
+template __global__ void k_lu_solve<float>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
+template __global__ void k_lu_solve<double>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
 
 }  // namespace bpf

@@ -302,6 +302,11 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     std::vector<size_t> fill;
     for (int gi = 0; gi < (int)groups.size(); gi++) {
       size_t np = gpairs(groups[gi]);
+      if (np > (size_t)CHUNK_PAIRS) {
+        if (err) *err = "a pivot's factorization products exceed one LDS chunk (" +
+                        std::to_string(np) + " > " + std::to_string(CHUNK_PAIRS) + ")";
+        return false;
+      }
       int c = -1;
       for (int q = 0; q < (int)chunks.size(); q++)
         if (fill[q] + np <= (size_t)CHUNK_PAIRS) { c = q; break; }
@@ -446,6 +451,43 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   }
   group(lf, S.nfw, S.fw_ptr, S.fw_rows, S.fw_max, lenL);
   group(lb, S.nbw, S.bw_ptr, S.bw_rows, S.bw_max, lenU);
+  // entry-parallel chunks (rows already longest first within a level)
+  auto chunks = [&](const std::vector<int> &ptr, const std::vector<int> &rows,
+                    int nl, bool fw) {
+    int nch = 0;
+    for (int l = 0; l < nl; l++) {
+      int r = ptr[l];
+      while (r < ptr[l + 1]) {
+        const int r0 = (int)S.sw_row.size() / 4, e0 = (int)S.sw_ent.size() / 2;
+        int cmax = 0;
+        while (r < ptr[l + 1]) {
+          const int i = rows[r];
+          const int p0 = fw ? S.lrow[i] : S.ldiag[i] + 1;
+          const int p1 = fw ? S.ldiag[i] : S.lrow[i + 1];
+          const int ne = (int)S.sw_ent.size() / 2;
+          if (ne > e0 && ne - e0 + (p1 - p0) > SWEEP_CHUNK) break;
+          S.sw_row.insert(S.sw_row.end(), {i, ne, ne + (p1 - p0), fw ? -1 : S.ldiag[i]});
+          for (int p = p0; p < p1; p++) {
+            S.sw_ent.push_back(p);
+            S.sw_ent.push_back(S.lcol[p]);
+          }
+          cmax = std::max(cmax, p1 - p0);
+          r++;
+        }
+        S.sw_chunk.insert(S.sw_chunk.end(), {r0, (int)S.sw_row.size() / 4, e0,
+                                             (int)S.sw_ent.size() / 2});
+        S.sw_cmax.push_back(cmax);
+        nch++;
+      }
+    }
+    return nch;
+  };
+  S.sw_nfw = chunks(S.fw_ptr, S.fw_rows, S.nfw, true);
+  S.sw_nbw = chunks(S.bw_ptr, S.bw_rows, S.nbw, false);
+  // a row longer than a chunk cannot take the entry-parallel path
+  S.sw_ok = 1;
+  for (size_t c = 0; c < S.sw_cmax.size(); c++)
+    if (S.sw_chunk[4 * c + 3] - S.sw_chunk[4 * c + 2] > SWEEP_CHUNK) S.sw_ok = 0;
   return true;
 }
 

@@ -17,6 +17,7 @@ constexpr int TASK_DIAG = 1 << 29;   // pivot block: store its inverse
 constexpr int TASK_LCOL = 1 << 28;   // L block: scale by the pivot inverse
 constexpr int TASK_POS_MASK = (1 << 28) - 1;
 constexpr int CHUNK_PAIRS = 4096;    // products per factorization chunk (LDS)
+constexpr int SWEEP_CHUNK = 2048;    // entries per sweep chunk (LDS)
 constexpr int TAIL_MAX = 64;         // dense tail: at most 64 blocks square
 
 struct Symbolic {
@@ -72,6 +73,14 @@ struct Symbolic {
   int nfw = 0, nbw = 0;
   std::vector<int> fw_ptr, fw_rows, bw_ptr, bw_rows;
   std::vector<int> fw_max, bw_max;           // longest row per level
+  // entry-parallel form of the same sweeps: the rows of each level packed
+  // into chunks of at most SWEEP_CHUNK entries.  Chunks 0 .. sw_nfw-1 are
+  // the forward sweep, the rest the backward sweep.
+  int sw_nfw = 0, sw_nbw = 0, sw_ok = 0;
+  std::vector<int> sw_chunk;   // {first row, end row, first entry, end entry}
+  std::vector<int> sw_cmax;    // longest row of the chunk
+  std::vector<int> sw_row;     // {block row, first entry, end entry, pivot position}
+  std::vector<int> sw_ent;     // {L or U position, column block}
 };
 
 // bus_type: MATPOWER codes; buses of type PV or PQ are unknowns, every other

@@ -29,7 +29,7 @@ EXPORTS = [
     "bpf_last_error", "bpf_get_info", "bpf_get_ybus", "bpf_stage_mismatch",
     "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_adjoint",
     "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
-    "bpf_analyse",
+    "bpf_analyse", "bpf_analyse_sweeps",
 ]
 
 
@@ -66,7 +66,9 @@ class BpfInfo(C.Structure):
                 ("crout_tasks", C.c_int32), ("chunks", C.c_int32),
                 ("graph_used", C.c_int32), ("tail_level", C.c_int32),
                 ("tail_n", C.c_int32), ("tail_entries", C.c_int32),
-                ("tail_pairs", C.c_int32), ("tail_row_entries", C.c_int32)]
+                ("tail_pairs", C.c_int32), ("tail_row_entries", C.c_int32),
+                ("sweep_fw_chunks", C.c_int32), ("sweep_bw_chunks", C.c_int32),
+                ("sweep_rows", C.c_int32), ("sweep_entries", C.c_int32)]
 
 
 _LIB = None
@@ -110,6 +112,7 @@ def load_library(path: str = LIB_PATH):
     L.bpf_get_kernel_time.argtypes = [V, C.c_char_p, _pi, _pd]
     L.bpf_set_launch.argtypes = [V, C.c_int32, C.c_int32, C.c_int32]
     L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 21
+    L.bpf_analyse_sweeps.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 3
     for name in EXPORTS:
         getattr(L, name)
     _LIB = L
@@ -183,6 +186,15 @@ def analyse(case):
     if st != 0:
         raise EngineError(f"bpf_analyse failed ({st})")
     out["info"] = i
+    nch = i["sweep_fw_chunks"] + i["sweep_bw_chunks"]
+    sw = dict(sw_chunk=np.zeros((nch, 4), np.int32),
+              sw_row=np.zeros((i["sweep_rows"], 4), np.int32),
+              sw_ent=np.zeros((i["sweep_entries"], 2), np.int32))
+    st = L.bpf_analyse_sweeps(C.byref(s), C.byref(info), _ptr(sw["sw_chunk"], _pi),
+                              _ptr(sw["sw_row"], _pi), _ptr(sw["sw_ent"], _pi))
+    if st != 0:
+        raise EngineError(f"bpf_analyse_sweeps failed ({st})")
+    out.update(sw)
     return out
 
 

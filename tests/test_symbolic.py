@@ -150,6 +150,38 @@ def _sweeps(a, A, Sinv, b, trans=False):
     return x.ravel()
 
 
+def _sweeps_ep(a, A, Sinv, b):
+    """the entry-parallel sweeps: per chunk, all entry products L*x_col (or
+    U*x_col) first, then every row of the chunk reduces its range; the tail
+    rows between the forward and the backward chunks"""
+    x = b.reshape(-1, 2).copy()
+    nfw = a["info"]["sweep_fw_chunks"]
+    ch, rows, ent = a["sw_chunk"], a["sw_row"], a["sw_ent"]
+
+    def tail():
+        s = a["info"]["tail_n"]
+        piv = a["tail_piv"]
+        rs = np.zeros((s, 2))
+        for k in range(s):
+            rs[k] = x[piv[k]]
+            for q in range(a["ts_ptr"][k], a["ts_ptr"][k + 1]):
+                rs[k] -= A[a["ts_pos"][q]] @ x[a["ts_col"][q]]
+        for k in range(s):
+            x[piv[k]] = sum(Sinv[k, j] @ rs[j] for j in range(s))
+
+    for c in range(len(ch)):
+        if c == nfw:
+            tail()
+        r0, r1, e0, e1 = ch[c]
+        prod = np.array([A[p] @ x[col] for p, col in ent[e0:e1]]).reshape(-1, 2)
+        for i, f0, f1, dpos in rows[r0:r1]:
+            xi = x[i] - prod[f0 - e0:f1 - e0].sum(0)
+            x[i] = A[dpos] @ xi if c >= nfw else xi
+    if nfw == len(ch):
+        tail()
+    return x.ravel()
+
+
 def _system(name, seed=0):
     case, cont = synth.workload(name)
     a = engine.analyse(case)
@@ -171,6 +203,9 @@ def test_schedule_replay_gives_the_newton_step(name):
     ref = sla.spsolve(J.tocsc(), F)
     got = np.array([x[2 * b + c] for b, c in rowmap])
     assert np.abs(got - ref).max() <= 1e-10 * max(1, np.abs(ref).max())
+    # the entry-parallel chunks give the same step
+    xe = _sweeps_ep(a, LU, Sinv, rhs)
+    assert np.abs(xe - x).max() <= 1e-12 * max(1, np.abs(x).max())
     # the PV magnitude unknowns of the superset system come out exactly zero
     pv_blocks = [r[0] for r in rowmap[:int((case.bus_type == 2).sum())]]
     assert np.all(x[2 * np.array(pv_blocks, int) + 1] == 0.0)
@@ -257,3 +292,26 @@ def test_schedule_invariants():
     n_init = int((a["pairs"][:, 1] < 0).sum())
     assert n_init + int((a["tk_a"][:, 0] & HAS_A > 0).sum()) \
         + int((a["tl_dense"] & HAS_A > 0).sum()) == i["nnz_jac_blocks"]
+
+
+def test_sweep_chunks_cover_the_levels():
+    """the entry-parallel chunks hold every non-tail row once per sweep, in
+    level order, with the row's own entries, at most 2048 per chunk"""
+    case, _ = synth.workload("pf9241x200")
+    a = engine.analyse(case)
+    i = a["info"]
+    rp, ci = a["lu_rowptr"], a["lu_colind"]
+    diag = _diag(a)
+    nfw = i["sweep_fw_chunks"]
+    ch, rows, ent = a["sw_chunk"], a["sw_row"], a["sw_ent"]
+    assert np.all(ch[:, 3] - ch[:, 2] <= 2048)
+    for lo, hi, order, fw in ((0, nfw, a["fw_rows"], True),
+                              (nfw, len(ch), a["bw_rows"], False)):
+        rws = rows[ch[lo, 0]:ch[hi - 1, 1]] if hi > lo else rows[:0]
+        assert np.array_equal(rws[:, 0], order)
+        for r, f0, f1, dpos in rws:
+            p0, p1 = (rp[r], diag[r]) if fw else (diag[r] + 1, rp[r + 1])
+            assert f1 - f0 == p1 - p0
+            assert np.array_equal(ent[f0:f1, 0], np.arange(p0, p1))
+            assert np.array_equal(ent[f0:f1, 1], ci[p0:p1])
+            assert dpos == (-1 if fw else diag[r])
