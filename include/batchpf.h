@@ -161,6 +161,25 @@ void bpf_destroy(bpf_engine *e);
 const char *bpf_last_error(const bpf_engine *e);
 bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info);
 
+/* MATPOWER version-2 case file into a bpf_case, host only.  Replaces
+ * loadMatpowerFormat (src/files/loadMatpowerFormat.h:14-209) with the same
+ * conventions: loads, shunts and generation divided by baseMVA, Va in
+ * radians, bus ids mapped to 0-based file order, every generator counted
+ * whatever its status column says (BPF_MP_GEN_STATUS honours it), the last
+ * generator of a bus sets its VG.  Loads and generation stay separate (the
+ * reference stores Pg - Pd); Inf reactive limits become +-infinity (the
+ * reference stores +-DBL_MAX, equal in every comparison).  An unknown bus id
+ * is BPF_ERR_INVALID_ARG (the reference indexes out of range).
+ * The arrays of *out (and *rate_a, branch rateA / baseMVA, if requested)
+ * live in *owner until bpf_free_case; on failure *owner (if not NULL) holds
+ * the message for bpf_case_error and must be freed as well. */
+#define BPF_MP_GEN_STATUS 1
+typedef struct bpf_case_data bpf_case_data;
+bpf_status bpf_load_matpower(const char *path, int32_t flags, bpf_case *out,
+                             const double **rate_a, bpf_case_data **owner);
+const char *bpf_case_error(const bpf_case_data *owner);
+void bpf_free_case(bpf_case_data *owner);
+
 /* Host-only symbolic analysis (no device needed): the block order, the L+U
  * block pattern and the schedules bpf_create builds (the J pattern of
  * newtonpf.h:215-310 extended by the fill of the factorization).  Sizes come

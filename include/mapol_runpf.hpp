@@ -14,6 +14,8 @@
 //   hybrid_free()                               mapol::hybrid_free()
 //   mkl_init / mkl_runpf / mkl_clean            mapol::mkl_* (same engine, one
 //                                               instance per call)
+//   loadMatpowerFormat(path)                    mapol::loadMatpowerFormat(path)
+//   (src/files/loadMatpowerFormat.h:14)
 //
 // Differences kept on purpose: errors throw mapol::Error instead of exit()
 // (helper_cuda.h:967-977); the state lives in one handle instead of module
@@ -82,6 +84,63 @@ struct Particula {
   double operator[](unsigned i) const { return X[i]; }
   void mudarFitness(double v) { fitness = v; }
 };
+
+// src/files/loadMatpowerFormat.h:14-209 through bpf_load_matpower: bus P/Q
+// are net injections (sum of generation minus load, pu), O in radians, VG
+// from the last generator of the bus, ids in file order.  Generator limits
+// are kept in pu (the reference's Generator keeps MW; its power flow never
+// reads them).  Throws mapol::Error where the reference calls exit().
+inline Topology loadMatpowerFormat(const std::string &path) {
+  bpf_case c;
+  const double *rate = nullptr;
+  bpf_case_data *owner = nullptr;
+  bpf_status s = bpf_load_matpower(path.c_str(), 0, &c, &rate, &owner);
+  if (s != BPF_OK) {
+    std::string msg = owner ? bpf_case_error(owner) : "bpf_load_matpower";
+    bpf_free_case(owner);
+    throw Error("loadMatpowerFormat: " + msg);
+  }
+  Topology t;
+  t.baseMVA = c.baseMVA;
+  t.buses.resize(c.nbus);
+  for (int i = 0; i < c.nbus; i++) {
+    Bus &b = t.buses[i];
+    b.id = (unsigned)i;
+    b.type = (Bus::Type)c.bus_type[i];
+    b.V = c.Vm[i];
+    b.O = c.Va[i];
+    b.P = -c.Pd[i];
+    b.Q = -c.Qd[i];
+    b.Gsh = c.Gs[i];
+    b.Bsh = c.Bs[i];
+  }
+  t.gens.resize(c.ngen);
+  for (int k = 0; k < c.ngen; k++) {
+    Generator &g = t.gens[k];
+    g.busID = (unsigned)c.gen_bus[k];
+    g.P = c.Pg[k];
+    g.Q = c.Qg[k];
+    g.Qmax = c.Qmax[k];
+    g.Qmin = c.Qmin[k];
+    t.buses[g.busID].P += c.Pg[k];
+    t.buses[g.busID].Q += c.Qg[k];
+    t.buses[g.busID].VG = c.VG[k];
+  }
+  t.branches.resize(c.nbranch);
+  for (int k = 0; k < c.nbranch; k++) {
+    Branch &b = t.branches[k];
+    b.from = (unsigned)c.br_from[k];
+    b.to = (unsigned)c.br_to[k];
+    b.R = c.R[k];
+    b.X = c.X[k];
+    b.B = c.B[k];
+    b.tap = c.tap[k];
+    b.shift = c.shift[k];
+    b.inservice = c.br_status[k] != 0;
+  }
+  bpf_free_case(owner);
+  return t;
+}
 
 namespace detail {
 struct State {

@@ -1,7 +1,7 @@
 """Build the engine library ``mapol_amd/libbatchpf.so`` for gfx950.
 
-One hipcc invocation compiles the kernels, the C-ABI and the host-side
-symbolic analysis into a single shared library.  Packed FP32 instructions are
+One hipcc invocation compiles the kernels, the C-ABI, the host-side symbolic
+analysis and the MATPOWER reader into a single shared library.  Packed FP32 instructions are
 disabled for all device code (``-packed-fp32-ops``).
 """
 from __future__ import annotations
@@ -26,7 +26,8 @@ def hipcc() -> str:
 
 
 def command(out: str = LIB):
-    srcs = [os.path.join(CSRC, "engine.hip"), os.path.join(CSRC, "symbolic.cpp")]
+    srcs = [os.path.join(CSRC, "engine.hip"), os.path.join(CSRC, "symbolic.cpp"),
+            os.path.join(CSRC, "matpower.cpp")]
     return [hipcc(), "-O3", "-std=c++17", "-fPIC", "-shared",
             f"--offload-arch={ARCH}", "-Wno-unused-result", "-Wno-unused-value",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -29,7 +29,8 @@ EXPORTS = [
     "bpf_last_error", "bpf_get_info", "bpf_get_ybus", "bpf_stage_mismatch",
     "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_adjoint",
     "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
-    "bpf_analyse", "bpf_analyse_sweeps",
+    "bpf_analyse", "bpf_analyse_sweeps", "bpf_load_matpower",
+    "bpf_case_error", "bpf_free_case",
 ]
 
 
@@ -113,6 +114,12 @@ def load_library(path: str = LIB_PATH):
     L.bpf_set_launch.argtypes = [V, C.c_int32, C.c_int32, C.c_int32]
     L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 21
     L.bpf_analyse_sweeps.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 3
+    L.bpf_load_matpower.argtypes = [C.c_char_p, C.c_int32, C.POINTER(BpfCase),
+                                    C.POINTER(_pd), C.POINTER(C.c_void_p)]
+    L.bpf_case_error.argtypes = [C.c_void_p]
+    L.bpf_case_error.restype = C.c_char_p
+    L.bpf_free_case.argtypes = [C.c_void_p]
+    L.bpf_free_case.restype = None
     for name in EXPORTS:
         getattr(L, name)
     _LIB = L
@@ -142,6 +149,55 @@ def _case_struct(c, keep):
     s.gen_bus = _ptr(arr(c.gen_bus, np.int32), _pi)
     s.gen_status = _ptr(arr(c.gen_status, np.uint8), _pu8)
     return s
+
+
+BPF_MP_GEN_STATUS = 1
+
+
+def load_matpower(path: str, honour_gen_status: bool = False):
+    """Read a MATPOWER version-2 case file through ``bpf_load_matpower``
+    (host only; the reference's loadMatpowerFormat conventions).  Returns
+    ``(case, rate_a)``: a :class:`mapol_amd.synth.Case` with copies of the
+    arrays and the branch ratings (rateA / baseMVA)."""
+    from .synth import Case
+
+    L = load_library()
+    s = BpfCase()
+    rate = _pd()
+    owner = C.c_void_p()
+    st = L.bpf_load_matpower(os.fsencode(path),
+                             BPF_MP_GEN_STATUS if honour_gen_status else 0,
+                             C.byref(s), C.byref(rate), C.byref(owner))
+    try:
+        if st != 0:
+            msg = L.bpf_case_error(owner).decode() if owner.value else ""
+            raise EngineError(f"bpf_load_matpower failed ({st}): {msg}")
+        n, m, g = s.nbus, s.nbranch, s.ngen
+
+        def take(ptr, k, dt):
+            return np.ctypeslib.as_array(ptr, shape=(k,)).astype(dt, copy=True) \
+                if k else np.zeros(0, dt)
+
+        case = Case(
+            bus_type=take(s.bus_type, n, np.int32),
+            Pd=take(s.Pd, n, np.float64), Qd=take(s.Qd, n, np.float64),
+            Gs=take(s.Gs, n, np.float64), Bs=take(s.Bs, n, np.float64),
+            Vm=take(s.Vm, n, np.float64), Va=take(s.Va, n, np.float64),
+            br_from=take(s.br_from, m, np.int32), br_to=take(s.br_to, m, np.int32),
+            R=take(s.R, m, np.float64), X=take(s.X, m, np.float64),
+            B=take(s.B, m, np.float64), tap=take(s.tap, m, np.float64),
+            shift=take(s.shift, m, np.float64),
+            br_status=take(s.br_status, m, np.uint8),
+            gen_bus=take(s.gen_bus, g, np.int32), Pg=take(s.Pg, g, np.float64),
+            Qg=take(s.Qg, g, np.float64), Qmax=take(s.Qmax, g, np.float64),
+            Qmin=take(s.Qmin, g, np.float64), VG=take(s.VG, g, np.float64),
+            gen_status=take(s.gen_status, g, np.uint8), baseMVA=s.baseMVA,
+            name=os.path.splitext(os.path.basename(path))[0],
+            rate=take(rate, m, np.float64))
+        return case, case.rate
+    finally:
+        if owner.value:
+            L.bpf_free_case(owner)
 
 
 def analyse(case):

@@ -6,7 +6,28 @@
 
 #include "mapol_runpf.hpp"
 
-int main() {
+// with a MATPOWER case file as argument: the case read through
+// mapol::loadMatpowerFormat, two particles without controls (base-case loss)
+static int run_case_file(const char *path) {
+  try {
+    mapol::Topology t = mapol::loadMatpowerFormat(path);
+    std::vector<mapol::Estrutura> est;
+    std::vector<mapol::Particula> swarm(2);
+    mapol::hybrid_init(t, 2, 1, est, 0);
+    mapol::hybrid_runpf(est, swarm);
+    mapol::hybrid_free();
+    std::printf("buses %zu branches %zu gens %zu\n", t.buses.size(),
+                t.branches.size(), t.gens.size());
+    for (int k = 0; k < 2; k++) std::printf("particle %d loss %.12e\n", k, swarm[k].fitness);
+  } catch (const mapol::Error &e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc > 1) return run_case_file(argv[1]);
   mapol::Topology t;
   t.baseMVA = 100;
   t.buses.resize(3);

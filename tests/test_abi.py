@@ -88,3 +88,26 @@ def test_cpp_mirror_runs_like_reference(tmp_path):
         r = orc.solve(vset=vset, tap=tap)
         assert r["converged"]
         assert got == pytest.approx(r["loss"], rel=1e-9)
+
+
+@pytest.mark.gpu
+def test_cpp_mirror_reads_matpower_case(tmp_path):
+    """mapol::loadMatpowerFormat (the mirror of loadMatpowerFormat) feeds
+    hybrid_runpf: the base-case loss of case118 equals the oracle's on the
+    same file read by the Python reader."""
+    import subprocess
+    import numpy as np
+    import oracle
+    from mapol_amd import matpower
+    from tests.test_matpower import write_case
+    z = np.load(os.path.join(ROOT, "tests", "golden", "matpower_cases.npz"))
+    path = str(tmp_path / "case118.m")
+    write_case(path, float(z["case118/baseMVA"]), z["case118/bus"],
+               z["case118/gen"], z["case118/branch"], style=1)
+    out = subprocess.run([_build_demo(tmp_path), path], check=True,
+                         capture_output=True, text=True).stdout.split("\n")
+    assert out[0] == "buses 118 branches 186 gens 54"
+    losses = [float(l.split()[-1]) for l in out if l.startswith("particle")]
+    r = oracle.Oracle(matpower.load(path)).solve()
+    assert r["converged"]
+    assert losses == pytest.approx([r["loss"]] * 2, rel=1e-9)
