@@ -423,6 +423,7 @@ extern "C" bpf_status bpf_set_instances(bpf_engine *e, int32_t T,
   AL(e->d_St, cd, Tm);
   AL(e->d_loss, double, T);
   AL(d.part, double, (size_t)T * div_up(e->n, 64));
+  AL(d.partv, int, (size_t)T * div_up(e->n, 64));
 #undef AL
   return BPF_OK;
 }
@@ -529,7 +530,7 @@ static void mismatch_pair(bpf_engine *e, int mode, int maxit, int qlim,
   const unsigned nparts = div_up(e->n, e->mis_threads);
   hipLaunchKernelGGL(k_mismatch_mb, xgrid(e->T, e->n, e->mis_threads),
                      dim3(e->mis_threads), 0, e->stream, e->g, e->d, mode,
-                     fd_norm);
+                     fd_norm, e->opt.qlim_tol, qlim);
   hipLaunchKernelGGL(k_verdict, dim3(e->T), dim3(256), 0, e->stream, e->g, e->d,
                      (int)nparts, mode, maxit, e->opt.tol, qlim,
                      e->opt.max_qlim_rounds, e->opt.qlim_tol, fd_norm);
@@ -615,24 +616,35 @@ static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
   }
   e->graph_used = use_graph ? 1 : 0;
   const int cap = e->opt.maxit_nr * (e->opt.max_qlim_rounds + 1) + 2;
+  // an error left by an earlier call must not be read as this loop's
+  hipError_t stale = hipGetLastError();
+  if (stale != hipSuccess && e->graph_err.empty())
+    e->graph_err = std::string("error pending before the Newton loop: ") +
+                   hipGetErrorName(stale);
   for (int it = 0; it < cap; it++) {
     hipError_t r = hipSuccess;
+    const char *where = "";
     if (use_graph) {
+      where = "hipGraphLaunch";
       r = hipGraphLaunch(e->graph, e->stream);
-      if (r == hipSuccess)
+      if (r == hipSuccess) {
+        where = "hipMemcpyAsync";
         r = hipMemcpyAsync(e->h_count, e->d.active_count, sizeof(int),
                            hipMemcpyDeviceToHost, e->stream);
+      }
     } else {
       launch_iteration(e);
     }
-    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
-    if (r == hipSuccess) r = hipGetLastError();
+    if (r == hipSuccess) { where = "hipStreamSynchronize"; r = hipStreamSynchronize(e->stream); }
+    if (r == hipSuccess) { where = "hipGetLastError"; r = hipGetLastError(); }
     if (r != hipSuccess) {
       if (use_graph) {
         // the runtime refused the replay: remember why and let bpf_solve
         // start the solve again with direct launches
         e->graph_broken = true;
-        e->graph_err = std::string("graph replay: ") + hipGetErrorString(r);
+        e->graph_err = std::string("graph replay (") + where + ", iteration " +
+                       std::to_string(it) + "): " + hipGetErrorName(r) + " " +
+                       hipGetErrorString(r);
         (void)hipGetLastError();
         *graph_failed = true;
         return BPF_OK;

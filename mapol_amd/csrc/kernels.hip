@@ -213,7 +213,8 @@ __device__ int block_add(int v, int *sh) {
 // one bus: Ibus_i = sum_j Y_ij V_j, and for a non-slack bus its P (and, PQ,
 // Q) mismatch at its block position
 __device__ void mis_bus(const DevTopo &g, const DevInst &d, int t, int i,
-                        int fd_norm, double &lmax, int &lnan) {
+                        int fd_norm, double qtol, int qlim, double &lmax,
+                        int &lnan, int &lviol) {
   const cd *Y = d.Y + (size_t)t * g.nnzY;
   const cd *V = d.V + (size_t)t * g.n;
   cd I = mk(0.0, 0.0);
@@ -223,7 +224,13 @@ __device__ void mis_bus(const DevTopo &g, const DevInst &d, int t, int i,
   d.Ib[o] = I;
   const int k = g.blk_of_bus[i];
   if (k < 0) return;
-  cd mis = csub(cmul(V[i], cconj(I)), mk(d.Pinj[o], d.Qinj[o]));
+  const cd Sb = cmul(V[i], cconj(I));
+  if (qlim && d.type[o] == 2) {
+    // reactive limit of a PV bus (used by the verdict once converged)
+    const double qg = Sb.y + g.Qd[i];
+    lviol += (qg > d.Qmax[o] + qtol || qg < d.Qmin[o] - qtol) ? 1 : 0;
+  }
+  cd mis = csub(Sb, mk(d.Pinj[o], d.Qinj[o]));
   if (fd_norm) mis = cdiv(mis, mk(cabs_(V[i]), 0.0));
   bool pq = d.type[o] == 1;
   double *F = d.F + (size_t)t * 2 * g.nb;
@@ -234,7 +241,8 @@ __device__ void mis_bus(const DevTopo &g, const DevInst &d, int t, int i,
   if (pq) { lmax = fmax(lmax, fabs(mis.y)); lnan |= isnan(mis.y); }
 }
 
-__global__ void k_mismatch_mb(DevTopo g, DevInst d, int mode, int fd_norm) {
+__global__ void k_mismatch_mb(DevTopo g, DevInst d, int mode, int fd_norm,
+                              double qtol, int qlim) {
   __shared__ double shd[16];
   __shared__ int shi[16];
   const int parts = parts_of(g.n);
@@ -243,13 +251,16 @@ __global__ void k_mismatch_mb(DevTopo g, DevInst d, int mode, int fd_norm) {
   if (t >= d.T) return;
   if (mode != 2 && d.st[t].done) return;
   double lmax = 0.0;
-  int lnan = 0;
+  int lnan = 0, lviol = 0;
   const int i = sl.part * blockDim.x + threadIdx.x;
-  if (i < g.n) mis_bus(g, d, t, i, fd_norm, lmax, lnan);
+  if (i < g.n) mis_bus(g, d, t, i, fd_norm, qtol, qlim, lmax, lnan, lviol);
   double bmax = block_max(lmax, shd);
   int bnan = block_or(lnan, shi);
-  if (threadIdx.x == 0)
+  int bviol = qlim ? block_add(lviol, shi) : 0;
+  if (threadIdx.x == 0) {
     d.part[(size_t)t * parts + sl.part] = bnan ? (double)NAN : bmax;
+    d.partv[(size_t)t * parts + sl.part] = bviol;
+  }
 }
 
 __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
@@ -261,14 +272,18 @@ __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
   InstState *sp = d.st + t;
   if (mode != 2 && sp->done) return;
   double pm = 0.0;
-  int pn = 0;
+  int pn = 0, pv = 0;
   for (int q = threadIdx.x; q < nparts; q += blockDim.x) {
     double v = d.part[(size_t)t * nparts + q];
     pn |= isnan(v);
     pm = fmax(pm, v);
+    pv += d.partv[(size_t)t * nparts + q];
   }
   double nrm = block_max(pm, shd);
   int anynan = block_or(pn, shi);
+  // PV buses beyond their reactive limits at this V (counted per bus by the
+  // mismatch kernel); after a switch the remaining ones are recounted here
+  int nviol_at_v = enforce_qlim ? block_add(pv, shi) : 0;
   InstState s = *sp;
   if (mode == 1 || mode == 3) { s.it_round++; s.it_total++; }
   int8_t *ty = d.type + (size_t)t * g.n;
@@ -283,14 +298,17 @@ __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
     if (anynan || isinf(nrm)) { s.verdict = BPF_V_NONFINITE; s.done = 1; break; }
     if (nrm < tol) {
       if (!enforce_qlim) { s.verdict = BPF_V_CONVERGED; s.done = 1; break; }
-      int lv = 0;
-      for (int i = threadIdx.x; i < g.n; i += blockDim.x) {
-        if (ty[i] != 2) continue;
-        double qg = cmul(V[i], cconj(Ib[i])).y + g.Qd[i];
-        size_t o = (size_t)t * g.n + i;
-        if (qg > d.Qmax[o] + qtol || qg < d.Qmin[o] - qtol) lv++;
+      int nv = nviol_at_v;
+      if (nv < 0) {
+        int lv = 0;
+        for (int i = threadIdx.x; i < g.n; i += blockDim.x) {
+          if (ty[i] != 2) continue;
+          double qg = cmul(V[i], cconj(Ib[i])).y + g.Qd[i];
+          size_t o = (size_t)t * g.n + i;
+          if (qg > d.Qmax[o] + qtol || qg < d.Qmin[o] - qtol) lv++;
+        }
+        nv = block_add(lv, shi);
       }
-      int nv = block_add(lv, shi);
       if (nv == 0) { s.verdict = BPF_V_CONVERGED; s.done = 1; break; }
       if (s.rounds >= max_rounds) { s.verdict = BPF_V_MAX_ITER; s.done = 1; break; }
       s.rounds++;
@@ -319,6 +337,7 @@ __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
       }
       nrm = fmax(nrm, block_max(lm, shd));
       anynan |= block_or(ln, shi);
+      nviol_at_v = -1;  // bus types changed: count again
       continue;  // decide again with the new bus types
     }
     if (mode != 3 && s.it_round >= maxit) { s.verdict = BPF_V_MAX_ITER; s.done = 1; break; }
@@ -499,9 +518,38 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   if (threadIdx.x == 0) sing = 0;
   B4<T> *A = LU + (size_t)t * g.nnzLU;
   __syncthreads();
+  // product indices of the next chunk are loaded while the current chunk
+  // reduces; a thread's first output descriptors while the products form
+  constexpr int PF = CHUNK_PAIRS / 1024;
+  int2 pidx[PF];
+  int nq0 = 0, nq1 = 0;
+  auto prefetch_pairs = [&](int c) {
+    nq0 = P.ch_pair[c];
+    nq1 = P.ch_pair[c + 1];
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+      const int q = nq0 + (int)threadIdx.x + j * (int)blockDim.x;
+      pidx[j] = q < nq1 ? P.pairs[q] : make_int2(0, 0);
+    }
+  };
+  if (P.nchunk > 0) prefetch_pairs(0);
   for (int c = 0; c < P.nchunk; c++) {
-    const int q0 = P.ch_pair[c], q1 = P.ch_pair[c + 1];
-    for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    const int q0 = nq0, q1 = nq1;
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+      const int q = q0 + (int)threadIdx.x + j * (int)blockDim.x;
+      if (q < q1) {
+        B4<T> v;
+        if (pidx[j].y < 0) {
+          v = A[pidx[j].x];
+        } else {
+          v = blk_zero<T>();
+          blk_fms(v, A[pidx[j].x], A[pidx[j].y]);
+        }
+        buf[q - q0] = v;
+      }
+    }
+    for (int q = q0 + threadIdx.x + PF * blockDim.x; q < q1; q += blockDim.x) {
       const int2 pr = P.pairs[q];
       B4<T> v;
       if (pr.y < 0) {
@@ -512,18 +560,25 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       }
       buf[q - q0] = v;
     }
-    __syncthreads();
     const int k0 = P.ch_task[c], k1 = P.ch_task[c + 1];
     const int gsz = lane_group(k1 - k0, P.ch_max[c]);
     const int lane = threadIdx.x & (gsz - 1);
-    for (int k = k0 + (int)threadIdx.x / gsz; k < k1; k += (int)blockDim.x / gsz) {
-      const int4 ta = P.tk_a[k];
+    const int kf = k0 + (int)threadIdx.x / gsz;
+    int4 ta_f = make_int4(0, 0, 0, 0), tb_f = make_int4(0, 0, 0, 0);
+    if (kf < k1) {
+      ta_f = P.tk_a[kf];
+      tb_f = P.tk_b[kf];
+    }
+    __syncthreads();
+    if (c + 1 < P.nchunk) prefetch_pairs(c + 1);
+    for (int k = kf; k < k1; k += (int)blockDim.x / gsz) {
+      const int4 ta = (k == kf) ? ta_f : P.tk_a[k];
       const int p = ta.x & TASK_POS_MASK;
       B4<T> e = (lane == 0 && (ta.x & TASK_HAS_A)) ? A[p] : blk_zero<T>();
       for (int q = ta.y + lane; q < ta.z; q += gsz) blk_acc(e, buf[q - q0]);
       group_sum4(e, gsz);
       if (ta.x & TASK_LCOL) {
-        const int4 tb = P.tk_b[k];
+        const int4 tb = (k == kf) ? tb_f : P.tk_b[k];
         B4<T> pv = blk_zero<T>();
         for (int q = tb.y + lane; q < tb.z; q += gsz) blk_acc(pv, buf[q - q0]);
         group_sum4(pv, gsz);

@@ -58,7 +58,8 @@ struct DevInst {
   InstState *st;           // [T]
   int *active_count;       // [1]
   // partial mismatch norms, one per mismatch workgroup (NaN: a NaN entry)
-  double *part;            // [T][nb / 64 rounded up]
+  double *part;            // [T][n / 64 rounded up]
+  int *partv;              // [T][n / 64 rounded up] PV buses beyond their Q limits
 };
 
 }  // namespace bpf
