@@ -157,7 +157,8 @@ bpf_status bpf_get_branch_flows(bpf_engine *e, double *Sf, double *St);
 void bpf_destroy(bpf_engine *e);
 
 /* Last error message; when there is none but hipGraph replay had to be
- * abandoned (the solve then ran with direct launches), its reason. */
+ * abandoned (the solve then ran with direct launches), its reason, else a
+ * note on what the runtime reported around replays (empty when nothing). */
 const char *bpf_last_error(const bpf_engine *e);
 bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info);
 

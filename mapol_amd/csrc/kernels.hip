@@ -629,34 +629,81 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       pvb[0] = iv;
     }
     __syncthreads();
-    for (int k = 0; k < s; k++) {
-      const int cur = k & 1, nxt = cur ^ 1;
-      const B4<T> Pk = pvb[cur];
-      for (int q = threadIdx.x; q < ss; q += blockDim.x) {
-        const int i = q / s, j = q - i * s;
-        B4<T> v;
-        if (i == k) {
-          v = (j == k) ? Pk : blk_mul(Pk, rowb[cur][j]);
-        } else if (j == k) {
-          v = blk_zero<T>();
-          blk_fms(v, colb[cur][i], Pk);
-        } else {
-          v = D[q];
-          blk_fms(v, colb[cur][i], blk_mul(Pk, rowb[cur][j]));
-        }
-        D[q] = v;
-        if (j == k + 1) colb[nxt][i] = v;
-        if (i == k + 1) rowb[nxt][j] = v;
-        if (i == k + 1 && j == k + 1) {
-          B4<T> iv;
-          if (blk_inv(v, iv)) sing = 1;
-          pvb[nxt] = iv;
-        }
-      }
-      __syncthreads();
-    }
+    constexpr int GJR = 4;  // entries a thread keeps in registers
     B4<T> *Ti = TL + (size_t)t * ss;
-    for (int q = threadIdx.x; q < ss; q += blockDim.x) Ti[q] = D[q];
+    if (ss <= GJR * (int)blockDim.x) {
+      // each thread owns up to GJR entries of S for the whole elimination:
+      // only the pivot row, column and inverse pass through LDS
+      B4<T> dv[GJR];
+      int di[GJR], dj[GJR];
+#pragma unroll
+      for (int m = 0; m < GJR; m++) {
+        const int q = threadIdx.x + m * blockDim.x;
+        di[m] = q < ss ? q / s : -1;
+        dj[m] = q < ss ? q - di[m] * s : -1;
+        dv[m] = q < ss ? D[q] : blk_zero<T>();
+      }
+      for (int k = 0; k < s; k++) {
+        const int cur = k & 1, nxt = cur ^ 1;
+        const B4<T> Pk = pvb[cur];
+#pragma unroll
+        for (int m = 0; m < GJR; m++) {
+          const int i = di[m], j = dj[m];
+          if (i < 0) continue;
+          B4<T> v;
+          if (i == k) {
+            v = (j == k) ? Pk : blk_mul(Pk, rowb[cur][j]);
+          } else if (j == k) {
+            v = blk_zero<T>();
+            blk_fms(v, colb[cur][i], Pk);
+          } else {
+            v = dv[m];
+            blk_fms(v, colb[cur][i], blk_mul(Pk, rowb[cur][j]));
+          }
+          dv[m] = v;
+          if (j == k + 1) colb[nxt][i] = v;
+          if (i == k + 1) rowb[nxt][j] = v;
+          if (i == k + 1 && j == k + 1) {
+            B4<T> iv;
+            if (blk_inv(v, iv)) sing = 1;
+            pvb[nxt] = iv;
+          }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int m = 0; m < GJR; m++)
+        if (di[m] >= 0) Ti[threadIdx.x + m * blockDim.x] = dv[m];
+    } else {
+      // more entries than registers allow: S stays in LDS
+      for (int k = 0; k < s; k++) {
+        const int cur = k & 1, nxt = cur ^ 1;
+        const B4<T> Pk = pvb[cur];
+        for (int q = threadIdx.x; q < ss; q += blockDim.x) {
+          const int i = q / s, j = q - i * s;
+          B4<T> v;
+          if (i == k) {
+            v = (j == k) ? Pk : blk_mul(Pk, rowb[cur][j]);
+          } else if (j == k) {
+            v = blk_zero<T>();
+            blk_fms(v, colb[cur][i], Pk);
+          } else {
+            v = D[q];
+            blk_fms(v, colb[cur][i], blk_mul(Pk, rowb[cur][j]));
+          }
+          D[q] = v;
+          if (j == k + 1) colb[nxt][i] = v;
+          if (i == k + 1) rowb[nxt][j] = v;
+          if (i == k + 1 && j == k + 1) {
+            B4<T> iv;
+            if (blk_inv(v, iv)) sing = 1;
+            pvb[nxt] = iv;
+          }
+        }
+        __syncthreads();
+      }
+      for (int q = threadIdx.x; q < ss; q += blockDim.x) Ti[q] = D[q];
+    }
   }
   if (threadIdx.x == 0 && sing) {
     InstState s = st[t];
@@ -795,32 +842,45 @@ __device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, const B4<T> *Ti,
 // Entry-parallel sweeps (X and a product buffer in LDS).  Per chunk of a
 // level: every entry's product (L or U block) * x_col is formed by its own
 // thread into LDS, then each row reduces its products with a lane group
-// and updates x.  The factor blocks and columns of the next chunk (static
-// for the whole solve) are loaded into registers while the current chunk
-// reduces; the sweeps write only LDS, so those loads stay in flight across
-// the barriers and a chunk's entry phase touches LDS only.
-template <typename T>
-struct EntPref {
-  B4<T> a0, a1;
-  int c0, c1;
+// and updates x.  Everything a chunk reads from HBM is static for the whole
+// solve, so it is loaded ahead in two stages: the chunk's descriptor, the
+// thread's first row descriptor and its entry indices two chunks ahead, the
+// factor blocks (and the backward pivot inverse) one chunk ahead from those
+// indices.  The sweeps write only LDS, so the loads stay in flight across
+// the barriers and neither phase of a chunk waits on HBM for its first row
+// and first two entries.
+struct EntIdx {
+  int4 meta;        // {row0, row1, entry0, entry1}
+  int gsz, rfirst;  // lane group of the row phase, the thread's first row
+  int4 rw;          // its descriptor
+  int2 e0, e1;      // the thread's two entries {position, column} (x < 0: none)
 };
 
 template <typename T>
-__device__ __forceinline__ void ep_prefetch(const SolvePlan &P, const B4<T> *A,
-                                            int c, int4 &meta, EntPref<T> &pf) {
-  meta = P.sw_chunk[c];
-  const int e0 = meta.z + threadIdx.x, e1 = e0 + blockDim.x;
-  pf.c0 = pf.c1 = -1;
-  if (e0 < meta.w) {
-    const int2 en = P.sw_ent[e0];
-    pf.a0 = A[en.x];
-    pf.c0 = en.y;
-  }
-  if (e1 < meta.w) {
-    const int2 en = P.sw_ent[e1];
-    pf.a1 = A[en.x];
-    pf.c1 = en.y;
-  }
+struct EntVal {
+  EntIdx ix;
+  B4<T> a0, a1;     // factor blocks of the two entries
+  B4<T> D;          // pivot inverse of the first row (backward sweep)
+};
+
+__device__ __forceinline__ void ep_index(const SolvePlan &P, int c, EntIdx &ix) {
+  ix.meta = P.sw_chunk[c];
+  ix.gsz = lane_group(ix.meta.y - ix.meta.x, P.sw_cmax[c]);
+  ix.rfirst = ix.meta.x + (int)threadIdx.x / ix.gsz;
+  ix.rw = ix.rfirst < ix.meta.y ? P.sw_row[ix.rfirst] : make_int4(0, 0, 0, 0);
+  const int e0 = ix.meta.z + threadIdx.x, e1 = e0 + blockDim.x;
+  ix.e0 = e0 < ix.meta.w ? P.sw_ent[e0] : make_int2(-1, -1);
+  ix.e1 = e1 < ix.meta.w ? P.sw_ent[e1] : make_int2(-1, -1);
+}
+
+template <typename T>
+__device__ __forceinline__ void ep_values(const B4<T> *A, const EntIdx &ix,
+                                          bool bw, EntVal<T> &v) {
+  v.ix = ix;
+  v.a0 = ix.e0.x >= 0 ? A[ix.e0.x] : blk_zero<T>();
+  v.a1 = ix.e1.x >= 0 ? A[ix.e1.x] : blk_zero<T>();
+  const bool lead = (threadIdx.x & (ix.gsz - 1)) == 0;
+  v.D = (bw && lead && ix.rfirst < ix.meta.y) ? A[ix.rw.w] : blk_zero<T>();
 }
 
 template <typename T>
@@ -831,23 +891,31 @@ __device__ __forceinline__ void ep_product(T *prod, int e, const B4<T> &a,
   prod[2 * e + 1] = a.c * y0 + a.d * y1;
 }
 
+// one chunk; before its row phase, the next chunk's values are taken from
+// its loaded indices and the indices of the one after are requested
 template <typename T>
 __device__ void ep_chunk(const SolvePlan &P, const B4<T> *A, T *X, T *prod,
-                         int c, const int4 &meta, const EntPref<T> &pf,
-                         int cnext, int4 &mnext, EntPref<T> &pnext, bool bw) {
-  if (pf.c0 >= 0) ep_product(prod, threadIdx.x, pf.a0, X, pf.c0);
-  if (pf.c1 >= 0) ep_product(prod, threadIdx.x + blockDim.x, pf.a1, X, pf.c1);
+                         const EntVal<T> &cur, bool bw, bool has_next,
+                         bool next_bw, int c_after, EntIdx &ix_next,
+                         EntVal<T> &nxt) {
+  const int4 meta = cur.ix.meta;
+  if (cur.ix.e0.x >= 0) ep_product(prod, threadIdx.x, cur.a0, X, cur.ix.e0.y);
+  if (cur.ix.e1.x >= 0) ep_product(prod, threadIdx.x + blockDim.x, cur.a1, X, cur.ix.e1.y);
   for (int e = meta.z + threadIdx.x + 2 * blockDim.x; e < meta.w; e += blockDim.x) {
     const int2 en = P.sw_ent[e];
     ep_product(prod, e - meta.z, A[en.x], X, en.y);
   }
   __syncthreads();
-  if (cnext >= 0) ep_prefetch(P, A, cnext, mnext, pnext);
-  const int gsz = lane_group(meta.y - meta.x, P.sw_cmax[c]);
+  if (has_next) {
+    ep_values(A, ix_next, next_bw, nxt);
+    if (c_after >= 0) ep_index(P, c_after, ix_next);
+  }
+  const int gsz = cur.ix.gsz;
   const int lane = threadIdx.x & (gsz - 1);
-  for (int r = meta.x + (int)threadIdx.x / gsz; r < meta.y; r += (int)blockDim.x / gsz) {
-    const int4 rw = P.sw_row[r];
-    const B4<T> Dg = (bw && lane == 0) ? A[rw.w] : blk_zero<T>();
+  for (int r = cur.ix.rfirst; r < meta.y; r += (int)blockDim.x / gsz) {
+    const bool first = r == cur.ix.rfirst;
+    const int4 rw = first ? cur.ix.rw : P.sw_row[r];
+    const B4<T> Dg = first ? cur.D : ((bw && lane == 0) ? A[rw.w] : blk_zero<T>());
     T s0 = (T)0, s1 = (T)0;
     for (int e = rw.y + lane; e < rw.z; e += gsz) {
       s0 += prod[2 * (e - meta.z)];
@@ -874,26 +942,26 @@ template <typename T>
 __device__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti, T *X, T *prod,
                              T *rs, const SolvePlan &P) {
   const int nc = P.sw_nfw + P.sw_nbw;
-  int4 meta, mnext;
-  EntPref<T> pf, pn;
-  if (nc > 0) ep_prefetch(P, A, 0, meta, pf);
+  EntIdx ixn;
+  EntVal<T> cur, nxt;
+  if (nc > 0) {
+    ep_index(P, 0, ixn);
+    ep_values(A, ixn, 0 >= P.sw_nfw, cur);
+  }
+  if (nc > 1) ep_index(P, 1, ixn);
   for (int c = 0; c < nc; c++) {
     if (c == P.sw_nfw && P.tn > 0) tail_rows<T>(A, Ti, X, rs, P, false);
-    ep_chunk(P, A, X, prod, c, meta, pf, c + 1 < nc ? c + 1 : -1, mnext, pn,
-             c >= P.sw_nfw);
-    meta = mnext;
-    pf = pn;
+    ep_chunk(P, A, X, prod, cur, c >= P.sw_nfw, c + 1 < nc, c + 1 >= P.sw_nfw,
+             c + 2 < nc ? c + 2 : -1, ixn, nxt);
+    cur = nxt;
   }
   if (P.sw_nbw == 0 && P.tn > 0) tail_rows<T>(A, Ti, X, rs, P, false);
 }
 
 // ---------------------------------------------------------------------------
-// Newton step: dx = J^-1 F with the factors in precision T, then
-// `refine` FP64 iterative-refinement steps whose residual r = F - J dx is
-// evaluated matrix-free in FP64 from Ybus, V and Ibus:
-//   dS_i = dV_i conj(I_i) + V_i conj(sum_j Y_ij dV_j),
-//   dV_j = V_j (j dtheta_j + dVm_j / |V_j|).
-// One workgroup per instance.
+// Newton step solve: dx = J^-1 rhs with the factors in precision T (one pass;
+// mixed precision runs it again on the FP64 residual of k_residual and
+// accumulates).  One workgroup per instance.
 // ---------------------------------------------------------------------------
 // LDS reserved for the solve vector of one instance: 150 KB of gfx950's
 // 160 KB, i.e. 19200 blocks in FP32 or 9600 in FP64; larger systems keep the
@@ -905,7 +973,7 @@ __host__ __device__ constexpr int ep_offset(int nb) { return (2 * nb + 3) & ~3; 
 
 template <typename T>
 __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
-                           const B4<T> *TL, int refine, int rhs_mode,
+                           const B4<T> *TL, int rhs_mode, int accumulate,
                            SolvePlan P, int use_lds, int ep, T *xglob) {
   __shared__ T smem[SOLVE_LDS_BYTES / sizeof(T)];
   __shared__ T rs[2 * TAIL_MAX];
@@ -915,44 +983,54 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
   const B4<T> *A = LU + (size_t)t * g.nnzLU;
   const B4<T> *Ti = TL + (size_t)t * P.tn * P.tn;
   const double *F = d.F + (size_t)t * 2 * g.nb;
+  const double *R = d.rhs + (size_t)t * 2 * g.nb;
   double *dx = d.dx + (size_t)t * 2 * g.nb;
+  // rhs_mode 0: (P, Q) mismatch; 1: (P, 0) and 2: (0, Q) for the decoupled
+  // half steps; 3: the refinement residual of k_residual
+  for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
+    X[k] = rhs_mode == 3 ? (T)R[k]
+           : (rhs_mode == 0 || (rhs_mode == 1) == ((k & 1) == 0)) ? (T)F[k] : (T)0;
+  __syncthreads();
+  if (ep) lu_sweeps_ep<T>(A, Ti, X, smem + ep_offset(g.nb), rs, P);
+  else lu_sweeps<T>(g, A, Ti, X, rs, P);
+  for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
+    dx[k] = (accumulate ? dx[k] : 0.0) + (double)X[k];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 refinement residual r = F - J dx, matrix-free from Ybus, V and Ibus:
+//   dS_i = dV_i conj(I_i) + V_i conj(sum_j Y_ij dV_j),
+//   dV_j = V_j (j dtheta_j + dVm_j / |V_j|);
+// P rows: F_P - Re dS; Q rows: F_Q - Im dS for PQ buses, -dVm for PV buses
+// (their identity rows).  Many workgroups per instance, bus order.
+// ---------------------------------------------------------------------------
+__global__ void k_residual(DevTopo g, DevInst d) {
+  const XSlot sl = xcd_slot(parts_of(g.n));
+  const int t = sl.t, i = sl.part * blockDim.x + threadIdx.x;
+  if (t >= d.T || i >= g.n) return;
+  if (d.st[t].done) return;
+  const int k = g.blk_of_bus[i];
+  if (k < 0) return;
+  const double *F = d.F + (size_t)t * 2 * g.nb;
+  const double *dx = d.dx + (size_t)t * 2 * g.nb;
   const cd *Y = d.Y + (size_t)t * g.nnzY;
   const cd *V = d.V + (size_t)t * g.n;
-  const cd *Ib = d.Ib + (size_t)t * g.n;
-  const int8_t *ty = d.type + (size_t)t * g.n;
-  for (int s = 0; s <= refine; s++) {
-    if (s == 0) {
-      // rhs_mode 0: (P, Q); 1: (P, 0) and 2: (0, Q) for the decoupled steps
-      for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
-        X[k] = (rhs_mode == 0 || (rhs_mode == 1) == ((k & 1) == 0)) ? (T)F[k] : (T)0;
-    } else {
-      for (int i = threadIdx.x; i < g.n; i += blockDim.x) {
-        const int k = g.blk_of_bus[i];
-        if (k < 0) continue;
-        cd acc = mk(0.0, 0.0);
-        for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++) {
-          int j = g.ycol[p];
-          int bj = g.blk_of_bus[j];
-          if (bj < 0) continue;
-          cd vj = V[j];
-          double m = dx[2 * bj + 1] / cabs_(vj);
-          cd dvj = cmul(vj, mk(m, dx[2 * bj]));
-          acc = cadd(acc, cmul(Y[p], dvj));
-        }
-        cd vi = V[i];
-        cd dvi = cmul(vi, mk(dx[2 * k + 1] / cabs_(vi), dx[2 * k]));
-        cd dS = cadd(cmul(dvi, cconj(Ib[i])), cmul(vi, cconj(acc)));
-        X[2 * k] = (T)(F[2 * k] - dS.x);
-        X[2 * k + 1] = (T)(ty[i] == 1 ? F[2 * k + 1] - dS.y : -dx[2 * k + 1]);
-      }
-    }
-    __syncthreads();
-    if (ep) lu_sweeps_ep<T>(A, Ti, X, smem + ep_offset(g.nb), rs, P);
-    else lu_sweeps<T>(g, A, Ti, X, rs, P);
-    for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
-      dx[k] = (s == 0 ? 0.0 : dx[k]) + (double)X[k];
-    __syncthreads();
+  cd acc = mk(0.0, 0.0);
+  for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++) {
+    const int j = g.ycol[p];
+    const int bj = g.blk_of_bus[j];
+    if (bj < 0) continue;
+    const cd vj = V[j];
+    const double m = dx[2 * bj + 1] / cabs_(vj);
+    acc = cadd(acc, cmul(Y[p], cmul(vj, mk(m, dx[2 * bj]))));
   }
+  const size_t o = (size_t)t * g.n + i;
+  const cd vi = V[i];
+  const cd dvi = cmul(vi, mk(dx[2 * k + 1] / cabs_(vi), dx[2 * k]));
+  const cd dS = cadd(cmul(dvi, cconj(d.Ib[o])), cmul(vi, cconj(acc)));
+  double *R = d.rhs + (size_t)t * 2 * g.nb;
+  R[2 * k] = F[2 * k] - dS.x;
+  R[2 * k + 1] = d.type[o] == 1 ? F[2 * k + 1] - dS.y : -dx[2 * k + 1];
 }
 
 // ---------------------------------------------------------------------------

@@ -55,6 +55,7 @@ struct DevInst {
   cd *Ib;                  // [T][n] Ybus*V at the last mismatch
   double *F;               // [T][2nb] block order (P, Q) per block
   double *dx;              // [T][2nb]
+  double *rhs;             // [T][2nb] refinement residual (block order)
   InstState *st;           // [T]
   int *active_count;       // [1]
   // partial mismatch norms, one per mismatch workgroup (NaN: a NaN entry)
