@@ -254,7 +254,8 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(g.ydiag_bus, S.ydiag_bus); UP(g.yf_ptr, S.yf_ptr);
   UP(g.yf_item, S.yf_item); UP(g.yt_ptr, S.yt_ptr); UP(g.yt_item, S.yt_item);
   UP(g.blk_of_bus, S.blk_of_bus); UP(g.bus_of_blk, S.bus_of_blk);
-  UP(g.y2lu, S.y2lu); UP(g.lrow, S.lrow); UP(g.lcol, S.lcol);
+  UP(g.y2lu, S.y2lu); UP(g.jac_lu, S.jac_lu); UP(g.jac_y, S.jac_y);
+  g.nJb = (int)S.jac_lu.size(); UP(g.lrow, S.lrow); UP(g.lcol, S.lcol);
   UP(g.ldiag, S.ldiag); UP(g.mirror, S.mirror);
   UP(g.br_from, e->br_from); UP(g.br_to, e->br_to);
   UP(g.bbr_ptr, S.bbr_ptr); UP(g.bbr, S.bbr);
@@ -521,7 +522,7 @@ template <typename T>
 static void launch_newton_step(bpf_engine *e) {
   B4<T> *LU = (B4<T> *)e->d_LU;
   tbegin(e, "jacobian");
-  hipLaunchKernelGGL(k_jacobian<T>, xgrid(e->T, e->g.nnzY, 256),
+  hipLaunchKernelGGL(k_jacobian<T>, xgrid(e->T, e->g.nJb, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU);
   tend(e);
   tbegin(e, "lu_factor");
@@ -965,7 +966,7 @@ extern "C" bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V,
   HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
   B4<double> *LU = (B4<double> *)e->d_LU;
   HIPCHK(hipMemset(LU, 0, sizeof(B4<double>) * (size_t)e->T * e->g.nnzLU));
-  hipLaunchKernelGGL(k_jacobian<double>, xgrid(e->T, e->g.nnzY, 256),
+  hipLaunchKernelGGL(k_jacobian<double>, xgrid(e->T, e->g.nJb, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
@@ -1075,7 +1076,7 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   B4<double> *LU = (B4<double> *)e->d_LU;
   const unsigned TT = (unsigned)T;
   tbegin(e, "adjoint");
-  hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, e->g.nnzY, 256),
+  hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, e->g.nJb, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
                      e->stream, e->g, e->d.st, 0, e->fplan, LU,

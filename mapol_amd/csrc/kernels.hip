@@ -1,12 +1,14 @@
 // HIP kernels of the batched AC power-flow engine (gfx950, wave64).
 //
 // Launch shapes:
-//   * element kernels (Ybus assembly, Jacobian fill, voltage update): 2-D grid
-//     (elements / 256, instances), one thread per element;
-//   * per-instance kernels (mismatch + verdicts, LU factorization, triangular
-//     solves with refinement, losses): one workgroup per instance; levels of
-//     the elimination tree are separated by workgroup barriers only, so no
-//     grid-wide synchronisation and no host round trip inside a stage.
+//   * element kernels (Ybus assembly, V0, mismatch rows, Jacobian fill,
+//     refinement residual, voltage update): one thread per element, several
+//     workgroups per instance, all of an instance's workgroups on one XCD
+//     (xcd_slot);
+//   * per-instance kernels (verdicts, LU factorization, triangular solves,
+//     losses): one workgroup per instance; the steps of the factorization and
+//     of the sweeps are separated by workgroup barriers only, so no grid-wide
+//     synchronisation and no host round trip inside a stage.
 #include <float.h>
 
 #include "kernels.hpp"
@@ -352,19 +354,20 @@ __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
 
 // ---------------------------------------------------------------------------
 // Jacobian fill into the L+U storage (compuateJacobianMatrix,
-// newtonpf.h:97-186): one thread per (Ybus position, instance) writes one
+// newtonpf.h:97-186): one thread per (Jacobian block, instance) writes one
 // 2x2 block.  Superset pattern: a PV bus keeps an identity Q row and a zero
 // magnitude column, so PV->PQ switching never changes the pattern.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *d.active_count = 0;
-  const XSlot sl = xcd_slot(parts_of(g.nnzY));
-  const int t = sl.t, p = sl.part * blockDim.x + threadIdx.x;
-  if (t >= d.T || p >= g.nnzY) return;
+  // one thread per Jacobian block, in block-position order: the writes into
+  // the factor slab are nearly contiguous, the reads of Ybus are gathered
+  const XSlot sl = xcd_slot(parts_of(g.nJb));
+  const int t = sl.t, r = sl.part * blockDim.x + threadIdx.x;
+  if (t >= d.T || r >= g.nJb) return;
   if (d.st[t].done) return;
-  int q = g.y2lu[p];
-  if (q < 0) return;
+  const int q = g.jac_lu[r], p = g.jac_y[r];
   int ii = g.yrowof[p], jj = g.ycol[p];
   const cd *V = d.V + (size_t)t * g.n;
   cd adm = d.Y[(size_t)t * g.nnzY + p];
@@ -521,6 +524,7 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   // product indices of the next chunk are loaded while the current chunk
   // reduces; a thread's first output descriptors while the products form
   constexpr int PF = CHUNK_PAIRS / 1024;
+  constexpr int NB = sizeof(T) == 4 ? 4 : 2;  // outputs per thread per batch
   int2 pidx[PF];
   int nq0 = 0, nq1 = 0;
   auto prefetch_pairs = [&](int c) {
@@ -535,6 +539,21 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   if (P.nchunk > 0) prefetch_pairs(0);
   for (int c = 0; c < P.nchunk; c++) {
     const int q0 = nq0, q1 = nq1;
+    // the thread's first batch of outputs: descriptors requested now, their
+    // Jacobian values after the products (all in flight across the barrier)
+    const int k0 = P.ch_task[c], k1 = P.ch_task[c + 1];
+    const int gsz = lane_group(k1 - k0, P.ch_max[c]);
+    const int lane = threadIdx.x & (gsz - 1);
+    const int stride = (int)blockDim.x / gsz;
+    const int kf = k0 + (int)threadIdx.x / gsz;
+    int4 ta[NB], tb[NB];
+    B4<T> av[NB];
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+      const int k = kf + u * stride;
+      ta[u] = k < k1 ? P.tk_a[k] : make_int4(0, 0, 0, 0);
+      tb[u] = k < k1 ? P.tk_b[k] : make_int4(0, 0, 0, 0);
+    }
 #pragma unroll
     for (int j = 0; j < PF; j++) {
       const int q = q0 + (int)threadIdx.x + j * (int)blockDim.x;
@@ -560,40 +579,51 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       }
       buf[q - q0] = v;
     }
-    const int k0 = P.ch_task[c], k1 = P.ch_task[c + 1];
-    const int gsz = lane_group(k1 - k0, P.ch_max[c]);
-    const int lane = threadIdx.x & (gsz - 1);
-    const int kf = k0 + (int)threadIdx.x / gsz;
-    int4 ta_f = make_int4(0, 0, 0, 0), tb_f = make_int4(0, 0, 0, 0);
-    if (kf < k1) {
-      ta_f = P.tk_a[kf];
-      tb_f = P.tk_b[kf];
-    }
+#pragma unroll
+    for (int u = 0; u < NB; u++)
+      av[u] = (lane == 0 && (ta[u].x & TASK_HAS_A)) ? A[ta[u].x & TASK_POS_MASK]
+                                                    : blk_zero<T>();
     __syncthreads();
     if (c + 1 < P.nchunk) prefetch_pairs(c + 1);
-    for (int k = kf; k < k1; k += (int)blockDim.x / gsz) {
-      const int4 ta = (k == kf) ? ta_f : P.tk_a[k];
-      const int p = ta.x & TASK_POS_MASK;
-      B4<T> e = (lane == 0 && (ta.x & TASK_HAS_A)) ? A[p] : blk_zero<T>();
-      for (int q = ta.y + lane; q < ta.z; q += gsz) blk_acc(e, buf[q - q0]);
-      group_sum4(e, gsz);
-      if (ta.x & TASK_LCOL) {
-        const int4 tb = (k == kf) ? tb_f : P.tk_b[k];
-        B4<T> pv = blk_zero<T>();
-        for (int q = tb.y + lane; q < tb.z; q += gsz) blk_acc(pv, buf[q - q0]);
-        group_sum4(pv, gsz);
-        if (lane == 0) {
-          B4<T> inv;
-          blk_inv(pv, inv);   // a singular pivot is flagged by its own output
-          A[p] = blk_mul(e, inv);
+    // outputs in batches of NB per thread: a later batch's descriptors, then
+    // its Jacobian values, are requested together before any is reduced
+    for (int kb = kf; kb < k1; kb += NB * stride) {
+      if (kb != kf) {
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+          const int k = kb + u * stride;
+          ta[u] = k < k1 ? P.tk_a[k] : make_int4(0, 0, 0, 0);
+          tb[u] = k < k1 ? P.tk_b[k] : make_int4(0, 0, 0, 0);
         }
-      } else if (lane == 0) {
-        if (ta.x & TASK_DIAG) {
-          B4<T> inv;
-          if (blk_inv(e, inv)) sing = 1;
-          A[p] = inv;
-        } else {
-          A[p] = e;
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+          av[u] = (lane == 0 && (ta[u].x & TASK_HAS_A)) ? A[ta[u].x & TASK_POS_MASK]
+                                                        : blk_zero<T>();
+      }
+#pragma unroll
+      for (int u = 0; u < NB; u++) {
+        if (kb + u * stride >= k1) break;
+        const int p = ta[u].x & TASK_POS_MASK;
+        B4<T> e = av[u];
+        for (int q = ta[u].y + lane; q < ta[u].z; q += gsz) blk_acc(e, buf[q - q0]);
+        group_sum4(e, gsz);
+        if (ta[u].x & TASK_LCOL) {
+          B4<T> pv = blk_zero<T>();
+          for (int q = tb[u].y + lane; q < tb[u].z; q += gsz) blk_acc(pv, buf[q - q0]);
+          group_sum4(pv, gsz);
+          if (lane == 0) {
+            B4<T> inv;
+            blk_inv(pv, inv);   // a singular pivot is flagged by its own output
+            A[p] = blk_mul(e, inv);
+          }
+        } else if (lane == 0) {
+          if (ta[u].x & TASK_DIAG) {
+            B4<T> inv;
+            if (blk_inv(e, inv)) sing = 1;
+            A[p] = inv;
+          } else {
+            A[p] = e;
+          }
         }
       }
     }

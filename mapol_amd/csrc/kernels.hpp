@@ -38,6 +38,9 @@ struct DevTopo {
   const int *br_from, *br_to, *bbr_ptr, *bbr;
   const double *R, *X, *B, *shift, *Gs, *Qd, *Va0;
   const int *base_type;
+  // Jacobian blocks in L+U position order: position, Ybus position
+  int nJb;
+  const int *jac_lu, *jac_y;
 };
 
 // per-instance arrays on the device

@@ -222,6 +222,13 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     S.y2lu[p] = q;
     if (!inJ[q]) { inJ[q] = 1; S.nnz_jac_blocks++; }
   }
+  {
+    std::vector<int> yof(nnzLU, -1);
+    for (int p = 0; p < nnzY; p++)
+      if (S.y2lu[p] >= 0) yof[S.y2lu[p]] = p;
+    for (int q = 0; q < nnzLU; q++)
+      if (yof[q] >= 0) { S.jac_lu.push_back(q); S.jac_y.push_back(yof[q]); }
+  }
   // ---- Crout schedule ------------------------------------------------------
   std::vector<int> lev(nb, 0);
   for (int k = 0; k < nb; k++)
@@ -319,7 +326,13 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
       // pivot's products)
       std::vector<std::pair<int, int>> ord;  // (group, index in group)
       for (int gi : cg)
-        for (int q = 0; q < (int)groups[gi].size(); q++) ord.push_back({gi, q});
+        for (int q = 0; q < (int)groups[gi].size(); q++) {
+          const Task &t = groups[gi][q];
+          // a U block without products keeps its Jacobian value: nothing to do
+          if (t.L.empty() && (t.dst & TASK_HAS_A) && !(t.dst & (TASK_DIAG | TASK_LCOL)))
+            continue;
+          ord.push_back({gi, q});
+        }
       auto work = [&](const std::pair<int, int> &x) {
         const Task &t = groups[x.first][x.second];
         size_t w = t.L.size();

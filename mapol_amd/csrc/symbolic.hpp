@@ -36,6 +36,8 @@ struct Symbolic {
   // block CSR of L+U (structurally symmetric), rows in elimination order
   std::vector<int> lrow, lcol, ldiag, mirror;
   std::vector<int> y2lu;  // Ybus position -> block position (-1: slack)
+  // the Jacobian blocks sorted by block position: position, Ybus position
+  std::vector<int> jac_lu, jac_y;
   int nnz_jac_blocks = 0;
   // Crout factorization schedule (one level = one elimination-tree height)
   int nlev = 0;

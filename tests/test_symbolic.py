@@ -260,18 +260,24 @@ def test_schedule_invariants():
     pat = {(r, c) for r in range(nb) for c in ci[rp[r]:rp[r + 1]]}
     assert all((c, r) in pat for r, c in pat)
     assert all((r, r) in pat for r in range(nb))
-    # every position is written exactly once: by a chunk output or, for the
-    # blocks of the dense tail, not at all (the tail keeps its inverse)
+    # every position is written at most once, by a chunk output; the blocks
+    # of the dense tail are not written (the tail keeps its inverse) and a
+    # U block without products keeps its Jacobian value
     dst = a["tk_a"][:, 0] & MASK
     tail = set(int(k) for k in a["tail_piv"])
     in_tail = np.array([(r in tail and ci[p] in tail)
                         for r in range(nb) for p in range(rp[r], rp[r + 1])])
     assert len(np.unique(dst)) == len(dst)
-    assert np.array_equal(np.sort(dst), np.flatnonzero(~in_tail))
+    assert not np.any(in_tail[dst])
+    rowof = np.repeat(np.arange(nb), np.diff(rp))
+    unwritten = np.setdiff1d(np.flatnonzero(~in_tail), dst)
+    # unwritten non-tail positions: U blocks (row < column) holding J values
+    assert np.all(rowof[unwritten] < ci[unwritten])
     # a product only reads positions finished in an earlier chunk (or its
     # own pivot's block, as an init entry); an L output's pivot range lies
     # in its own chunk
     done_at = np.full(nnz, 10 ** 9)
+    done_at[unwritten] = -1          # final from the start
     for c in range(i["chunks"]):
         done_at[dst[a["ch_task"][c]:a["ch_task"][c + 1]]] = c
     for c in range(i["chunks"]):
@@ -290,7 +296,7 @@ def test_schedule_invariants():
     # every block of the Jacobian pattern outside the tail holds a Jacobian
     # value: directly (HAS_A) or as its pivot's init entry
     n_init = int((a["pairs"][:, 1] < 0).sum())
-    assert n_init + int((a["tk_a"][:, 0] & HAS_A > 0).sum()) \
+    assert n_init + int((a["tk_a"][:, 0] & HAS_A > 0).sum()) + len(unwritten) \
         + int((a["tl_dense"] & HAS_A > 0).sum()) == i["nnz_jac_blocks"]
 
 
