@@ -517,8 +517,16 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   const int t = blockIdx.x;
   if (st[t].done) return;
   __shared__ B4<T> buf[CHUNK_PAIRS];
+  // chunk tables in LDS: scalar loads inside a phase would delay its LDS
+  // accesses (both wait on the same counter)
+  __shared__ int s_ct[MAX_CHUNKS + 1], s_cp[MAX_CHUNKS + 1], s_cm[MAX_CHUNKS];
   __shared__ int sing;
   if (threadIdx.x == 0) sing = 0;
+  for (int q = threadIdx.x; q <= P.nchunk; q += blockDim.x) {
+    s_ct[q] = P.ch_task[q];
+    s_cp[q] = P.ch_pair[q];
+    if (q < P.nchunk) s_cm[q] = P.ch_max[q];
+  }
   B4<T> *A = LU + (size_t)t * g.nnzLU;
   __syncthreads();
   // product indices of the next chunk are loaded while the current chunk
@@ -528,8 +536,8 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   int2 pidx[PF];
   int nq0 = 0, nq1 = 0;
   auto prefetch_pairs = [&](int c) {
-    nq0 = P.ch_pair[c];
-    nq1 = P.ch_pair[c + 1];
+    nq0 = s_cp[c];
+    nq1 = s_cp[c + 1];
 #pragma unroll
     for (int j = 0; j < PF; j++) {
       const int q = nq0 + (int)threadIdx.x + j * (int)blockDim.x;
@@ -541,8 +549,8 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     const int q0 = nq0, q1 = nq1;
     // the thread's first batch of outputs: descriptors requested now, their
     // Jacobian values after the products (all in flight across the barrier)
-    const int k0 = P.ch_task[c], k1 = P.ch_task[c + 1];
-    const int gsz = lane_group(k1 - k0, P.ch_max[c]);
+    const int k0 = s_ct[c], k1 = s_ct[c + 1];
+    const int gsz = lane_group(k1 - k0, s_cm[c]);
     const int lane = threadIdx.x & (gsz - 1);
     const int stride = (int)blockDim.x / gsz;
     const int kf = k0 + (int)threadIdx.x / gsz;
@@ -893,9 +901,10 @@ struct EntVal {
   B4<T> D;          // pivot inverse of the first row (backward sweep)
 };
 
-__device__ __forceinline__ void ep_index(const SolvePlan &P, int c, EntIdx &ix) {
-  ix.meta = P.sw_chunk[c];
-  ix.gsz = lane_group(ix.meta.y - ix.meta.x, P.sw_cmax[c]);
+__device__ __forceinline__ void ep_index(const SolvePlan &P, const int4 *swc,
+                                         const int *swm, int c, EntIdx &ix) {
+  ix.meta = swc[c];
+  ix.gsz = lane_group(ix.meta.y - ix.meta.x, swm[c]);
   ix.rfirst = ix.meta.x + (int)threadIdx.x / ix.gsz;
   ix.rw = ix.rfirst < ix.meta.y ? P.sw_row[ix.rfirst] : make_int4(0, 0, 0, 0);
   const int e0 = ix.meta.z + threadIdx.x, e1 = e0 + blockDim.x;
@@ -924,10 +933,12 @@ __device__ __forceinline__ void ep_product(T *prod, int e, const B4<T> &a,
 // one chunk; before its row phase, the next chunk's values are taken from
 // its loaded indices and the indices of the one after are requested
 template <typename T>
-__device__ void ep_chunk(const SolvePlan &P, const B4<T> *A, T *X, T *prod,
-                         const EntVal<T> &cur, bool bw, bool has_next,
-                         bool next_bw, int c_after, EntIdx &ix_next,
-                         EntVal<T> &nxt) {
+__device__ __forceinline__ void ep_chunk(const SolvePlan &P, const int4 *swc,
+                                         const int *swm, const B4<T> *A, T *X,
+                                         T *prod, const EntVal<T> &cur, bool bw,
+                                         bool has_next, bool next_bw,
+                                         int c_after, EntIdx &ix_next,
+                                         EntVal<T> &nxt) {
   const int4 meta = cur.ix.meta;
   if (cur.ix.e0.x >= 0) ep_product(prod, threadIdx.x, cur.a0, X, cur.ix.e0.y);
   if (cur.ix.e1.x >= 0) ep_product(prod, threadIdx.x + blockDim.x, cur.a1, X, cur.ix.e1.y);
@@ -938,7 +949,7 @@ __device__ void ep_chunk(const SolvePlan &P, const B4<T> *A, T *X, T *prod,
   __syncthreads();
   if (has_next) {
     ep_values(A, ix_next, next_bw, nxt);
-    if (c_after >= 0) ep_index(P, c_after, ix_next);
+    if (c_after >= 0) ep_index(P, swc, swm, c_after, ix_next);
   }
   const int gsz = cur.ix.gsz;
   const int lane = threadIdx.x & (gsz - 1);
@@ -969,20 +980,22 @@ __device__ void ep_chunk(const SolvePlan &P, const B4<T> *A, T *X, T *prod,
 }
 
 template <typename T>
-__device__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti, T *X, T *prod,
-                             T *rs, const SolvePlan &P) {
+__device__ __forceinline__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti,
+                                             T *X, T *prod, T *rs,
+                                             const int4 *swc, const int *swm,
+                                             const SolvePlan &P) {
   const int nc = P.sw_nfw + P.sw_nbw;
   EntIdx ixn;
   EntVal<T> cur, nxt;
   if (nc > 0) {
-    ep_index(P, 0, ixn);
+    ep_index(P, swc, swm, 0, ixn);
     ep_values(A, ixn, 0 >= P.sw_nfw, cur);
   }
-  if (nc > 1) ep_index(P, 1, ixn);
+  if (nc > 1) ep_index(P, swc, swm, 1, ixn);
   for (int c = 0; c < nc; c++) {
     if (c == P.sw_nfw && P.tn > 0) tail_rows<T>(A, Ti, X, rs, P, false);
-    ep_chunk(P, A, X, prod, cur, c >= P.sw_nfw, c + 1 < nc, c + 1 >= P.sw_nfw,
-             c + 2 < nc ? c + 2 : -1, ixn, nxt);
+    ep_chunk(P, swc, swm, A, X, prod, cur, c >= P.sw_nfw, c + 1 < nc,
+             c + 1 >= P.sw_nfw, c + 2 < nc ? c + 2 : -1, ixn, nxt);
     cur = nxt;
   }
   if (P.sw_nbw == 0 && P.tn > 0) tail_rows<T>(A, Ti, X, rs, P, false);
@@ -993,10 +1006,11 @@ __device__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti, T *X, T *prod,
 // mixed precision runs it again on the FP64 residual of k_residual and
 // accumulates).  One workgroup per instance.
 // ---------------------------------------------------------------------------
-// LDS reserved for the solve vector of one instance: 150 KB of gfx950's
-// 160 KB, i.e. 19200 blocks in FP32 or 9600 in FP64; larger systems keep the
-// vector in global memory (use_lds = 0)
-constexpr int SOLVE_LDS_BYTES = 150 * 1024;
+// LDS reserved for the solve vector of one instance: 148 KB of gfx950's
+// 160 KB (the rest holds the tail right-hand side and the chunk table),
+// i.e. 18944 blocks in FP32 or 9472 in FP64; larger systems keep the vector
+// in global memory (use_lds = 0)
+constexpr int SOLVE_LDS_BYTES = 148 * 1024;
 // the entry-parallel sweeps keep their product buffer (2 * SWEEP_CHUNK
 // scalars) after X in the same LDS block
 __host__ __device__ constexpr int ep_offset(int nb) { return (2 * nb + 3) & ~3; }
@@ -1007,6 +1021,9 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
                            SolvePlan P, int use_lds, int ep, T *xglob) {
   __shared__ T smem[SOLVE_LDS_BYTES / sizeof(T)];
   __shared__ T rs[2 * TAIL_MAX];
+  // the entry-parallel chunk table in LDS (see the factorization)
+  __shared__ int4 s_swc[MAX_CHUNKS];
+  __shared__ int s_swm[MAX_CHUNKS];
   const int t = blockIdx.x;
   if (d.st[t].done) return;
   T *X = use_lds ? smem : xglob + (size_t)t * 2 * g.nb;
@@ -1014,6 +1031,11 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
   const B4<T> *Ti = TL + (size_t)t * P.tn * P.tn;
   const double *F = d.F + (size_t)t * 2 * g.nb;
   const double *R = d.rhs + (size_t)t * 2 * g.nb;
+  if (ep)
+    for (int q = threadIdx.x; q < P.sw_nfw + P.sw_nbw; q += blockDim.x) {
+      s_swc[q] = P.sw_chunk[q];
+      s_swm[q] = P.sw_cmax[q];
+    }
   double *dx = d.dx + (size_t)t * 2 * g.nb;
   // rhs_mode 0: (P, Q) mismatch; 1: (P, 0) and 2: (0, Q) for the decoupled
   // half steps; 3: the refinement residual of k_residual
@@ -1021,7 +1043,7 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
     X[k] = rhs_mode == 3 ? (T)R[k]
            : (rhs_mode == 0 || (rhs_mode == 1) == ((k & 1) == 0)) ? (T)F[k] : (T)0;
   __syncthreads();
-  if (ep) lu_sweeps_ep<T>(A, Ti, X, smem + ep_offset(g.nb), rs, P);
+  if (ep) lu_sweeps_ep<T>(A, Ti, X, smem + ep_offset(g.nb), rs, s_swc, s_swm, P);
   else lu_sweeps<T>(g, A, Ti, X, rs, P);
   for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
     dx[k] = (accumulate ? dx[k] : 0.0) + (double)X[k];

@@ -375,6 +375,12 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   }
   S.nchunk = (int)S.ch_max.size();
   S.npairs = (int64_t)S.pairs.size() / 2;
+  if (S.nchunk > MAX_CHUNKS) {
+    if (err) *err = "factorization schedule has " + std::to_string(S.nchunk) +
+                    " chunks, more than the " + std::to_string(MAX_CHUNKS) +
+                    " the kernel keeps in LDS";
+    return false;
+  }
   // ---- dense tail ------------------------------------------------------------
   {
     const int Ls = S.tail_level;
@@ -477,6 +483,8 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
           const int i = rows[r];
           const int p0 = fw ? S.lrow[i] : S.ldiag[i] + 1;
           const int p1 = fw ? S.ldiag[i] : S.lrow[i + 1];
+          // a forward row without L entries keeps x = b: nothing to do
+          if (fw && p1 == p0) { r++; continue; }
           const int ne = (int)S.sw_ent.size() / 2;
           if (ne > e0 && ne - e0 + (p1 - p0) > SWEEP_CHUNK) break;
           S.sw_row.insert(S.sw_row.end(), {i, ne, ne + (p1 - p0), fw ? -1 : S.ldiag[i]});
@@ -487,6 +495,7 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
           cmax = std::max(cmax, p1 - p0);
           r++;
         }
+        if ((int)S.sw_row.size() / 4 == r0) continue;  // level left empty
         S.sw_chunk.insert(S.sw_chunk.end(), {r0, (int)S.sw_row.size() / 4, e0,
                                              (int)S.sw_ent.size() / 2});
         S.sw_cmax.push_back(cmax);
@@ -498,7 +507,8 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   S.sw_nfw = chunks(S.fw_ptr, S.fw_rows, S.nfw, true);
   S.sw_nbw = chunks(S.bw_ptr, S.bw_rows, S.nbw, false);
   // a row longer than a chunk cannot take the entry-parallel path
-  S.sw_ok = 1;
+  // (and the chunk table must fit the kernel's LDS copy)
+  S.sw_ok = (int)S.sw_cmax.size() <= MAX_CHUNKS;
   for (size_t c = 0; c < S.sw_cmax.size(); c++)
     if (S.sw_chunk[4 * c + 3] - S.sw_chunk[4 * c + 2] > SWEEP_CHUNK) S.sw_ok = 0;
   return true;

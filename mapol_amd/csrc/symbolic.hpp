@@ -18,6 +18,7 @@ constexpr int TASK_LCOL = 1 << 28;   // L block: scale by the pivot inverse
 constexpr int TASK_POS_MASK = (1 << 28) - 1;
 constexpr int CHUNK_PAIRS = 4096;    // products per factorization chunk (LDS)
 constexpr int SWEEP_CHUNK = 2048;    // entries per sweep chunk (LDS)
+constexpr int MAX_CHUNKS = 512;      // chunk tables the kernels keep in LDS
 constexpr int TAIL_MAX = 64;         // dense tail: at most 64 blocks square
 
 struct Symbolic {
