@@ -25,7 +25,7 @@ METRIC = ("contingency PF solves/s, 9241-bus×200, mixed vs FP64 NR at "
           "1e-8 pu mismatch")
 HBM_PEAK_GBS = 8000.0
 KERNELS = ["ybus", "init", "mismatch", "jacobian", "lu_factor", "lu_solve",
-           "update", "loss"]
+           "residual", "update", "loss"]
 
 
 def log(*a):
@@ -45,14 +45,17 @@ def algorithmic_bytes(info, case, T, its_total, s, refine):
         "mismatch": 16 * nY + 48 * n + 16 * nb + n,
         "jacobian": 16 * nY + 32 * n + n + 4 * s * nJ,
         "lu_factor": 4 * s * nJ + 4 * s * nLU,
-        "lu_solve": (1 + refine) * 4 * s * nLU + 32 * nb
-        + refine * (16 * nY + 32 * n + 16 * nb),
+        # every pass reads the factor and its right-hand side and writes dx;
+        # the refinement passes also read dx (they accumulate)
+        "lu_solve": (1 + refine) * (4 * s * nLU + 32 * nb) + refine * 16 * nb,
+        "residual": refine * (16 * nY + 33 * n + 48 * nb),
         "update": 48 * nb + n,
         "loss": 16 * n + 41 * m,
     }
     launches_inst = {
         "ybus": T, "loss": T, "mismatch": T + its_total,
         "jacobian": its_total, "lu_factor": its_total, "lu_solve": its_total,
+        "residual": its_total,
         "update": its_total,
     }
     return {k: per_inst[k] * launches_inst[k] for k in per_inst}

@@ -123,6 +123,12 @@ static bpf_status dalloc(bpf_engine *e, std::vector<DBuf> &pool, size_t bytes,
   return BPF_OK;
 }
 
+// plan arrays the kernels index with clamped indices: never empty on the
+// device (a zero element of `width` ints stands in for an empty one)
+static std::vector<int> nonempty(const std::vector<int> &v, int width) {
+  return v.empty() ? std::vector<int>(width, 0) : v;
+}
+
 template <typename V>
 static bpf_status upload(bpf_engine *e, std::vector<DBuf> &pool, const V &v,
                          const typename V::value_type **out) {
@@ -267,7 +273,8 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(F.ch_task, S.ch_task); UP(F.ch_pair, S.ch_pair); UP(F.ch_max, S.ch_max);
   {
     const int *ta, *tb, *pr;
-    UP(ta, S.tk_a); UP(tb, S.tk_b); UP(pr, S.pairs);
+    UP(ta, nonempty(S.tk_a, 4)); UP(tb, nonempty(S.tk_b, 4));
+    UP(pr, nonempty(S.pairs, 2));
     F.tk_a = (const int4 *)ta;
     F.tk_b = (const int4 *)tb;
     F.pairs = (const int2 *)pr;
@@ -291,7 +298,8 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(SP.sw_cmax, S.sw_cmax);
   {
     const int *ch, *rw, *en;
-    UP(ch, S.sw_chunk); UP(rw, S.sw_row); UP(en, S.sw_ent);
+    UP(ch, S.sw_chunk); UP(rw, nonempty(S.sw_row, 4));
+    UP(en, nonempty(S.sw_ent, 2));
     SP.sw_chunk = (const int4 *)ch;
     SP.sw_row = (const int4 *)rw;
     SP.sw_ent = (const int2 *)en;
@@ -536,18 +544,21 @@ static void launch_newton_step(bpf_engine *e) {
   const int ep = use_entry_sweeps(e->S, sizeof(T), use_lds, e->sweep_rows);
   // one solve on the mismatch, then per refinement step the FP64 residual
   // (many workgroups per instance) and a solve on it, accumulated into dx
-  tbegin(e, "lu_solve");
   for (int s = 0; s <= refine; s++) {
-    if (s > 0)
+    if (s > 0) {
+      tbegin(e, "residual");
       hipLaunchKernelGGL(k_residual, xgrid(e->T, e->n, 256), dim3(256), 0,
                          e->stream, e->g, e->d);
+      tend(e);
+    }
+    tbegin(e, "lu_solve");
     hipLaunchKernelGGL(k_lu_solve<T>, dim3(e->T), dim3(e->solve_threads),
                        0, e->stream, e->g, e->d,
                        (const B4<T> *)LU, (const B4<T> *)e->d_TL,
                        s == 0 ? 0 : 3, s > 0 ? 1 : 0, e->splan, use_lds, ep,
                        (T *)e->d_xglob);
+    tend(e);
   }
-  tend(e);
 }
 
 static void launch_iteration(bpf_engine *e) {

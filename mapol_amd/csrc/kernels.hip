@@ -395,6 +395,17 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU) {
 // ---------------------------------------------------------------------------
 // 2x2 block helpers
 // ---------------------------------------------------------------------------
+// component-wise select: a select of whole blocks can become a select of
+// their stack addresses (scratch memory)
+template <typename T>
+__device__ __forceinline__ B4<T> blk_sel(bool c, const B4<T> &x, const B4<T> &y) {
+  B4<T> r;
+  r.a = c ? x.a : y.a;
+  r.b = c ? x.b : y.b;
+  r.c = c ? x.c : y.c;
+  r.d = c ? x.d : y.d;
+  return r;
+}
 template <typename T>
 __device__ __forceinline__ void blk_fms(B4<T> &acc, const B4<T> &l, const B4<T> &u) {
   acc.a -= l.a * u.a + l.b * u.c;
@@ -448,6 +459,17 @@ struct SolvePlan {
 // and few enough that every item of the level gets its group in one pass
 // when the workgroup is large enough.  Groups are aligned inside a wavefront,
 // so their partial sums combine with cross-lane shuffles.
+// s_waitcnt immediate (gfx9 encoding) that waits for every vector memory
+// operation: vmcnt 0, expcnt and lgkmcnt left at their maxima
+constexpr int WAIT_VMCNT0 = 0x0F70;
+
+// whether any lane of the wave has an index below `end`, for an index that
+// grows with the lane: the first lane decides, so the answer is one scalar
+// for the wave and a load under it is skipped by a scalar branch
+__device__ __forceinline__ bool wave_any(int index, int end) {
+  return __builtin_amdgcn_readfirstlane(index) < end;
+}
+
 __device__ __forceinline__ int lane_group(int items, int maxlen) {
   int gsz = 1;
   while (gsz < 64 && gsz < maxlen && items * 2 * gsz <= (int)blockDim.x) gsz <<= 1;
@@ -530,7 +552,12 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   B4<T> *A = LU + (size_t)t * g.nnzLU;
   __syncthreads();
   // product indices of the next chunk are loaded while the current chunk
-  // reduces; a thread's first output descriptors while the products form
+  // reduces; a thread's first output descriptors while the products form.
+  // Loads past the end of a range read its last element (every plan array
+  // holds at least one) and are masked where used: a load under a condition
+  // whose result merges with a default makes the compiler wait for it at
+  // once, and global loads complete in order, so one such wait drains every
+  // prefetch in flight.
   constexpr int PF = CHUNK_PAIRS / 1024;
   constexpr int NB = sizeof(T) == 4 ? 4 : 2;  // outputs per thread per batch
   int2 pidx[PF];
@@ -538,43 +565,81 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   auto prefetch_pairs = [&](int c) {
     nq0 = s_cp[c];
     nq1 = s_cp[c + 1];
+    const int qlast = max(nq1 - 1, 0);
 #pragma unroll
     for (int j = 0; j < PF; j++) {
       const int q = nq0 + (int)threadIdx.x + j * (int)blockDim.x;
-      pidx[j] = q < nq1 ? P.pairs[q] : make_int2(0, 0);
+      if (wave_any(q, nq1)) pidx[j] = P.pairs[min(q, qlast)];
     }
   };
   if (P.nchunk > 0) prefetch_pairs(0);
+  // Loads of a slot no lane of the wave needs are skipped by a scalar branch
+  // (wave_any), so the number of loads in flight is not known to the
+  // compiler and any wait for one of them waits for all (global loads
+  // complete in order).  The order below makes each such wait fall where
+  // everything in flight is needed anyway: one explicit wait at the top of
+  // the chunk (the pair indices and the previous chunk's stores), then
+  // operands and descriptors, then the Jacobian values (once the descriptors
+  // arrive, in flight while the products form), and the next chunk's pair
+  // indices only once the Jacobian values are in registers.
   for (int c = 0; c < P.nchunk; c++) {
+    __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     const int q0 = nq0, q1 = nq1;
-    // the thread's first batch of outputs: descriptors requested now, their
-    // Jacobian values after the products (all in flight across the barrier)
     const int k0 = s_ct[c], k1 = s_ct[c + 1];
     const int gsz = lane_group(k1 - k0, s_cm[c]);
     const int lane = threadIdx.x & (gsz - 1);
     const int stride = (int)blockDim.x / gsz;
     const int kf = k0 + (int)threadIdx.x / gsz;
+    const int klast = max(k1 - 1, 0);
+    // operands of the first PR slots requested before any product forms
+    // (all slots in FP32; FP64 takes two rounds to stay within registers)
+    constexpr int PR = sizeof(T) == 4 ? PF : PF / 2;
+    B4<T> lop[PR], rop[PR];
+    auto load_operands = [&](int j0) {
+#pragma unroll
+      for (int j = 0; j < PR; j++) {
+        if (wave_any(q0 + (int)threadIdx.x + (j0 + j) * (int)blockDim.x, q1)) {
+          lop[j] = A[pidx[j0 + j].x];
+          rop[j] = A[pidx[j0 + j].y < 0 ? pidx[j0 + j].x : pidx[j0 + j].y];
+        }
+      }
+    };
+    auto form_products = [&](int j0) {
+#pragma unroll
+      for (int j = 0; j < PR; j++) {
+        const int q = q0 + (int)threadIdx.x + (j0 + j) * (int)blockDim.x;
+        if (wave_any(q, q1)) {
+          B4<T> v = blk_zero<T>();
+          blk_fms(v, lop[j], rop[j]);
+          v = blk_sel(pidx[j0 + j].y < 0, lop[j], v);   // an init entry carries the block itself
+          if (q < q1) buf[q - q0] = v;
+        }
+      }
+    };
+    load_operands(0);
+    // the thread's first batch of outputs: descriptors
     int4 ta[NB], tb[NB];
     B4<T> av[NB];
 #pragma unroll
     for (int u = 0; u < NB; u++) {
       const int k = kf + u * stride;
-      ta[u] = k < k1 ? P.tk_a[k] : make_int4(0, 0, 0, 0);
-      tb[u] = k < k1 ? P.tk_b[k] : make_int4(0, 0, 0, 0);
-    }
-#pragma unroll
-    for (int j = 0; j < PF; j++) {
-      const int q = q0 + (int)threadIdx.x + j * (int)blockDim.x;
-      if (q < q1) {
-        B4<T> v;
-        if (pidx[j].y < 0) {
-          v = A[pidx[j].x];
-        } else {
-          v = blk_zero<T>();
-          blk_fms(v, A[pidx[j].x], A[pidx[j].y]);
-        }
-        buf[q - q0] = v;
+      if (wave_any(k, k1)) {
+        ta[u] = P.tk_a[min(k, klast)];
+        tb[u] = P.tk_b[min(k, klast)];
       }
+    }
+    __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+    // Jacobian values of the first batch; outputs without one read block 0
+    // (cached) and drop it
+#pragma unroll
+    for (int u = 0; u < NB; u++)
+      if (wave_any(kf + u * stride, k1))
+        av[u] = A[(ta[u].x & TASK_HAS_A) ? (ta[u].x & TASK_POS_MASK) : 0];
+    form_products(0);
+#pragma unroll
+    for (int j0 = PR; j0 < PF; j0 += PR) {
+      load_operands(j0);
+      form_products(j0);
     }
     for (int q = q0 + threadIdx.x + PF * blockDim.x; q < q1; q += blockDim.x) {
       const int2 pr = P.pairs[q];
@@ -587,11 +652,13 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       }
       buf[q - q0] = v;
     }
+    __syncthreads();
+    B4<T> ev[NB];
 #pragma unroll
     for (int u = 0; u < NB; u++)
-      av[u] = (lane == 0 && (ta[u].x & TASK_HAS_A)) ? A[ta[u].x & TASK_POS_MASK]
-                                                    : blk_zero<T>();
-    __syncthreads();
+      ev[u] = wave_any(kf + u * stride, k1)
+                  ? blk_sel(lane == 0 && (ta[u].x & TASK_HAS_A), av[u], blk_zero<T>())
+                  : blk_zero<T>();
     if (c + 1 < P.nchunk) prefetch_pairs(c + 1);
     // outputs in batches of NB per thread: a later batch's descriptors, then
     // its Jacobian values, are requested together before any is reduced
@@ -600,19 +667,23 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
 #pragma unroll
         for (int u = 0; u < NB; u++) {
           const int k = kb + u * stride;
-          ta[u] = k < k1 ? P.tk_a[k] : make_int4(0, 0, 0, 0);
-          tb[u] = k < k1 ? P.tk_b[k] : make_int4(0, 0, 0, 0);
+          if (wave_any(k, k1)) {
+            ta[u] = P.tk_a[min(k, klast)];
+            tb[u] = P.tk_b[min(k, klast)];
+          }
         }
 #pragma unroll
         for (int u = 0; u < NB; u++)
-          av[u] = (lane == 0 && (ta[u].x & TASK_HAS_A)) ? A[ta[u].x & TASK_POS_MASK]
-                                                        : blk_zero<T>();
+          if (wave_any(kb + u * stride, k1))
+            ev[u] = blk_sel(lane == 0 && (ta[u].x & TASK_HAS_A),
+                            A[(ta[u].x & TASK_HAS_A) ? (ta[u].x & TASK_POS_MASK) : 0],
+                            blk_zero<T>());
       }
 #pragma unroll
       for (int u = 0; u < NB; u++) {
         if (kb + u * stride >= k1) break;
         const int p = ta[u].x & TASK_POS_MASK;
-        B4<T> e = av[u];
+        B4<T> e = ev[u];
         for (int q = ta[u].y + lane; q < ta[u].z; q += gsz) blk_acc(e, buf[q - q0]);
         group_sum4(e, gsz);
         if (ta[u].x & TASK_LCOL) {
@@ -646,12 +717,33 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     {
       const int gsz = lane_group(P.tnent, P.tmax);
       const int lane = threadIdx.x & (gsz - 1);
+      constexpr int TU = 4;  // products of one lane requested together
       for (int k = (int)threadIdx.x / gsz; k < P.tnent; k += (int)blockDim.x / gsz) {
         const int dd = P.tdense[k];
-        B4<T> acc = (lane == 0 && (dd & TASK_HAS_A)) ? A[P.tpos[k]] : blk_zero<T>();
-        const int q1 = P.tpp[k + 1];
-        for (int q = P.tpp[k] + lane; q < q1; q += gsz)
-          blk_fms(acc, A[P.tpairL[q]], A[P.tpairU[q]]);
+        const int pos = P.tpos[k];
+        const int qa = P.tpp[k], q1 = P.tpp[k + 1];
+        const B4<T> a0 = A[(dd & TASK_HAS_A) ? pos : 0];
+        B4<T> acc = blk_sel(lane == 0 && (dd & TASK_HAS_A), a0, blk_zero<T>());
+        for (int q = qa + lane; q < q1; q += TU * gsz) {
+          int il[TU], iu[TU];
+#pragma unroll
+          for (int u = 0; u < TU; u++) {
+            const int qq = min(q + u * gsz, q1 - 1);
+            il[u] = P.tpairL[qq];
+            iu[u] = P.tpairU[qq];
+          }
+          B4<T> lv[TU], uv[TU];
+#pragma unroll
+          for (int u = 0; u < TU; u++) {
+            lv[u] = A[il[u]];
+            uv[u] = A[iu[u]];
+          }
+          // a slot past the range repeats the last product with a zero left
+          // factor: the sum and its order are those of the plain loop
+#pragma unroll
+          for (int u = 0; u < TU; u++)
+            blk_fms(acc, blk_sel(q + u * gsz < q1, lv[u], blk_zero<T>()), uv[u]);
+        }
         group_sum4(acc, gsz);
         if (lane == 0) D[dd & TASK_POS_MASK] = acc;
       }
@@ -765,17 +857,31 @@ __device__ void tail_rows(const B4<T> *A, const B4<T> *Ti, T *X, T *rs,
     for (int a = (int)threadIdx.x / gsz; a < s; a += (int)blockDim.x / gsz) {
       T s0 = (T)0, s1 = (T)0;
       const int q1 = P.ts_ptr[a + 1];
-      for (int q = P.ts_ptr[a] + lane; q < q1; q += gsz) {
-        const int c = P.ts_col[q];
-        const T y0 = X[2 * c], y1 = X[2 * c + 1];
-        if (trans) {
-          const B4<T> U = A[P.ts_mir[q]];
-          s0 += U.a * y0 + U.c * y1;
-          s1 += U.b * y0 + U.d * y1;
-        } else {
-          const B4<T> L = A[P.ts_pos[q]];
-          s0 += L.a * y0 + L.b * y1;
-          s1 += L.c * y0 + L.d * y1;
+      // TU entries per lane requested together; a slot past the range
+      // repeats the last entry with a zero block (sum and order unchanged)
+      constexpr int TU = 4;
+      for (int q = P.ts_ptr[a] + lane; q < q1; q += TU * gsz) {
+        int cc[TU], pp[TU];
+#pragma unroll
+        for (int u = 0; u < TU; u++) {
+          const int qq = min(q + u * gsz, q1 - 1);
+          cc[u] = P.ts_col[qq];
+          pp[u] = trans ? P.ts_mir[qq] : P.ts_pos[qq];
+        }
+        B4<T> M[TU];
+#pragma unroll
+        for (int u = 0; u < TU; u++) M[u] = A[pp[u]];
+#pragma unroll
+        for (int u = 0; u < TU; u++) {
+          const B4<T> L = blk_sel(q + u * gsz < q1, M[u], blk_zero<T>());
+          const T y0 = X[2 * cc[u]], y1 = X[2 * cc[u] + 1];
+          if (trans) {
+            s0 += L.a * y0 + L.c * y1;
+            s1 += L.b * y0 + L.d * y1;
+          } else {
+            s0 += L.a * y0 + L.b * y1;
+            s1 += L.c * y0 + L.d * y1;
+          }
         }
       }
       s0 = group_sum(s0, gsz);
@@ -793,16 +899,26 @@ __device__ void tail_rows(const B4<T> *A, const B4<T> *Ti, T *X, T *rs,
     const int lane = threadIdx.x & (gsz - 1);
     for (int a = (int)threadIdx.x / gsz; a < s; a += (int)blockDim.x / gsz) {
       T s0 = (T)0, s1 = (T)0;
-      for (int b = lane; b < s; b += gsz) {
-        const T r0 = rs[2 * b], r1 = rs[2 * b + 1];
-        if (trans) {
-          const B4<T> M = Ti[b * s + a];
-          s0 += M.a * r0 + M.c * r1;
-          s1 += M.b * r0 + M.d * r1;
-        } else {
-          const B4<T> M = Ti[a * s + b];
-          s0 += M.a * r0 + M.b * r1;
-          s1 += M.c * r0 + M.d * r1;
+      constexpr int TU = 4;
+      for (int b = lane; b < s; b += TU * gsz) {
+        B4<T> Mv[TU];
+#pragma unroll
+        for (int u = 0; u < TU; u++) {
+          const int bb = min(b + u * gsz, s - 1);
+          Mv[u] = trans ? Ti[bb * s + a] : Ti[a * s + bb];
+        }
+#pragma unroll
+        for (int u = 0; u < TU; u++) {
+          const int bb = min(b + u * gsz, s - 1);
+          const B4<T> M = blk_sel(b + u * gsz < s, Mv[u], blk_zero<T>());
+          const T r0 = rs[2 * bb], r1 = rs[2 * bb + 1];
+          if (trans) {
+            s0 += M.a * r0 + M.c * r1;
+            s1 += M.b * r0 + M.d * r1;
+          } else {
+            s0 += M.a * r0 + M.b * r1;
+            s1 += M.c * r0 + M.d * r1;
+          }
         }
       }
       s0 = group_sum(s0, gsz);
@@ -890,8 +1006,8 @@ __device__ void lu_sweeps(const DevTopo &g, const B4<T> *A, const B4<T> *Ti,
 struct EntIdx {
   int4 meta;        // {row0, row1, entry0, entry1}
   int gsz, rfirst;  // lane group of the row phase, the thread's first row
-  int4 rw;          // its descriptor
-  int2 e0, e1;      // the thread's two entries {position, column} (x < 0: none)
+  int4 rw;          // its descriptor (row index clamped into the chunk)
+  int2 e0, e1;      // the thread's two entries {position, column} (clamped)
 };
 
 template <typename T>
@@ -901,25 +1017,32 @@ struct EntVal {
   B4<T> D;          // pivot inverse of the first row (backward sweep)
 };
 
+// Every load of the pipeline is unconditional, so that each chunk issues a
+// fixed number of loads and the compiler waits for exactly the ones a step
+// needs: an index past the end of its range is clamped to the range's last
+// element (or to element 0: the plan arrays are never empty) and the result
+// is masked where it is used.  A load under a condition (per lane or per
+// wave) leaves the count of younger loads unknown, and as global loads
+// complete in order the compiler then waits for all of them, prefetches
+// included.
 __device__ __forceinline__ void ep_index(const SolvePlan &P, const int4 *swc,
                                          const int *swm, int c, EntIdx &ix) {
   ix.meta = swc[c];
   ix.gsz = lane_group(ix.meta.y - ix.meta.x, swm[c]);
   ix.rfirst = ix.meta.x + (int)threadIdx.x / ix.gsz;
-  ix.rw = ix.rfirst < ix.meta.y ? P.sw_row[ix.rfirst] : make_int4(0, 0, 0, 0);
-  const int e0 = ix.meta.z + threadIdx.x, e1 = e0 + blockDim.x;
-  ix.e0 = e0 < ix.meta.w ? P.sw_ent[e0] : make_int2(-1, -1);
-  ix.e1 = e1 < ix.meta.w ? P.sw_ent[e1] : make_int2(-1, -1);
+  ix.rw = P.sw_row[max(min(ix.rfirst, ix.meta.y - 1), 0)];
+  const int e0 = ix.meta.z + threadIdx.x, elast = max(ix.meta.w - 1, 0);
+  ix.e0 = P.sw_ent[min(e0, elast)];
+  ix.e1 = P.sw_ent[min(e0 + (int)blockDim.x, elast)];
 }
 
 template <typename T>
 __device__ __forceinline__ void ep_values(const B4<T> *A, const EntIdx &ix,
                                           bool bw, EntVal<T> &v) {
   v.ix = ix;
-  v.a0 = ix.e0.x >= 0 ? A[ix.e0.x] : blk_zero<T>();
-  v.a1 = ix.e1.x >= 0 ? A[ix.e1.x] : blk_zero<T>();
-  const bool lead = (threadIdx.x & (ix.gsz - 1)) == 0;
-  v.D = (bw && lead && ix.rfirst < ix.meta.y) ? A[ix.rw.w] : blk_zero<T>();
+  v.a0 = A[ix.e0.x];
+  v.a1 = A[ix.e1.x];
+  v.D = A[bw ? ix.rw.w : 0];
 }
 
 template <typename T>
@@ -930,75 +1053,98 @@ __device__ __forceinline__ void ep_product(T *prod, int e, const B4<T> &a,
   prod[2 * e + 1] = a.c * y0 + a.d * y1;
 }
 
-// one chunk; before its row phase, the next chunk's values are taken from
-// its loaded indices and the indices of the one after are requested
+// one row of the row phase: subtract its products from X (and apply the
+// pivot inverse in the backward sweep)
+template <typename T>
+__device__ __forceinline__ void ep_row(T *X, const T *prod, int e_base,
+                                       const int4 &rw, const B4<T> &Dg, bool bw,
+                                       int gsz, int lane) {
+  T s0 = (T)0, s1 = (T)0;
+  for (int e = rw.y + lane; e < rw.z; e += gsz) {
+    s0 += prod[2 * (e - e_base)];
+    s1 += prod[2 * (e - e_base) + 1];
+  }
+  s0 = group_sum(s0, gsz);
+  s1 = group_sum(s1, gsz);
+  if (lane == 0) {
+    const int i = rw.x;
+    const T x0 = X[2 * i] - s0, x1 = X[2 * i + 1] - s1;
+    if (bw) {
+      X[2 * i] = Dg.a * x0 + Dg.b * x1;
+      X[2 * i + 1] = Dg.c * x0 + Dg.d * x1;
+    } else {
+      X[2 * i] = x0;
+      X[2 * i + 1] = x1;
+    }
+  }
+}
+
+// one chunk; before its row phase, the next chunk's values are requested
+// from its loaded indices and the indices of the one after are requested
+// (always: past the last chunk the caller passes the last chunk again, so
+// no branch separates the loaded registers from their uses).  The thread's
+// first row comes from registers; further rows (wide chunks only) load
+// their descriptors there.
 template <typename T>
 __device__ __forceinline__ void ep_chunk(const SolvePlan &P, const int4 *swc,
                                          const int *swm, const B4<T> *A, T *X,
                                          T *prod, const EntVal<T> &cur, bool bw,
-                                         bool has_next, bool next_bw,
-                                         int c_after, EntIdx &ix_next,
-                                         EntVal<T> &nxt) {
+                                         bool next_bw, int c_after,
+                                         EntIdx &ix_next, EntVal<T> &nxt) {
   const int4 meta = cur.ix.meta;
-  if (cur.ix.e0.x >= 0) ep_product(prod, threadIdx.x, cur.a0, X, cur.ix.e0.y);
-  if (cur.ix.e1.x >= 0) ep_product(prod, threadIdx.x + blockDim.x, cur.a1, X, cur.ix.e1.y);
-  for (int e = meta.z + threadIdx.x + 2 * blockDim.x; e < meta.w; e += blockDim.x) {
+  const int e0 = meta.z + threadIdx.x;
+  if (e0 < meta.w) ep_product(prod, threadIdx.x, cur.a0, X, cur.ix.e0.y);
+  if (e0 + (int)blockDim.x < meta.w)
+    ep_product(prod, threadIdx.x + blockDim.x, cur.a1, X, cur.ix.e1.y);
+  for (int e = e0 + 2 * blockDim.x; e < meta.w; e += blockDim.x) {
     const int2 en = P.sw_ent[e];
     ep_product(prod, e - meta.z, A[en.x], X, en.y);
   }
   __syncthreads();
-  if (has_next) {
-    ep_values(A, ix_next, next_bw, nxt);
-    if (c_after >= 0) ep_index(P, swc, swm, c_after, ix_next);
-  }
+  ep_values(A, ix_next, next_bw, nxt);
+  ep_index(P, swc, swm, c_after, ix_next);
   const int gsz = cur.ix.gsz;
   const int lane = threadIdx.x & (gsz - 1);
-  for (int r = cur.ix.rfirst; r < meta.y; r += (int)blockDim.x / gsz) {
-    const bool first = r == cur.ix.rfirst;
-    const int4 rw = first ? cur.ix.rw : P.sw_row[r];
-    const B4<T> Dg = first ? cur.D : ((bw && lane == 0) ? A[rw.w] : blk_zero<T>());
-    T s0 = (T)0, s1 = (T)0;
-    for (int e = rw.y + lane; e < rw.z; e += gsz) {
-      s0 += prod[2 * (e - meta.z)];
-      s1 += prod[2 * (e - meta.z) + 1];
-    }
-    s0 = group_sum(s0, gsz);
-    s1 = group_sum(s1, gsz);
-    if (lane == 0) {
-      const int i = rw.x;
-      const T x0 = X[2 * i] - s0, x1 = X[2 * i + 1] - s1;
-      if (bw) {
-        X[2 * i] = Dg.a * x0 + Dg.b * x1;
-        X[2 * i + 1] = Dg.c * x0 + Dg.d * x1;
-      } else {
-        X[2 * i] = x0;
-        X[2 * i + 1] = x1;
-      }
-    }
+  const int rstep = (int)blockDim.x / gsz;
+  if (cur.ix.rfirst < meta.y)
+    ep_row(X, prod, meta.z, cur.ix.rw, cur.D, bw, gsz, lane);
+  for (int r = cur.ix.rfirst + rstep; r < meta.y; r += rstep) {
+    const int4 rw = P.sw_row[r];
+    ep_row(X, prod, meta.z, rw, bw ? A[rw.w] : blk_zero<T>(), bw, gsz, lane);
   }
   __syncthreads();
 }
 
+// the chunk loop runs two chunks per trip with the value buffers swapping
+// roles, so no loaded value is copied between registers (a copy would wait
+// for the load)
 template <typename T>
 __device__ __forceinline__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti,
                                              T *X, T *prod, T *rs,
                                              const int4 *swc, const int *swm,
                                              const SolvePlan &P) {
   const int nc = P.sw_nfw + P.sw_nbw;
+  const bool tail = P.tn > 0;
+  if (nc == 0) {
+    if (tail) tail_rows<T>(A, Ti, X, rs, P, false);
+    return;
+  }
   EntIdx ixn;
-  EntVal<T> cur, nxt;
-  if (nc > 0) {
-    ep_index(P, swc, swm, 0, ixn);
-    ep_values(A, ixn, 0 >= P.sw_nfw, cur);
+  EntVal<T> va, vb;
+  const int cl = nc - 1;
+  ep_index(P, swc, swm, 0, ixn);
+  ep_values(A, ixn, 0 >= P.sw_nfw, va);
+  ep_index(P, swc, swm, min(1, cl), ixn);
+  for (int c = 0; c < nc; c += 2) {
+    if (c == P.sw_nfw && tail) tail_rows<T>(A, Ti, X, rs, P, false);
+    ep_chunk(P, swc, swm, A, X, prod, va, c >= P.sw_nfw,
+             min(c + 1, cl) >= P.sw_nfw, min(c + 2, cl), ixn, vb);
+    if (c + 1 >= nc) break;
+    if (c + 1 == P.sw_nfw && tail) tail_rows<T>(A, Ti, X, rs, P, false);
+    ep_chunk(P, swc, swm, A, X, prod, vb, c + 1 >= P.sw_nfw,
+             min(c + 2, cl) >= P.sw_nfw, min(c + 3, cl), ixn, va);
   }
-  if (nc > 1) ep_index(P, swc, swm, 1, ixn);
-  for (int c = 0; c < nc; c++) {
-    if (c == P.sw_nfw && P.tn > 0) tail_rows<T>(A, Ti, X, rs, P, false);
-    ep_chunk(P, swc, swm, A, X, prod, cur, c >= P.sw_nfw, c + 1 < nc,
-             c + 1 >= P.sw_nfw, c + 2 < nc ? c + 2 : -1, ixn, nxt);
-    cur = nxt;
-  }
-  if (P.sw_nbw == 0 && P.tn > 0) tail_rows<T>(A, Ti, X, rs, P, false);
+  if (P.sw_nbw == 0 && tail) tail_rows<T>(A, Ti, X, rs, P, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -1015,6 +1161,54 @@ constexpr int SOLVE_LDS_BYTES = 148 * 1024;
 // scalars) after X in the same LDS block
 __host__ __device__ constexpr int ep_offset(int nb) { return (2 * nb + 3) & ~3; }
 
+// one pass on X: right-hand side in, sweeps, dx out.  Instantiated once with
+// X in LDS and once in global memory, so that the compiler addresses X
+// directly: through a generic pointer every access would be a flat
+// instruction, whose wait also drains the prefetched global loads.
+template <typename T, bool LDS>
+__device__ __forceinline__ void solve_pass(const DevTopo &g, const B4<T> *A,
+                                           const B4<T> *Ti, T *X, T *prod,
+                                           T *rs, const int4 *swc,
+                                           const int *swm, const SolvePlan &P,
+                                           int ep, const double *src,
+                                           int rhs_mode, int accumulate,
+                                           double *dx) {
+  // rhs_mode 0: (P, Q) mismatch; 1: (P, 0) and 2: (0, Q) for the decoupled
+  // half steps; 3: the refinement residual of k_residual.  Vector loads in
+  // groups of VU per thread, all requested before the first is used.
+  constexpr int VU = 8;
+  const int n2 = 2 * g.nb;
+  const int stride = (int)blockDim.x;
+  for (int k0 = threadIdx.x; k0 < n2; k0 += VU * stride) {
+    double v[VU];
+#pragma unroll
+    for (int u = 0; u < VU; u++) v[u] = src[min(k0 + u * stride, n2 - 1)];
+#pragma unroll
+    for (int u = 0; u < VU; u++) {
+      const int k = k0 + u * stride;
+      const bool keep = rhs_mode == 0 || rhs_mode == 3 || (rhs_mode == 1) == ((k & 1) == 0);
+      if (k < n2) X[k] = keep ? (T)v[u] : (T)0;
+    }
+  }
+  __syncthreads();
+  if (LDS && ep) lu_sweeps_ep<T>(A, Ti, X, prod, rs, swc, swm, P);
+  else lu_sweeps<T>(g, A, Ti, X, rs, P);
+  if (accumulate) {
+    for (int k0 = threadIdx.x; k0 < n2; k0 += VU * stride) {
+      double v[VU];
+#pragma unroll
+      for (int u = 0; u < VU; u++) v[u] = dx[min(k0 + u * stride, n2 - 1)];
+#pragma unroll
+      for (int u = 0; u < VU; u++) {
+        const int k = k0 + u * stride;
+        if (k < n2) dx[k] = v[u] + (double)X[k];
+      }
+    }
+  } else {
+    for (int k = threadIdx.x; k < n2; k += stride) dx[k] = (double)X[k];
+  }
+}
+
 template <typename T>
 __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
                            const B4<T> *TL, int rhs_mode, int accumulate,
@@ -1026,27 +1220,21 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
   __shared__ int s_swm[MAX_CHUNKS];
   const int t = blockIdx.x;
   if (d.st[t].done) return;
-  T *X = use_lds ? smem : xglob + (size_t)t * 2 * g.nb;
   const B4<T> *A = LU + (size_t)t * g.nnzLU;
   const B4<T> *Ti = TL + (size_t)t * P.tn * P.tn;
-  const double *F = d.F + (size_t)t * 2 * g.nb;
-  const double *R = d.rhs + (size_t)t * 2 * g.nb;
+  const double *src = (rhs_mode == 3 ? d.rhs : d.F) + (size_t)t * 2 * g.nb;
   if (ep)
     for (int q = threadIdx.x; q < P.sw_nfw + P.sw_nbw; q += blockDim.x) {
       s_swc[q] = P.sw_chunk[q];
       s_swm[q] = P.sw_cmax[q];
     }
   double *dx = d.dx + (size_t)t * 2 * g.nb;
-  // rhs_mode 0: (P, Q) mismatch; 1: (P, 0) and 2: (0, Q) for the decoupled
-  // half steps; 3: the refinement residual of k_residual
-  for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
-    X[k] = rhs_mode == 3 ? (T)R[k]
-           : (rhs_mode == 0 || (rhs_mode == 1) == ((k & 1) == 0)) ? (T)F[k] : (T)0;
-  __syncthreads();
-  if (ep) lu_sweeps_ep<T>(A, Ti, X, smem + ep_offset(g.nb), rs, s_swc, s_swm, P);
-  else lu_sweeps<T>(g, A, Ti, X, rs, P);
-  for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x)
-    dx[k] = (accumulate ? dx[k] : 0.0) + (double)X[k];
+  if (use_lds)
+    solve_pass<T, true>(g, A, Ti, smem, smem + ep_offset(g.nb), rs, s_swc,
+                        s_swm, P, ep, src, rhs_mode, accumulate, dx);
+  else
+    solve_pass<T, false>(g, A, Ti, xglob + (size_t)t * 2 * g.nb, nullptr, rs,
+                         s_swc, s_swm, P, 0, src, rhs_mode, accumulate, dx);
 }
 
 // ---------------------------------------------------------------------------
