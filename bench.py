@@ -62,7 +62,7 @@ def algorithmic_bytes(info, case, T, its_total, s, refine):
 
 
 def run_gpu(case, cont, precision, refine, qlim, warmup, steps, timing=True,
-            tune=True, graph=True):
+            tune=True, graph=True, fixed=None):
     import torch
     from mapol_amd.engine import Engine
     from mapol_amd.tune import autotune
@@ -70,7 +70,11 @@ def run_gpu(case, cont, precision, refine, qlim, warmup, steps, timing=True,
     e = Engine(case, precision=precision, refine_steps=refine,
                enforce_qlim=qlim, use_graph=graph)
     e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
-    launch = autotune(e) if tune else None
+    if fixed:
+        e.set_launch(*fixed)
+        launch = dict(zip(("lu", "solve", "mismatch"), fixed))
+    else:
+        launch = autotune(e) if tune else None
     if launch:
         log(f"[bench] launch shapes: lu {launch['lu']} solve {launch['solve']} "
             f"mismatch {launch['mismatch']}")
@@ -154,6 +158,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--launch", default=None, metavar="LU,SOLVE,MISMATCH",
+                    help="fixed workgroup sizes for the mixed-precision run "
+                         "(no autotuning), e.g. the ones a tuned run chose")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each kernel directly instead of replaying "
                          "the captured Newton-iteration graph")
@@ -183,8 +190,10 @@ def main():
     T = cont.T
     log(f"[bench] workload {args.workload}: {case.counts()} T={T} qlim={qlim}")
 
+    fixed = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
     mix = run_gpu(case, cont, BPF_MIXED, args.refine, qlim, args.warmup,
-                  args.steps, tune=not args.no_tune, graph=not args.no_graph)
+                  args.steps, tune=not args.no_tune, graph=not args.no_graph,
+                  fixed=fixed)
     log(f"[bench] mixed: {mix['seconds'] / args.steps * 1e3:.2f} ms/step")
     fp = None
     if not args.no_fp64:
