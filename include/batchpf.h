@@ -114,6 +114,8 @@ typedef struct {
   int32_t tail_row_entries;        /* non-tail L entries of the tail rows */
   int32_t sweep_fw_chunks, sweep_bw_chunks;  /* entry-parallel sweep chunks */
   int32_t sweep_rows, sweep_entries;
+  int32_t fill_level0_chunks;      /* level-0 chunks finished by the Jacobian
+                                      fill in Newton steps (0: none) */
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */
@@ -226,6 +228,17 @@ bpf_status bpf_analyse(const bpf_case *c, bpf_info *info, int32_t *bus_of_blk,
 bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
                               int32_t *sw_chunk, int32_t *sw_row,
                               int32_t *sw_ent);
+
+/* Host-only: the Jacobian fill of a Newton step, which also finishes level
+ * 0 of the elimination tree (part of the factorization of newtonpf.h:312-363
+ * moved into the fill).  jac_lu[nnz_jac_blocks] = L+U position of every
+ * Jacobian block; jac_l0[nnz_jac_blocks] = -1 (block unchanged), -2 (a
+ * level-0 pivot: stored inverted) or the index of its level-0 pivot's
+ * Jacobian block (an L block: stored times that pivot's inverse).  The
+ * factorization then starts at chunk fill_level0_chunks.  Arrays may be
+ * NULL. */
+bpf_status bpf_analyse_fill(const bpf_case *c, bpf_info *info, int32_t *jac_lu,
+                            int32_t *jac_l0);
 
 /* ---- stage-level entry points (tests, benchmarks) ---------------------- */
 

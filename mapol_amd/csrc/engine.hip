@@ -261,6 +261,7 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(g.yf_item, S.yf_item); UP(g.yt_ptr, S.yt_ptr); UP(g.yt_item, S.yt_item);
   UP(g.blk_of_bus, S.blk_of_bus); UP(g.bus_of_blk, S.bus_of_blk);
   UP(g.y2lu, S.y2lu); UP(g.jac_lu, S.jac_lu); UP(g.jac_y, S.jac_y);
+  UP(g.jac_l0, nonempty(S.jac_l0, 1));
   g.nJb = (int)S.jac_lu.size(); UP(g.lrow, S.lrow); UP(g.lcol, S.lcol);
   UP(g.ldiag, S.ldiag); UP(g.mirror, S.mirror);
   UP(g.br_from, e->br_from); UP(g.br_to, e->br_to);
@@ -433,7 +434,9 @@ extern "C" bpf_status bpf_set_instances(bpf_engine *e, int32_t T,
   AL(e->d_loss, double, T);
   AL(d.part, double, (size_t)T * div_up(e->n, 64));
   AL(d.partv, int, (size_t)T * div_up(e->n, 64));
+  AL(d.jsing, int, T);
 #undef AL
+  HIPCHK(hipMemset(d.jsing, 0, sizeof(int) * (size_t)T));
   return BPF_OK;
 }
 
@@ -529,14 +532,16 @@ static void launch_mismatch(bpf_engine *e, int mode) {
 template <typename T>
 static void launch_newton_step(bpf_engine *e) {
   B4<T> *LU = (B4<T> *)e->d_LU;
+  // the fill also finishes level 0 when the analysis allows it
+  const int c0 = e->S.l0_chunks;
   tbegin(e, "jacobian");
   hipLaunchKernelGGL(k_jacobian<T>, xgrid(e->T, e->g.nJb, 256),
-                     dim3(256), 0, e->stream, e->g, e->d, LU);
+                     dim3(256), 0, e->stream, e->g, e->d, LU, c0 > 0 ? 1 : 0);
   tend(e);
   tbegin(e, "lu_factor");
   hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,
                      e->stream, e->g, e->d.st, 1, e->fplan, LU,
-                     (B4<T> *)e->d_TL);
+                     (B4<T> *)e->d_TL, c0, e->d.jsing);
   tend(e);
   size_t xs = sizeof(T) * 2 * (size_t)e->S.nb;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
@@ -679,7 +684,7 @@ static bpf_status run_fd(bpf_engine *e) {
   tbegin(e, "lu_factor");
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(T), dim3(e->lu_threads), 0,
                      e->stream, e->g, e->d.st, 0, e->fplan, LU,
-                     (B4<double> *)e->d_TL);
+                     (B4<double> *)e->d_TL, 0, e->d.jsing);
   tend(e);
   size_t xs = sizeof(double) * 2 * (size_t)e->S.nb;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
@@ -820,6 +825,7 @@ static void fill_info(const Symbolic &S, bpf_info *info) {
   info->sweep_bw_chunks = S.sw_nbw;
   info->sweep_rows = (int)S.sw_row.size() / 4;
   info->sweep_entries = (int)S.sw_ent.size() / 2;
+  info->fill_level0_chunks = S.l0_chunks;
 }
 
 extern "C" bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
@@ -838,6 +844,24 @@ extern "C" bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
     if (dst && !v.empty()) memcpy(dst, v.data(), sizeof(int32_t) * v.size());
   };
   put(sw_chunk, S.sw_chunk); put(sw_row, S.sw_row); put(sw_ent, S.sw_ent);
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_analyse_fill(const bpf_case *c, bpf_info *info,
+                                       int32_t *jac_lu, int32_t *jac_l0) {
+  if (!c || !info || c->nbus <= 0 || c->nbranch < 0 || !c->bus_type ||
+      (c->nbranch && (!c->br_from || !c->br_to)))
+    return BPF_ERR_INVALID_ARG;
+  Symbolic S;
+  std::string err;
+  if (!analyse(c->nbus, c->nbranch, c->br_from, c->br_to, c->bus_type, S, &err))
+    return BPF_ERR_INVALID_ARG;
+  memset(info, 0, sizeof(*info));
+  fill_info(S, info);
+  auto put = [](int32_t *dst, const std::vector<int> &v) {
+    if (dst && !v.empty()) memcpy(dst, v.data(), sizeof(int32_t) * v.size());
+  };
+  put(jac_lu, S.jac_lu); put(jac_l0, S.jac_l0);
   return BPF_OK;
 }
 
@@ -978,7 +1002,7 @@ extern "C" bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V,
   B4<double> *LU = (B4<double> *)e->d_LU;
   HIPCHK(hipMemset(LU, 0, sizeof(B4<double>) * (size_t)e->T * e->g.nnzLU));
   hipLaunchKernelGGL(k_jacobian<double>, xgrid(e->T, e->g.nJb, 256),
-                     dim3(256), 0, e->stream, e->g, e->d, LU);
+                     dim3(256), 0, e->stream, e->g, e->d, LU, 0);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(e->d.st, st.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
@@ -1088,10 +1112,10 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   const unsigned TT = (unsigned)T;
   tbegin(e, "adjoint");
   hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, e->g.nJb, 256),
-                     dim3(256), 0, e->stream, e->g, e->d, LU);
+                     dim3(256), 0, e->stream, e->g, e->d, LU, 0);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
                      e->stream, e->g, e->d.st, 0, e->fplan, LU,
-                     (B4<double> *)e->d_TL);
+                     (B4<double> *)e->d_TL, 0, e->d.jsing);
   hipLaunchKernelGGL(k_adj_rhs, dim3(div_up(e->S.nb, 256), TT), dim3(256), 0,
                      e->stream, e->g, e->d, (const double *)d_rate, w, d_G);
   size_t xs = sizeof(double) * nb2;

@@ -353,46 +353,6 @@ __global__ void k_verdict(DevTopo g, DevInst d, int nparts, int mode, int maxit,
 }
 
 // ---------------------------------------------------------------------------
-// Jacobian fill into the L+U storage (compuateJacobianMatrix,
-// newtonpf.h:97-186): one thread per (Jacobian block, instance) writes one
-// 2x2 block.  Superset pattern: a PV bus keeps an identity Q row and a zero
-// magnitude column, so PV->PQ switching never changes the pattern.
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *d.active_count = 0;
-  // one thread per Jacobian block, in block-position order: the writes into
-  // the factor slab are nearly contiguous, the reads of Ybus are gathered
-  const XSlot sl = xcd_slot(parts_of(g.nJb));
-  const int t = sl.t, r = sl.part * blockDim.x + threadIdx.x;
-  if (t >= d.T || r >= g.nJb) return;
-  if (d.st[t].done) return;
-  const int q = g.jac_lu[r], p = g.jac_y[r];
-  int ii = g.yrowof[p], jj = g.ycol[p];
-  const cd *V = d.V + (size_t)t * g.n;
-  cd adm = d.Y[(size_t)t * g.nnzY + p];
-  double yr = adm.x, yi = adm.y;
-  double vrj = V[jj].x, vij = V[jj].y, vri = V[ii].x, vii = V[ii].y;
-  double ibr = 0.0, ibi = 0.0;
-  if (ii == jj) { cd I = d.Ib[(size_t)t * g.n + ii]; ibr = I.x; ibi = I.y; }
-  double re = yr * vrj - yi * vij;
-  double im = yr * vij + yi * vrj;
-  double ab = sqrt(vrj * vrj + vij * vij);
-  double ren = yr * vrj / ab - yi * vij / ab;
-  double imn = yr * vij / ab + yi * vrj / ab;
-  double a = -vii * (ibr - re) - vri * (-ibi + im);
-  double b = vri * ren - vii * -imn + ibr * vrj / ab + ibi * vij / ab;
-  double c = -vii * (-ibi + im) + vri * (ibr - re);
-  double e = vri * -imn + vii * ren + ibr * vij / ab + -ibi * vrj / ab;
-  int ti = d.type[(size_t)t * g.n + ii], tj = d.type[(size_t)t * g.n + jj];
-  if (tj == 2) { b = 0.0; e = 0.0; }
-  if (ti == 2) { c = 0.0; e = (ii == jj) ? 1.0 : 0.0; }
-  B4<T> v;
-  v.a = (T)a; v.b = (T)b; v.c = (T)c; v.d = (T)e;
-  LU[(size_t)t * g.nnzLU + q] = v;
-}
-
-// ---------------------------------------------------------------------------
 // 2x2 block helpers
 // ---------------------------------------------------------------------------
 // component-wise select: a select of whole blocks can become a select of
@@ -514,6 +474,69 @@ __device__ __forceinline__ bool blk_inv(const B4<T> &a, B4<T> &inv) {
 }
 
 // ---------------------------------------------------------------------------
+// Jacobian fill into the L+U storage (compuateJacobianMatrix,
+// newtonpf.h:97-186): one thread per (Jacobian block, instance) writes one
+// 2x2 block.  Superset pattern: a PV bus keeps an identity Q row and a zero
+// magnitude column, so PV->PQ switching never changes the pattern.
+// With elim0 the fill also finishes level 0 of the elimination tree (the
+// factorization then starts at its level-1 chunks): a level-0 pivot is
+// stored inverted, an L block of one times the pivot's inverse (the thread
+// forms the pivot's block as well), in the factor's precision and the same
+// operations as the factorization, so the factors are the same either way.
+// A singular level-0 pivot sets jsing[t], which the factorization reports.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ B4<T> jac_block(const DevTopo &g, const DevInst &d,
+                                           int t, int r) {
+  const int p = g.jac_y[r];
+  int ii = g.yrowof[p], jj = g.ycol[p];
+  const cd *V = d.V + (size_t)t * g.n;
+  cd adm = d.Y[(size_t)t * g.nnzY + p];
+  double yr = adm.x, yi = adm.y;
+  double vrj = V[jj].x, vij = V[jj].y, vri = V[ii].x, vii = V[ii].y;
+  double ibr = 0.0, ibi = 0.0;
+  if (ii == jj) { cd I = d.Ib[(size_t)t * g.n + ii]; ibr = I.x; ibi = I.y; }
+  double re = yr * vrj - yi * vij;
+  double im = yr * vij + yi * vrj;
+  double ab = sqrt(vrj * vrj + vij * vij);
+  double ren = yr * vrj / ab - yi * vij / ab;
+  double imn = yr * vij / ab + yi * vrj / ab;
+  double a = -vii * (ibr - re) - vri * (-ibi + im);
+  double b = vri * ren - vii * -imn + ibr * vrj / ab + ibi * vij / ab;
+  double c = -vii * (-ibi + im) + vri * (ibr - re);
+  double e = vri * -imn + vii * ren + ibr * vij / ab + -ibi * vrj / ab;
+  int ti = d.type[(size_t)t * g.n + ii], tj = d.type[(size_t)t * g.n + jj];
+  if (tj == 2) { b = 0.0; e = 0.0; }
+  if (ti == 2) { c = 0.0; e = (ii == jj) ? 1.0 : 0.0; }
+  B4<T> v;
+  v.a = (T)a; v.b = (T)b; v.c = (T)c; v.d = (T)e;
+  return v;
+}
+
+template <typename T>
+__global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.active_count = 0;
+  // one thread per Jacobian block, in block-position order: the writes into
+  // the factor slab are nearly contiguous, the reads of Ybus are gathered
+  const XSlot sl = xcd_slot(parts_of(g.nJb));
+  const int t = sl.t, r = sl.part * blockDim.x + threadIdx.x;
+  if (t >= d.T || r >= g.nJb) return;
+  if (d.st[t].done) return;
+  B4<T> v = jac_block<T>(g, d, t, r);
+  const int role = elim0 ? g.jac_l0[r] : -1;
+  if (role == -2) {
+    B4<T> inv;
+    if (blk_inv(v, inv)) d.jsing[t] = 1;
+    v = inv;
+  } else if (role >= 0) {
+    B4<T> inv;
+    blk_inv(jac_block<T>(g, d, t, role), inv);  // flagged by the pivot's own block
+    v = blk_mul(v, inv);
+  }
+  LU[(size_t)t * g.nnzLU + g.jac_lu[r]] = v;
+}
+
+// ---------------------------------------------------------------------------
 // Batched LU of the 2x2-block Jacobian, static pivot order from the symbolic
 // analysis.  One workgroup per instance.
 // Sparse part: the outputs of the levels below the tail (each pivot's block,
@@ -533,7 +556,8 @@ __device__ __forceinline__ bool blk_inv(const B4<T> &a, B4<T> &inv) {
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
-                            FactorPlan P, B4<T> *LU, B4<T> *TL) {
+                            FactorPlan P, B4<T> *LU, B4<T> *TL, int c_start,
+                            int *jsing) {
   static_assert(TAIL_MAX * TAIL_MAX == CHUNK_PAIRS,
                 "the chunk products and the dense tail share one LDS buffer");
   const int t = blockIdx.x;
@@ -542,8 +566,16 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   // chunk tables in LDS: scalar loads inside a phase would delay its LDS
   // accesses (both wait on the same counter)
   __shared__ int s_ct[MAX_CHUNKS + 1], s_cp[MAX_CHUNKS + 1], s_cm[MAX_CHUNKS];
+  // chunks below c_start (level 0) were finished by the Jacobian fill, which
+  // flags a singular pivot in jsing[t] (taken over and cleared here)
   __shared__ int sing;
-  if (threadIdx.x == 0) sing = 0;
+  if (threadIdx.x == 0) {
+    sing = 0;
+    if (c_start > 0) {
+      sing = jsing[t];
+      jsing[t] = 0;
+    }
+  }
   for (int q = threadIdx.x; q <= P.nchunk; q += blockDim.x) {
     s_ct[q] = P.ch_task[q];
     s_cp[q] = P.ch_pair[q];
@@ -572,7 +604,7 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       if (wave_any(q, nq1)) pidx[j] = P.pairs[min(q, qlast)];
     }
   };
-  if (P.nchunk > 0) prefetch_pairs(0);
+  if (P.nchunk > c_start) prefetch_pairs(c_start);
   // Loads of a slot no lane of the wave needs are skipped by a scalar branch
   // (wave_any), so the number of loads in flight is not known to the
   // compiler and any wait for one of them waits for all (global loads
@@ -582,7 +614,7 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   // operands and descriptors, then the Jacobian values (once the descriptors
   // arrive, in flight while the products form), and the next chunk's pair
   // indices only once the Jacobian values are in registers.
-  for (int c = 0; c < P.nchunk; c++) {
+  for (int c = c_start; c < P.nchunk; c++) {
     __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     const int q0 = nq0, q1 = nq1;
     const int k0 = s_ct[c], k1 = s_ct[c + 1];
@@ -1603,10 +1635,10 @@ __global__ void k_loss(DevTopo g, DevInst d, cd *Sf, cd *St, double *loss) {
 }
 
 // explicit instantiations used by the engine
-template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *);
-template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *);
-template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *);
-template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *);
+template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *, int);
+template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *, int);
+template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *);
+template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *, int, int *);
 template __global__ void k_lu_solve<float>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
 template __global__ void k_lu_solve<double>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
 

@@ -41,6 +41,7 @@ struct DevTopo {
   // Jacobian blocks in L+U position order: position, Ybus position
   int nJb;
   const int *jac_lu, *jac_y;
+  const int *jac_l0;       // level-0 role of each Jacobian block (elim0 fill)
 };
 
 // per-instance arrays on the device
@@ -64,6 +65,7 @@ struct DevInst {
   // partial mismatch norms, one per mismatch workgroup (NaN: a NaN entry)
   double *part;            // [T][n / 64 rounded up]
   int *partv;              // [T][n / 64 rounded up] PV buses beyond their Q limits
+  int *jsing;              // [T] singular level-0 pivot seen by the fill
 };
 
 }  // namespace bpf

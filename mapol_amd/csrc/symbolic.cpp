@@ -375,6 +375,34 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   }
   S.nchunk = (int)S.ch_max.size();
   S.npairs = (int64_t)S.pairs.size() / 2;
+  // ---- level 0 in the Jacobian fill ------------------------------------------
+  // A level-0 pivot has no descendants: its block becomes its inverse, its L
+  // blocks J(r,k) J(k,k)^-1, its U blocks keep their Jacobian values.  Done
+  // by the fill when level 0 lies below the tail and every such block is a
+  // Jacobian block (no fill-in reaches level 0).
+  {
+    std::vector<int> jidx(nnzLU, -1);
+    for (int r = 0; r < (int)S.jac_lu.size(); r++) jidx[S.jac_lu[r]] = r;
+    S.jac_l0.assign(S.jac_lu.size(), -1);
+    bool ok = S.tail_level > 0 && S.lv_chunk.size() > 1 && S.lv_chunk[1] > 0;
+    for (size_t a = 0; ok && a < piv[0].size(); a++) {
+      const int k = piv[0][a];
+      const int rp = jidx[S.ldiag[k]];
+      if (rp < 0) { ok = false; break; }
+      S.jac_l0[rp] = -2;
+      for (int j : st[k]) {
+        const int rl = jidx[pos(j, k)];
+        if (rl < 0) { ok = false; break; }
+        S.jac_l0[rl] = rp;
+      }
+    }
+    if (ok) {
+      S.l0_chunks = S.lv_chunk[1];
+    } else {
+      S.jac_l0.assign(S.jac_lu.size(), -1);
+      S.l0_chunks = 0;
+    }
+  }
   if (S.nchunk > MAX_CHUNKS) {
     if (err) *err = "factorization schedule has " + std::to_string(S.nchunk) +
                     " chunks, more than the " + std::to_string(MAX_CHUNKS) +

@@ -39,6 +39,13 @@ struct Symbolic {
   std::vector<int> y2lu;  // Ybus position -> block position (-1: slack)
   // the Jacobian blocks sorted by block position: position, Ybus position
   std::vector<int> jac_lu, jac_y;
+  // Level 0 of the elimination tree (no descendants, no fill) finished by
+  // the Jacobian fill: per Jacobian block -1 (nothing), -2 (a level-0 pivot:
+  // store its inverse) or the Jacobian-block index of its level-0 pivot (an
+  // L block: times the pivot's inverse).  l0_chunks = the factorization
+  // chunks this replaces (0: the fill leaves level 0 to the factorization).
+  std::vector<int> jac_l0;
+  int l0_chunks = 0;
   int nnz_jac_blocks = 0;
   // Crout factorization schedule (one level = one elimination-tree height)
   int nlev = 0;

@@ -51,15 +51,15 @@ def _blocks(case, a, J, F):
     return A, rhs, rowmap
 
 
-def _crout(a, A):
-    """the kernel's factorization: chunks of the levels below the dense tail
+def _crout(a, A, c_start=0):
+    """the kernel's factorization: chunks c_start.. of the levels below the dense tail
     (each product L*U of a chunk formed from finished blocks, then every
     output = its Jacobian value or init entry minus its products; a pivot
     stores its inverse, an L output is scaled by its pivot's inverse), then
     the tail's Schur complement assembled from its pair lists and inverted by
     block Gauss-Jordan in the kernel's pivot order.  Returns (LU, S^-1)."""
     A = A.copy()
-    for c in range(a["info"]["chunks"]):
+    for c in range(c_start, a["info"]["chunks"]):
         q0, q1 = a["ch_pair"][c], a["ch_pair"][c + 1]
         prod = np.zeros((q1 - q0, 2, 2))
         for q in range(q0, q1):
@@ -96,6 +96,19 @@ def _crout(a, A):
                 else:
                     D[i, j] = D[i, j] - col[i] @ (P @ row[j])
     return A, D
+
+
+def _fill_level0(a, A):
+    """k_jacobian's elim0 fill: a level-0 pivot stored inverted, an L block
+    of one times the pivot's inverse (from the Jacobian values)"""
+    A = A.copy()
+    J = A.copy()
+    for r, (q, role) in enumerate(zip(a["jac_lu"], a["jac_l0"])):
+        if role == -2:
+            A[q] = np.linalg.inv(J[q])
+        elif role >= 0:
+            A[q] = J[q] @ np.linalg.inv(J[a["jac_lu"][role]])
+    return A
 
 
 def _diag(a):
@@ -209,6 +222,38 @@ def test_schedule_replay_gives_the_newton_step(name):
     # the PV magnitude unknowns of the superset system come out exactly zero
     pv_blocks = [r[0] for r in rowmap[:int((case.bus_type == 2).sum())]]
     assert np.all(x[2 * np.array(pv_blocks, int) + 1] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["pf14", "pf118x200", "pf1354x200", "pf9241x200"])
+def test_fill_finishes_level_zero(name):
+    """The Newton-step fill finishes level 0 of the elimination tree: the
+    chunks from fill_level0_chunks on give the factors of the whole
+    schedule, and the roles cover exactly the level-0 pivots and their L
+    columns (U rows keep their Jacobian values)."""
+    case, a, J, F, A, rhs, rowmap = _system(name, seed=3)
+    c0 = a["info"]["fill_level0_chunks"]
+    jl = a["jac_l0"]
+    if a["info"]["tail_level"] == 0:
+        # level 0 belongs to the dense tail: the fill leaves it alone
+        assert c0 == 0 and (jl == -1).all()
+        return
+    assert c0 >= 1
+    assert (jl >= -2).all() and (jl < len(jl)).all()
+    # every L role names a pivot role, in the same column, below the pivot
+    rp, ci = a["lu_rowptr"], a["lu_colind"]
+    row = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    piv = np.flatnonzero(jl == -2)
+    assert (row[a["jac_lu"][piv]] == ci[a["jac_lu"][piv]]).all()
+    for r in np.flatnonzero(jl >= 0):
+        qp, q = a["jac_lu"][jl[r]], a["jac_lu"][r]
+        assert jl[jl[r]] == -2 and ci[q] == ci[qp] and row[q] > row[qp]
+    # the level-0 chunks' outputs are exactly the roles
+    outs = {int(d) & MASK for d in a["tk_a"][:a["ch_task"][c0]][:, 0]}
+    assert outs == set(int(q) for q in a["jac_lu"][jl != -1])
+    LU, Sinv = _crout(a, A)
+    LU0, Sinv0 = _crout(a, _fill_level0(a, A), c_start=c0)
+    np.testing.assert_array_equal(LU0, LU)
+    np.testing.assert_array_equal(Sinv0, Sinv)
 
 
 @pytest.mark.parametrize("name", ["pf118x200", "pf1354x200"])
