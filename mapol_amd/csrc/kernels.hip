@@ -793,7 +793,65 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     __syncthreads();
     constexpr int GJR = 4;  // entries a thread keeps in registers
     B4<T> *Ti = TL + (size_t)t * ss;
-    if (ss <= GJR * (int)blockDim.x) {
+    const int nt = (s + 1) / 2;  // 2x2 tiles per side
+    if (nt * nt <= (int)blockDim.x) {
+      // each thread owns a 2x2 tile of S for the whole elimination: per
+      // pivot it reads two row and two column blocks (shared by its four
+      // entries) and forms P*row once per column
+      const int ti = (int)threadIdx.x / nt, tj = (int)threadIdx.x - ti * nt;
+      const bool own = ti < nt;
+      const int i0 = 2 * ti, j0 = 2 * tj;
+      const bool vi1 = i0 + 1 < s, vj1 = j0 + 1 < s;
+      B4<T> dv[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          const int i = min(i0 + a, s - 1), j = min(j0 + b, s - 1);
+          dv[a][b] = own ? D[i * s + j] : blk_zero<T>();
+        }
+      for (int k = 0; k < s; k++) {
+        const int cur = k & 1, nxt = cur ^ 1;
+        if (own) {
+          const B4<T> Pk = pvb[cur];
+          const B4<T> col[2] = {colb[cur][i0], colb[cur][min(i0 + 1, s - 1)]};
+          const B4<T> pr[2] = {blk_mul(Pk, rowb[cur][j0]),
+                               blk_mul(Pk, rowb[cur][min(j0 + 1, s - 1)])};
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+              if ((a && !vi1) || (b && !vj1)) continue;
+              const int i = i0 + a, j = j0 + b;
+              B4<T> v;
+              if (i == k) {
+                v = (j == k) ? Pk : pr[b];
+              } else if (j == k) {
+                v = blk_zero<T>();
+                blk_fms(v, col[a], Pk);
+              } else {
+                v = dv[a][b];
+                blk_fms(v, col[a], pr[b]);
+              }
+              dv[a][b] = v;
+              if (j == k + 1) colb[nxt][i] = v;
+              if (i == k + 1) rowb[nxt][j] = v;
+              if (i == k + 1 && j == k + 1) {
+                B4<T> iv;
+                if (blk_inv(v, iv)) sing = 1;
+                pvb[nxt] = iv;
+              }
+            }
+        }
+        __syncthreads();
+      }
+      if (own)
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++)
+            if ((!a || vi1) && (!b || vj1)) Ti[(i0 + a) * s + j0 + b] = dv[a][b];
+    } else if (ss <= GJR * (int)blockDim.x) {
       // each thread owns up to GJR entries of S for the whole elimination:
       // only the pivot row, column and inverse pass through LDS
       B4<T> dv[GJR];
