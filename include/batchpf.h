@@ -116,6 +116,8 @@ typedef struct {
   int32_t sweep_rows, sweep_entries;
   int32_t fill_level0_chunks;      /* level-0 chunks finished by the Jacobian
                                       fill in Newton steps (0: none) */
+  int32_t tail_stage;              /* distinct blocks of the tail products
+                                      staged in LDS (0: not staged) */
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */
@@ -239,6 +241,16 @@ bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
  * NULL. */
 bpf_status bpf_analyse_fill(const bpf_case *c, bpf_info *info, int32_t *jac_lu,
                             int32_t *jac_l0);
+
+/* Host-only: the staging of the dense tail's Schur-complement products (the
+ * FP32 factorization copies their distinct L and U blocks into LDS once).
+ * tl_stage[tail_stage] = L+U positions of those blocks, ascending;
+ * tl_pairS[tail_pairs] = for each pair of tl_pairL/tl_pairU its two indices
+ * into tl_stage, L | U << 16.  tail_stage = 0 when the blocks exceed the LDS
+ * capacity (5760): then the products read the factor directly.  Arrays may
+ * be NULL. */
+bpf_status bpf_analyse_tail(const bpf_case *c, bpf_info *info, int32_t *tl_stage,
+                            int32_t *tl_pairS);
 
 /* ---- stage-level entry points (tests, benchmarks) ---------------------- */
 

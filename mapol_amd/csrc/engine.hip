@@ -285,6 +285,8 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   F.tmax = S.tl_max;
   UP(F.tdense, S.tl_dense); UP(F.tpos, S.tl_pos); UP(F.tpp, S.tl_pp);
   UP(F.tpairL, S.tl_pairL); UP(F.tpairU, S.tl_pairU);
+  F.tstage_n = (int)S.tl_stage.size();
+  UP(F.tstage, nonempty(S.tl_stage, 1)); UP(F.tpairS, nonempty(S.tl_pairS, 1));
   SolvePlan &SP = e->splan;
   SP.nfw = S.nfw;
   SP.nbw = S.nbw;
@@ -826,6 +828,7 @@ static void fill_info(const Symbolic &S, bpf_info *info) {
   info->sweep_rows = (int)S.sw_row.size() / 4;
   info->sweep_entries = (int)S.sw_ent.size() / 2;
   info->fill_level0_chunks = S.l0_chunks;
+  info->tail_stage = (int)S.tl_stage.size();
 }
 
 extern "C" bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
@@ -862,6 +865,24 @@ extern "C" bpf_status bpf_analyse_fill(const bpf_case *c, bpf_info *info,
     if (dst && !v.empty()) memcpy(dst, v.data(), sizeof(int32_t) * v.size());
   };
   put(jac_lu, S.jac_lu); put(jac_l0, S.jac_l0);
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_analyse_tail(const bpf_case *c, bpf_info *info,
+                                       int32_t *tl_stage, int32_t *tl_pairS) {
+  if (!c || !info || c->nbus <= 0 || c->nbranch < 0 || !c->bus_type ||
+      (c->nbranch && (!c->br_from || !c->br_to)))
+    return BPF_ERR_INVALID_ARG;
+  Symbolic S;
+  std::string err;
+  if (!analyse(c->nbus, c->nbranch, c->br_from, c->br_to, c->bus_type, S, &err))
+    return BPF_ERR_INVALID_ARG;
+  memset(info, 0, sizeof(*info));
+  fill_info(S, info);
+  auto put = [](int32_t *dst, const std::vector<int> &v) {
+    if (dst && !v.empty()) memcpy(dst, v.data(), sizeof(int32_t) * v.size());
+  };
+  put(tl_stage, S.tl_stage); put(tl_pairS, S.tl_pairS);
   return BPF_OK;
 }
 

@@ -399,6 +399,8 @@ struct FactorPlan {
   const int2 *pairs;                   // {L, U} positions (U < 0: init entry)
   int tn, tnent, tmax;                 // dense tail: pivots, entries, longest list
   const int *tdense, *tpos, *tpp, *tpairL, *tpairU;
+  int tstage_n;                        // staged tail blocks (0: no staging)
+  const int *tstage, *tpairS;          // their positions; pair -> L | U << 16
 };
 
 struct SolvePlan {
@@ -563,9 +565,16 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   const int t = blockIdx.x;
   if (st[t].done) return;
   __shared__ B4<T> buf[CHUNK_PAIRS];
-  // chunk tables in LDS: scalar loads inside a phase would delay its LDS
-  // accesses (both wait on the same counter)
-  __shared__ int s_ct[MAX_CHUNKS + 1], s_cp[MAX_CHUNKS + 1], s_cm[MAX_CHUNKS];
+  // chunk tables in LDS (scalar loads inside a phase would delay its LDS
+  // accesses: both wait on the same counter); in FP32 the same space then
+  // holds the staged blocks of the dense tail's products
+  constexpr int STAGE = sizeof(T) == 4 ? TAIL_STAGE_MAX : 1;
+  union TabStage {
+    struct { int ct[MAX_CHUNKS + 1], cp[MAX_CHUNKS + 1], cm[MAX_CHUNKS]; } tab;
+    B4<T> stage[STAGE];
+  };
+  __shared__ TabStage sh;
+  int *const s_ct = sh.tab.ct, *const s_cp = sh.tab.cp, *const s_cm = sh.tab.cm;
   // chunks below c_start (level 0) were finished by the Jacobian fill, which
   // flags a singular pivot in jsing[t] (taken over and cleared here)
   __shared__ int sing;
@@ -746,7 +755,46 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     const int s = P.tn, ss = s * s;
     for (int q = threadIdx.x; q < ss; q += blockDim.x) D[q] = blk_zero<T>();
     __syncthreads();
-    {
+    if (sizeof(T) == 4 && P.tstage_n > 0) {
+      // FP32: the distinct L_SN and U_NS blocks are copied into LDS once
+      // (each serves several products), then every entry forms its products
+      // from LDS in the order of its pair list (the sums of the plain loop)
+      constexpr int SU = 6;
+      const int ns = P.tstage_n, bd = (int)blockDim.x;
+      for (int q0 = threadIdx.x; q0 < ns; q0 += SU * bd) {
+        int sp[SU];
+#pragma unroll
+        for (int u = 0; u < SU; u++) sp[u] = P.tstage[min(q0 + u * bd, ns - 1)];
+        B4<T> sv[SU];
+#pragma unroll
+        for (int u = 0; u < SU; u++) sv[u] = A[sp[u]];
+#pragma unroll
+        for (int u = 0; u < SU; u++)
+          if (q0 + u * bd < ns) sh.stage[q0 + u * bd] = sv[u];
+      }
+      __syncthreads();
+      const int gsz = lane_group(P.tnent, P.tmax);
+      const int lane = threadIdx.x & (gsz - 1);
+      constexpr int TU = 8;
+      for (int k = (int)threadIdx.x / gsz; k < P.tnent; k += (int)blockDim.x / gsz) {
+        const int dd = P.tdense[k];
+        const int pos = P.tpos[k];
+        const int qa = P.tpp[k], q1 = P.tpp[k + 1];
+        const B4<T> a0 = A[(dd & TASK_HAS_A) ? pos : 0];
+        B4<T> acc = blk_sel(lane == 0 && (dd & TASK_HAS_A), a0, blk_zero<T>());
+        for (int q = qa + lane; q < q1; q += TU * gsz) {
+          int ps[TU];
+#pragma unroll
+          for (int u = 0; u < TU; u++) ps[u] = P.tpairS[min(q + u * gsz, q1 - 1)];
+#pragma unroll
+          for (int u = 0; u < TU; u++)
+            blk_fms(acc, blk_sel(q + u * gsz < q1, sh.stage[ps[u] & 0xFFFF], blk_zero<T>()),
+                    sh.stage[ps[u] >> 16]);
+        }
+        group_sum4(acc, gsz);
+        if (lane == 0) D[dd & TASK_POS_MASK] = acc;
+      }
+    } else {
       const int gsz = lane_group(P.tnent, P.tmax);
       const int lane = threadIdx.x & (gsz - 1);
       constexpr int TU = 4;  // products of one lane requested together

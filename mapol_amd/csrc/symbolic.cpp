@@ -1,6 +1,7 @@
 // Symbolic analysis: Ybus pattern, bus ordering, L+U block pattern and the
 // level schedules of the batched factorization and triangular solves.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <numeric>
 #include <string>
@@ -242,8 +243,14 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     std::vector<int> cnt(S.nlev + 1, 0);  // pivots at levels >= l
     for (int k = 0; k < nb; k++) cnt[lev[k]]++;
     for (int l = S.nlev - 1; l >= 0; l--) cnt[l] += cnt[l + 1];
+    // BPF_TAIL_LIMIT (1..TAIL_MAX) caps the tail below its LDS capacity
+    int limit = TAIL_MAX;
+    if (const char *ev = getenv("BPF_TAIL_LIMIT")) {
+      const int v = atoi(ev);
+      if (v >= 1 && v <= TAIL_MAX) limit = v;
+    }
     int Ls = S.nlev;
-    while (Ls > 0 && cnt[Ls - 1] <= TAIL_MAX) Ls--;
+    while (Ls > 0 && cnt[Ls - 1] <= limit) Ls--;
     // worth it only when it removes a few levels
     if (S.nlev - Ls < 4) Ls = S.nlev;
     S.tail_level = Ls;
@@ -446,6 +453,21 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   }
   for (int q = 0; q + 1 < (int)S.tl_pp.size(); q++)
     S.tl_max = std::max(S.tl_max, S.tl_pp[q + 1] - S.tl_pp[q]);
+  {
+    std::vector<int> blocks(S.tl_pairL);
+    blocks.insert(blocks.end(), S.tl_pairU.begin(), S.tl_pairU.end());
+    std::sort(blocks.begin(), blocks.end());
+    blocks.erase(std::unique(blocks.begin(), blocks.end()), blocks.end());
+    if (!blocks.empty() && (int)blocks.size() <= TAIL_STAGE_MAX) {
+      S.tl_stage = blocks;
+      auto idx = [&](int p) {
+        return (int)(std::lower_bound(blocks.begin(), blocks.end(), p) - blocks.begin());
+      };
+      S.tl_pairS.resize(S.tl_pairL.size());
+      for (size_t q = 0; q < S.tl_pairL.size(); q++)
+        S.tl_pairS[q] = idx(S.tl_pairL[q]) | (idx(S.tl_pairU[q]) << 16);
+    }
+  }
   // ---- tail rows in the sweeps ----------------------------------------------
   std::vector<char> intail(nb, 0);
   for (int k : S.tail_piv) intail[k] = 1;

@@ -20,6 +20,9 @@ constexpr int CHUNK_PAIRS = 4096;    // products per factorization chunk (LDS)
 constexpr int SWEEP_CHUNK = 2048;    // entries per sweep chunk (LDS)
 constexpr int MAX_CHUNKS = 512;      // chunk tables the kernels keep in LDS
 constexpr int TAIL_MAX = 64;         // dense tail: at most 64 blocks square
+// FP32 factorization: distinct blocks of the tail products staged in LDS
+// (with the chunk tables, which the tail no longer needs, in the same space)
+constexpr int TAIL_STAGE_MAX = 5760;
 
 struct Symbolic {
   int n = 0, m = 0, nb = 0;
@@ -72,6 +75,10 @@ struct Symbolic {
   std::vector<int> tl_dense, tl_pos, tl_pp;  // per tail entry (flags in tl_dense)
   std::vector<int> tl_pairL, tl_pairU;
   int tl_max = 0;                            // longest tail pair list
+  // the distinct L and U blocks of the tail pairs (positions, ascending) and
+  // each pair's two indices into that list (L | U << 16); empty when more
+  // than TAIL_STAGE_MAX blocks
+  std::vector<int> tl_stage, tl_pairS;
   // The factorization leaves the inverse of the tail's Schur complement
   // (dense, tail_n x tail_n blocks); the sweeps solve the non-tail rows by
   // levels and the tail rows with that inverse.  Tail row a (block

@@ -29,7 +29,8 @@ EXPORTS = [
     "bpf_last_error", "bpf_get_info", "bpf_get_ybus", "bpf_stage_mismatch",
     "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_adjoint",
     "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
-    "bpf_analyse", "bpf_analyse_sweeps", "bpf_analyse_fill", "bpf_load_matpower",
+    "bpf_analyse", "bpf_analyse_sweeps", "bpf_analyse_fill", "bpf_analyse_tail",
+    "bpf_load_matpower",
     "bpf_case_error", "bpf_free_case",
 ]
 
@@ -70,7 +71,7 @@ class BpfInfo(C.Structure):
                 ("tail_pairs", C.c_int32), ("tail_row_entries", C.c_int32),
                 ("sweep_fw_chunks", C.c_int32), ("sweep_bw_chunks", C.c_int32),
                 ("sweep_rows", C.c_int32), ("sweep_entries", C.c_int32),
-                ("fill_level0_chunks", C.c_int32)]
+                ("fill_level0_chunks", C.c_int32), ("tail_stage", C.c_int32)]
 
 
 _LIB = None
@@ -116,6 +117,7 @@ def load_library(path: str = LIB_PATH):
     L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 21
     L.bpf_analyse_sweeps.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 3
     L.bpf_analyse_fill.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
+    L.bpf_analyse_tail.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
     L.bpf_load_matpower.argtypes = [C.c_char_p, C.c_int32, C.POINTER(BpfCase),
                                     C.POINTER(_pd), C.POINTER(C.c_void_p)]
     L.bpf_case_error.argtypes = [C.c_void_p]
@@ -260,6 +262,13 @@ def analyse(case):
     if st != 0:
         raise EngineError(f"bpf_analyse_fill failed ({st})")
     out.update(fl)
+    tl = dict(tl_stage=np.zeros(i["tail_stage"], np.int32),
+              tl_pairS=np.zeros(i["tail_pairs"] if i["tail_stage"] else 0, np.int32))
+    st = L.bpf_analyse_tail(C.byref(s), C.byref(info), _ptr(tl["tl_stage"], _pi),
+                            _ptr(tl["tl_pairS"], _pi))
+    if st != 0:
+        raise EngineError(f"bpf_analyse_tail failed ({st})")
+    out.update(tl)
     return out
 
 

@@ -256,6 +256,23 @@ def test_fill_finishes_level_zero(name):
     np.testing.assert_array_equal(Sinv0, Sinv)
 
 
+@pytest.mark.parametrize("name", ["pf118x200", "pf1354x200", "pf9241x200"])
+def test_tail_staging_reproduces_the_pair_lists(name):
+    """The FP32 tail assembly reads its products from blocks staged in LDS:
+    the staged positions are the distinct blocks of the pair lists, and each
+    pair's packed indices name exactly its L and U blocks."""
+    case, _ = synth.workload(name)
+    a = engine.analyse(case)
+    ns = a["info"]["tail_stage"]
+    L, U = a["tl_pairL"], a["tl_pairU"]
+    assert 0 < ns <= 5760
+    stg = a["tl_stage"]
+    np.testing.assert_array_equal(stg, np.unique(np.concatenate([L, U])))
+    ps = a["tl_pairS"]
+    np.testing.assert_array_equal(stg[ps & 0xFFFF], L)
+    np.testing.assert_array_equal(stg[ps >> 16], U)
+
+
 @pytest.mark.parametrize("name", ["pf118x200", "pf1354x200"])
 def test_transposed_replay_solves_the_adjoint_system(name):
     """k_adj_solve's schedule: (superset block matrix)^T lambda = g against a
