@@ -257,10 +257,28 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
 #define UP(dst, vec) \
   if ((s = upload(e, e->owned, vec, &dst))) return s;
   UP(g.yrow, S.yrow); UP(g.ycol, S.ycol); UP(g.yrowof, S.yrowof);
+  {
+    std::vector<int> yblk(S.ycol.size());
+    for (size_t p = 0; p < yblk.size(); p++) yblk[p] = S.blk_of_bus[S.ycol[p]];
+    UP(g.yblk, nonempty(yblk, 1));
+  }
   UP(g.ydiag_bus, S.ydiag_bus); UP(g.yf_ptr, S.yf_ptr);
   UP(g.yf_item, S.yf_item); UP(g.yt_ptr, S.yt_ptr); UP(g.yt_item, S.yt_item);
   UP(g.blk_of_bus, S.blk_of_bus); UP(g.bus_of_blk, S.bus_of_blk);
   UP(g.y2lu, S.y2lu); UP(g.jac_lu, S.jac_lu); UP(g.jac_y, S.jac_y);
+  {
+    std::vector<int> j4(4 * S.jac_lu.size());
+    for (size_t r = 0; r < S.jac_lu.size(); r++) {
+      const int p = S.jac_y[r];
+      j4[4 * r] = S.jac_lu[r];
+      j4[4 * r + 1] = p;
+      j4[4 * r + 2] = S.yrowof[p];
+      j4[4 * r + 3] = S.ycol[p];
+    }
+    const int *jp;
+    UP(jp, nonempty(j4, 4));
+    g.jac4 = (const int4 *)jp;
+  }
   UP(g.jac_l0, nonempty(S.jac_l0, 1));
   g.nJb = (int)S.jac_lu.size(); UP(g.lrow, S.lrow); UP(g.lcol, S.lcol);
   UP(g.ldiag, S.ldiag); UP(g.mirror, S.mirror);

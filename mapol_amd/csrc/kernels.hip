@@ -220,8 +220,29 @@ __device__ void mis_bus(const DevTopo &g, const DevInst &d, int t, int i,
   const cd *Y = d.Y + (size_t)t * g.nnzY;
   const cd *V = d.V + (size_t)t * g.n;
   cd I = mk(0.0, 0.0);
-  for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++)
-    I = cadd(I, cmul(Y[p], V[g.ycol[p]]));
+  // the row in groups of RU entries: all index and admittance loads, then
+  // all voltage loads, then the sum in row order (a slot past the row end
+  // repeats the last entry and leaves the sum unchanged)
+  constexpr int RU = 4;
+  const int pa = g.yrow[i], pe = g.yrow[i + 1];
+  for (int p0 = pa; p0 < pe; p0 += RU) {
+    int col[RU];
+    cd y[RU], v[RU];
+#pragma unroll
+    for (int u = 0; u < RU; u++) {
+      const int p = min(p0 + u, pe - 1);
+      col[u] = g.ycol[p];
+      y[u] = Y[p];
+    }
+#pragma unroll
+    for (int u = 0; u < RU; u++) v[u] = V[col[u]];
+#pragma unroll
+    for (int u = 0; u < RU; u++) {
+      const cd s = cadd(I, cmul(y[u], v[u]));
+      I.x = p0 + u < pe ? s.x : I.x;
+      I.y = p0 + u < pe ? s.y : I.y;
+    }
+  }
   size_t o = (size_t)t * g.n + i;
   d.Ib[o] = I;
   const int k = g.blk_of_bus[i];
@@ -489,9 +510,8 @@ __device__ __forceinline__ bool blk_inv(const B4<T> &a, B4<T> &inv) {
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ B4<T> jac_block(const DevTopo &g, const DevInst &d,
-                                           int t, int r) {
-  const int p = g.jac_y[r];
-  int ii = g.yrowof[p], jj = g.ycol[p];
+                                           int t, int4 jb) {
+  const int p = jb.y, ii = jb.z, jj = jb.w;
   const cd *V = d.V + (size_t)t * g.n;
   cd adm = d.Y[(size_t)t * g.nnzY + p];
   double yr = adm.x, yi = adm.y;
@@ -524,7 +544,8 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
   const int t = sl.t, r = sl.part * blockDim.x + threadIdx.x;
   if (t >= d.T || r >= g.nJb) return;
   if (d.st[t].done) return;
-  B4<T> v = jac_block<T>(g, d, t, r);
+  const int4 jb = g.jac4[r];
+  B4<T> v = jac_block<T>(g, d, t, jb);
   const int role = elim0 ? g.jac_l0[r] : -1;
   if (role == -2) {
     B4<T> inv;
@@ -532,10 +553,10 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
     v = inv;
   } else if (role >= 0) {
     B4<T> inv;
-    blk_inv(jac_block<T>(g, d, t, role), inv);  // flagged by the pivot's own block
+    blk_inv(jac_block<T>(g, d, t, g.jac4[role]), inv);  // flagged by the pivot's own block
     v = blk_mul(v, inv);
   }
-  LU[(size_t)t * g.nnzLU + g.jac_lu[r]] = v;
+  LU[(size_t)t * g.nnzLU + jb.x] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1394,13 +1415,37 @@ __global__ void k_residual(DevTopo g, DevInst d) {
   const cd *Y = d.Y + (size_t)t * g.nnzY;
   const cd *V = d.V + (size_t)t * g.n;
   cd acc = mk(0.0, 0.0);
-  for (int p = g.yrow[i]; p < g.yrow[i + 1]; p++) {
-    const int j = g.ycol[p];
-    const int bj = g.blk_of_bus[j];
-    if (bj < 0) continue;
-    const cd vj = V[j];
-    const double m = dx[2 * bj + 1] / cabs_(vj);
-    acc = cadd(acc, cmul(Y[p], cmul(vj, mk(m, dx[2 * bj]))));
+  // the row in groups of RU entries (loads of a level all requested
+  // together; terms added in row order, slack columns and slots past the
+  // row end leave the sum unchanged)
+  constexpr int RU = 4;
+  const int pa = g.yrow[i], pe = g.yrow[i + 1];
+  for (int p0 = pa; p0 < pe; p0 += RU) {
+    int col[RU], bj[RU];
+    cd y[RU], vj[RU];
+    double d0[RU], d1[RU];
+#pragma unroll
+    for (int u = 0; u < RU; u++) {
+      const int p = min(p0 + u, pe - 1);
+      col[u] = g.ycol[p];
+      bj[u] = g.yblk[p];
+      y[u] = Y[p];
+    }
+#pragma unroll
+    for (int u = 0; u < RU; u++) {
+      const int b = max(bj[u], 0);
+      vj[u] = V[col[u]];
+      d0[u] = dx[2 * b];
+      d1[u] = dx[2 * b + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < RU; u++) {
+      const double m = d1[u] / cabs_(vj[u]);
+      const cd s = cadd(acc, cmul(y[u], cmul(vj[u], mk(m, d0[u]))));
+      const bool use = p0 + u < pe && bj[u] >= 0;
+      acc.x = use ? s.x : acc.x;
+      acc.y = use ? s.y : acc.y;
+    }
   }
   const size_t o = (size_t)t * g.n + i;
   const cd vi = V[i];

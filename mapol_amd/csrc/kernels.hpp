@@ -32,6 +32,7 @@ struct InstState {
 struct DevTopo {
   int n, m, nb, nnzY, nnzLU;
   const int *yrow, *ycol, *yrowof, *ydiag_bus;
+  const int *yblk;         // per Ybus entry: block of its column bus (-1: slack)
   const int *yf_ptr, *yf_item, *yt_ptr, *yt_item;
   const int *blk_of_bus, *bus_of_blk, *y2lu;
   const int *lrow, *lcol, *ldiag, *mirror;
@@ -41,6 +42,7 @@ struct DevTopo {
   // Jacobian blocks in L+U position order: position, Ybus position
   int nJb;
   const int *jac_lu, *jac_y;
+  const int4 *jac4;        // per Jacobian block {L+U position, Ybus entry, row bus, column bus}
   const int *jac_l0;       // level-0 role of each Jacobian block (elim0 fill)
 };
 
