@@ -157,6 +157,18 @@ bpf_status bpf_get_results(bpf_engine *e, double *V, int32_t *iterations,
  * the from/to ends of the loss kernel's Yf*V, Yt*V products (runpf.h:543-572). */
 bpf_status bpf_get_branch_flows(bpf_engine *e, double *Sf, double *St);
 
+/* Branch loading and per-contingency violation report at the last solution
+ * (no function of the reference: derived from the flows of runpf.h:543-572
+ * for ranking contingencies).  rate[nbranch] = branch ratings in pu (MVA /
+ * baseMVA, e.g. the rate_a of bpf_load_matpower; 0 = no rating).  Outputs,
+ * each may be NULL: loading[T*nbranch] = 100 max(|Sf|, |St|) / rate (0 where
+ * rate is 0 or the branch is out of service), overloads[T] = branches above
+ * 100 %, max_loading[T] = the largest loading.  Non-converged instances are
+ * reported at their last iterate (their verdict says so). */
+bpf_status bpf_get_branch_loading(bpf_engine *e, const double *rate,
+                                  double *loading, int32_t *overloads,
+                                  double *max_loading);
+
 /* hybrid_free (runpf.h:984-1078), mkl_clean (runpf.h:452). */
 void bpf_destroy(bpf_engine *e);
 

@@ -823,6 +823,49 @@ extern "C" bpf_status bpf_get_branch_flows(bpf_engine *e, double *Sf, double *St
   return BPF_OK;
 }
 
+extern "C" bpf_status bpf_get_branch_loading(bpf_engine *e, const double *rate,
+                                             double *loading, int32_t *overloads,
+                                             double *max_loading) {
+  if (!e || !rate) return BPF_ERR_INVALID_ARG;
+  if (!e->solved) { e->err = "no solution yet"; return BPF_ERR_STATE; }
+  const int T = e->T, m = e->m;
+  for (int b = 0; b < m; b++)
+    if (!(rate[b] >= 0.0)) {
+      e->err = "branch ratings must be >= 0 (0: no rating)";
+      return BPF_ERR_INVALID_ARG;
+    }
+  std::vector<DBuf> tmp;
+  struct Free { std::vector<DBuf> &v; ~Free() { free_pool(v); } } fr{tmp};
+  void *p;
+  bpf_status s;
+  double *d_rate, *d_load = nullptr, *d_max;
+  int *d_over;
+  if ((s = dalloc(e, tmp, sizeof(double) * (size_t)std::max(m, 1), &p))) return s;
+  d_rate = (double *)p;
+  if (loading) {
+    if ((s = dalloc(e, tmp, sizeof(double) * (size_t)T * std::max(m, 1), &p))) return s;
+    d_load = (double *)p;
+  }
+  if ((s = dalloc(e, tmp, sizeof(int) * (size_t)T, &p))) return s;
+  d_over = (int *)p;
+  if ((s = dalloc(e, tmp, sizeof(double) * (size_t)T, &p))) return s;
+  d_max = (double *)p;
+  if (m) HIPCHK(hipMemcpy(d_rate, rate, sizeof(double) * m, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_branch_report, dim3(T), dim3(256), 0, e->stream, e->g, T,
+                     e->d_Sf, e->d_St, d_rate, d_load, d_over, d_max);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  if (loading)
+    HIPCHK(hipMemcpy(loading, d_load, sizeof(double) * (size_t)T * m,
+                     hipMemcpyDeviceToHost));
+  std::vector<int> ov(T);
+  HIPCHK(hipMemcpy(ov.data(), d_over, sizeof(int) * T, hipMemcpyDeviceToHost));
+  if (overloads) for (int t = 0; t < T; t++) overloads[t] = ov[t];
+  if (max_loading)
+    HIPCHK(hipMemcpy(max_loading, d_max, sizeof(double) * T, hipMemcpyDeviceToHost));
+  return BPF_OK;
+}
+
 static void fill_info(const Symbolic &S, bpf_info *info) {
   info->nbus = S.n;
   info->nbranch = S.m;

@@ -1785,6 +1785,38 @@ __global__ void k_loss(DevTopo g, DevInst d, cd *Sf, cd *St, double *loss) {
     loss[t] = (d.st[t].verdict == BPF_V_CONVERGED) ? tot : DBL_MAX;
 }
 
+// ---------------------------------------------------------------------------
+// Branch loading report from the flows of k_loss: per branch
+// 100 max(|Sf|, |St|) / rating (0 for a branch without a rating or out of
+// service, whose flows are zero), per instance the number of branches above
+// 100 % and the largest loading.  One workgroup per instance.
+// ---------------------------------------------------------------------------
+__global__ void k_branch_report(DevTopo g, int T, const cd *Sf, const cd *St,
+                                const double *rate, double *loading, int *over,
+                                double *maxl) {
+  __shared__ double shd[16];
+  __shared__ int shi[16];
+  const int t = blockIdx.x;
+  if (t >= T) return;
+  double lmax = 0.0;
+  int cnt = 0;
+  for (int b = threadIdx.x; b < g.m; b += blockDim.x) {
+    const size_t o = (size_t)t * g.m + b;
+    const double r = rate[b];
+    const double s = fmax(cabs_(Sf[o]), cabs_(St[o]));
+    const double l = r > 0.0 ? 100.0 * s / r : 0.0;
+    if (loading) loading[o] = l;
+    lmax = fmax(lmax, l);
+    cnt += l > 100.0 ? 1 : 0;
+  }
+  const double bm = block_max(lmax, shd);
+  const int bc = block_add(cnt, shi);
+  if (threadIdx.x == 0) {
+    over[t] = bc;
+    maxl[t] = bm;
+  }
+}
+
 // explicit instantiations used by the engine
 template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *, int);
 template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *, int);
