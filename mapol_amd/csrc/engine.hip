@@ -365,6 +365,10 @@ extern "C" bpf_status bpf_set_instances(bpf_engine *e, int32_t T,
   }
   if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
   free_pool(e->inst_owned);
+  // the factor storage lived in that pool too: forget it, so that ensure_lu
+  // allocates it for the new T instead of freeing the stale address again
+  // (which a new buffer of this call may have been given)
+  e->d_LU = e->d_xglob = e->d_TL = nullptr;
   e->T = T;
   e->solved = false;
   e->lu_fp64 = -1;

@@ -192,3 +192,25 @@ def test_fdpf_matches_golden(golden, alg, tag):
     ok = r["verdict"] == CONVERGED
     assert np.abs(r["V"][ok] - golden[f"pf118x200/{tag}/V"][ok]).max() <= DV_TOL
     e.close()
+
+
+def test_instances_can_be_set_again_with_another_count():
+    """A handle re-used with a different number of instances (as the C++
+    mirror's hybrid_runpf does on every call) gives what a fresh handle
+    gives: the factor storage of the previous count must not survive."""
+    case, cont = synth.workload("pf118x200")
+    e = Engine(case, precision=BPF_MIXED, refine_steps=1)
+    for T in (1, 16, 4, 16):
+        print(f"[re-set instances] T={T}", flush=True)
+        e.set_instances(br_on=cont.br_on[:T], gen_on=cont.gen_on[:T])
+        e.solve()
+        r = e.results()
+        f = Engine(case, precision=BPF_MIXED, refine_steps=1)
+        f.set_instances(br_on=cont.br_on[:T], gen_on=cont.gen_on[:T])
+        f.solve()
+        rf = f.results()
+        f.close()
+        for k in ("iterations", "verdict", "loss"):
+            np.testing.assert_array_equal(r[k], rf[k], err_msg=f"T={T} {k}")
+        np.testing.assert_array_equal(r["V"], rf["V"])
+    e.close()
