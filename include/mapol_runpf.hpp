@@ -216,7 +216,10 @@ inline void init(State &S, Topology &t, int nTest,
     S.ctl.push_back(bpf_ctl{kind, (int32_t)e.id});
   }
 }
-inline void run(State &S, std::vector<Particula> &swarm) {
+// losses of the swarm's control vectors (any particle type with operator[]
+// over its controls), DBL_MAX where the power flow does not converge
+template <class P>
+inline std::vector<double> losses(State &S, const std::vector<P> &swarm) {
   if (!S.eng) throw Error("runpf before init");
   const int T = (int)swarm.size(), nc = (int)S.ctl.size();
   std::vector<double> vals((size_t)T * nc);
@@ -231,7 +234,7 @@ inline void run(State &S, std::vector<Particula> &swarm) {
   check(S.eng, bpf_get_results(S.eng, nullptr, nullptr, nullptr, nullptr,
                                loss.data(), nullptr, nullptr),
         "bpf_get_results");
-  for (int t = 0; t < T; t++) swarm[t].mudarFitness(loss[t]);
+  return loss;
 }
 }  // namespace detail
 
@@ -241,10 +244,14 @@ inline void hybrid_init(Topology &topology, int nTest, int /*nThreads*/,
   detail::init(detail::state(false), topology, nTest, estrutura, algPF);
 }
 
-// runpf.h:606 -- all particles in one batched solve
+// runpf.h:606 -- all particles in one batched solve; each particle's
+// mudarFitness receives its loss (runpf.h:782-790).  P: Particula or any
+// particle type with operator[] and mudarFitness (the swarm of mapol_pso.hpp)
+template <class P>
 inline void hybrid_runpf(std::vector<Estrutura> & /*estrutura*/,
-                         std::vector<Particula> &enxame) {
-  detail::run(detail::state(false), enxame);
+                         std::vector<P> &enxame) {
+  const std::vector<double> loss = detail::losses(detail::state(false), enxame);
+  for (size_t t = 0; t < enxame.size(); t++) enxame[t].mudarFitness(loss[t]);
 }
 
 // runpf.h:984
@@ -259,12 +266,11 @@ inline void mkl_init(Topology &topology, int nTest,
                      std::vector<Estrutura> estrutura, int algPF) {
   detail::init(detail::state(true), topology, nTest, estrutura, algPF);
 }
+// returns the loss; the caller hands it to mudarFitness (main.cu:300-306)
+template <class P>
 inline double mkl_runpf(std::vector<Estrutura> & /*estrutura*/,
-                        Particula &particula) {
-  std::vector<Particula> one(1, particula);
-  detail::run(detail::state(true), one);
-  particula.fitness = one[0].fitness;
-  return one[0].fitness;
+                        P &particula) {
+  return detail::losses(detail::state(true), std::vector<P>(1, particula))[0];
 }
 inline void mkl_clean() {
   auto &S = detail::state(true);
