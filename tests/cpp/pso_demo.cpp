@@ -99,12 +99,23 @@ static int case_file(const char *path, unsigned seed, unsigned np, unsigned nit)
     std::vector<mapol::pso::Solucao> melhores;
     mapol::pso::run(&melhores, 0.9, 0.4, 2.0, 2.0, np, nit,
                     (unsigned)est.size(), &est);
+    // a probe away from the defaults: every control at its range's midpoint
+    // step (AVR 1.0, OLTC 1.0, SHC max/3)
+    std::vector<Particula> probe(1);
+    probe[0].inicializar((unsigned)est.size(), &none, &est, true);
+    for (size_t i = 0; i < est.size(); i++)
+      probe[0].X[i] = est[i].tipo == Estrutura::SHC ? est[i].max / 3 : 1.0;
+    mapol::hybrid_runpf(est, probe);
     mapol::hybrid_free();
     for (const auto &s : melhores)
       std::printf("best it %u fitness %.17g\n", s.iteracao(), s.fitness());
     const auto &b = melhores.back();
     for (size_t i = 0; i < est.size(); i++)
-      std::printf("control %d %u %.17g\n", (int)est[i].tipo, est[i].id, b[(unsigned)i]);
+      std::printf("control %d %u %.17g %.17g\n", (int)est[i].tipo, est[i].id,
+                  b[(unsigned)i], est[i].defaultValue);
+    std::printf("probe loss %.17g\n", probe[0].fitness());
+    for (size_t i = 0; i < est.size(); i++)
+      std::printf("probe %d %u %.17g\n", (int)est[i].tipo, est[i].id, probe[0].X[i]);
   } catch (const mapol::Error &e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;
