@@ -118,6 +118,9 @@ typedef struct {
                                       fill in Newton steps (0: none) */
   int32_t tail_stage;              /* distinct blocks of the tail products
                                       staged in LDS (0: not staged) */
+  int32_t slot_fw_chunks, slot_bw_chunks;    /* slot-form sweep chunks */
+  int32_t slots;                   /* lane slots of the slot form (0: the
+                                      slot form does not apply) */
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */
@@ -242,6 +245,20 @@ bpf_status bpf_analyse(const bpf_case *c, bpf_info *info, int32_t *bus_of_blk,
 bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
                               int32_t *sw_chunk, int32_t *sw_row,
                               int32_t *sw_ent);
+
+/* Host-only: the slot form of the sweeps (the default when the solve vector
+ * fits in LDS and the solve workgroup has 256, 512 or 1024 lanes; the
+ * reference solves with a library, newtonpf.h:312-363).  The same levels as
+ * fw/bw_rows; each row owns a lane group of 2^g lanes (the power of two at
+ * least its entry count, at most 64) that starts at a multiple of its size,
+ * one lane per entry.  sl_chunk[4*(slot_fw_chunks+slot_bw_chunks)] = {first
+ * slot, end slot, largest g, 0}, at most 2048 slots each;
+ * sl_slot[4*slots] = {L or U position (padding: the row's first entry),
+ * column block + 1 (0: padding) | g << 27, block row (a group's first slot)
+ * or -1, pivot position (a group's first slot, backward) or the position}.
+ * slots = 0 when a row has more than 64 entries.  Arrays may be NULL. */
+bpf_status bpf_analyse_slots(const bpf_case *c, bpf_info *info,
+                             int32_t *sl_chunk, int32_t *sl_slot);
 
 /* Host-only: the Jacobian fill of a Newton step, which also finishes level
  * 0 of the elimination tree (part of the factorization of newtonpf.h:312-363

@@ -33,12 +33,21 @@ struct TimedEvent {
 
 static unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
 
-// entry-parallel sweeps: X and the product buffer both in the solve LDS
-static int use_entry_sweeps(const Symbolic &S, size_t scalar, int use_lds,
-                            int forced_rows) {
-  if (!use_lds || !S.sw_ok || forced_rows) return 0;
+// form of the sweeps (k_lu_solve's ep): 2 slot form (X in the solve LDS,
+// workgroups of 1024, 512 or 256 lanes), 1 entry-parallel (X and the
+// product buffer both in the solve LDS), 0 level rows.  form: 0 the fastest
+// that applies, 1 (BPF_SWEEP_ENTRY) entry-parallel at most, 2
+// (BPF_SWEEP_ROWS) level rows.
+static int sweep_mode(const Symbolic &S, size_t scalar, int use_lds, int form,
+                      int threads) {
+  if (!use_lds || form == 2) return 0;
+  if (form == 0 && S.sl_ok && (threads == 1024 || threads == 512 || threads == 256))
+    return 2;
+  if (!S.sw_ok) return 0;
   return ((size_t)ep_offset(S.nb) + 2 * (size_t)SWEEP_CHUNK) * scalar <=
-         (size_t)SOLVE_LDS_BYTES;
+                 (size_t)SOLVE_LDS_BYTES
+             ? 1
+             : 0;
 }
 // 1-D grid of the kernels that keep an instance's workgroups on one XCD
 // (xcd_slot): 8 * ceil(T / 8) * parts workgroups
@@ -92,7 +101,7 @@ struct bpf_engine {
   std::string launch_err;            // first failed launch (kernel name)
   const char *cur_kernel = "";
   int debug_sync = 0;                // BPF_DEBUG_SYNC: sync after launches
-  int sweep_rows = 0;                // BPF_SWEEP_ROWS: level-row sweeps only
+  int sweep_form = 0;                // sweep_mode's form (environment)
   bool capturing = false;            // recording the iteration graph
   std::map<std::string, std::pair<int, double>> ktime;
   bool solved = false;
@@ -325,8 +334,16 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
     SP.sw_row = (const int4 *)rw;
     SP.sw_ent = (const int2 *)en;
   }
-  const char *swr = getenv("BPF_SWEEP_ROWS");
-  e->sweep_rows = (swr && swr[0] == '1') ? 1 : 0;
+  SP.sl_nfw = S.sl_nfw;
+  SP.sl_nbw = S.sl_nbw;
+  {
+    const int *ch, *sl;
+    UP(ch, nonempty(S.sl_chunk, 4)); UP(sl, nonempty(S.sl_slot, 4));
+    SP.sl_chunk = (const int4 *)ch;
+    SP.sl_slot = (const int4 *)sl;
+  }
+  const char *swr = getenv("BPF_SWEEP_ROWS"), *swe = getenv("BPF_SWEEP_ENTRY");
+  e->sweep_form = (swr && swr[0] == '1') ? 2 : (swe && swe[0] == '1') ? 1 : 0;
 #undef UP
   return BPF_OK;
 }
@@ -553,6 +570,23 @@ static void launch_mismatch(bpf_engine *e, int mode) {
   tend(e);
 }
 
+// one sweep pass of k_lu_solve; the slot form (ep 2) has one kernel per
+// workgroup size (SLOT_CHUNK / threads slots per lane)
+template <typename T>
+static void launch_solve(bpf_engine *e, int T_, const void *LU, int rhs_mode,
+                         int accumulate, int use_lds, int ep) {
+  const int th = e->solve_threads;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(T_), dim3(th), 0, e->stream, e->g, e->d,
+                       (const B4<T> *)LU, (const B4<T> *)e->d_TL, rhs_mode,
+                       accumulate, e->splan, use_lds, ep, (T *)e->d_xglob);
+  };
+  if (ep == 2 && th == 1024) go(k_lu_solve<T, SLOT_CHUNK / 1024>);
+  else if (ep == 2 && th == 512) go(k_lu_solve<T, SLOT_CHUNK / 512>);
+  else if (ep == 2 && th == 256) go(k_lu_solve<T, SLOT_CHUNK / 256>);
+  else go(k_lu_solve<T, 0>);
+}
+
 template <typename T>
 static void launch_newton_step(bpf_engine *e) {
   B4<T> *LU = (B4<T> *)e->d_LU;
@@ -570,7 +604,7 @@ static void launch_newton_step(bpf_engine *e) {
   size_t xs = sizeof(T) * 2 * (size_t)e->S.nb;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
   const int refine = (sizeof(T) == 4) ? e->opt.refine_steps : 0;
-  const int ep = use_entry_sweeps(e->S, sizeof(T), use_lds, e->sweep_rows);
+  const int ep = sweep_mode(e->S, sizeof(T), use_lds, e->sweep_form, e->solve_threads);
   // one solve on the mismatch, then per refinement step the FP64 residual
   // (many workgroups per instance) and a solve on it, accumulated into dx
   for (int s = 0; s <= refine; s++) {
@@ -581,11 +615,7 @@ static void launch_newton_step(bpf_engine *e) {
       tend(e);
     }
     tbegin(e, "lu_solve");
-    hipLaunchKernelGGL(k_lu_solve<T>, dim3(e->T), dim3(e->solve_threads),
-                       0, e->stream, e->g, e->d,
-                       (const B4<T> *)LU, (const B4<T> *)e->d_TL,
-                       s == 0 ? 0 : 3, s > 0 ? 1 : 0, e->splan, use_lds, ep,
-                       (T *)e->d_xglob);
+    launch_solve<T>(e, e->T, LU, s == 0 ? 0 : 3, s > 0 ? 1 : 0, use_lds, ep);
     tend(e);
   }
 }
@@ -715,13 +745,9 @@ static bpf_status run_fd(bpf_engine *e) {
   for (int it = 0; it < e->opt.maxit_fd + 1; it++) {
     for (int half = 1; half <= 2; half++) {
       tbegin(e, "fd_solve");
-      hipLaunchKernelGGL(k_lu_solve<double>, dim3(T), dim3(e->solve_threads),
-                         0, e->stream, e->g, e->d, (const B4<double> *)LU,
-                         (const B4<double> *)e->d_TL, half, 0, e->splan,
-                         use_lds,
-                         use_entry_sweeps(e->S, sizeof(double), use_lds,
-                                          e->sweep_rows),
-                         (double *)e->d_xglob);
+      launch_solve<double>(e, T, LU, half, 0, use_lds,
+                           sweep_mode(e->S, sizeof(double), use_lds,
+                                      e->sweep_form, e->solve_threads));
       tend(e);
       tbegin(e, "update");
       hipLaunchKernelGGL(k_fd_update, xgrid(T, e->n, 256), dim3(256),
@@ -894,6 +920,9 @@ static void fill_info(const Symbolic &S, bpf_info *info) {
   info->sweep_entries = (int)S.sw_ent.size() / 2;
   info->fill_level0_chunks = S.l0_chunks;
   info->tail_stage = (int)S.tl_stage.size();
+  info->slot_fw_chunks = S.sl_ok ? S.sl_nfw : 0;
+  info->slot_bw_chunks = S.sl_ok ? S.sl_nbw : 0;
+  info->slots = S.sl_ok ? (int)S.sl_slot.size() / 4 : 0;
 }
 
 extern "C" bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
@@ -912,6 +941,26 @@ extern "C" bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
     if (dst && !v.empty()) memcpy(dst, v.data(), sizeof(int32_t) * v.size());
   };
   put(sw_chunk, S.sw_chunk); put(sw_row, S.sw_row); put(sw_ent, S.sw_ent);
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_analyse_slots(const bpf_case *c, bpf_info *info,
+                                        int32_t *sl_chunk, int32_t *sl_slot) {
+  if (!c || !info || c->nbus <= 0 || c->nbranch < 0 || !c->bus_type ||
+      (c->nbranch && (!c->br_from || !c->br_to)))
+    return BPF_ERR_INVALID_ARG;
+  Symbolic S;
+  std::string err;
+  if (!analyse(c->nbus, c->nbranch, c->br_from, c->br_to, c->bus_type, S, &err))
+    return BPF_ERR_INVALID_ARG;
+  memset(info, 0, sizeof(*info));
+  fill_info(S, info);
+  if (S.sl_ok) {
+    if (sl_chunk && !S.sl_chunk.empty())
+      memcpy(sl_chunk, S.sl_chunk.data(), sizeof(int32_t) * S.sl_chunk.size());
+    if (sl_slot && !S.sl_slot.empty())
+      memcpy(sl_slot, S.sl_slot.data(), sizeof(int32_t) * S.sl_slot.size());
+  }
   return BPF_OK;
 }
 

@@ -435,6 +435,10 @@ struct SolvePlan {
   const int *sw_cmax;                  // longest row of the chunk
   const int4 *sw_row;                  // {block row, entry0, entry1, pivot position}
   const int2 *sw_ent;                  // {block position, column block}
+  // slot form: one lane per entry, a power-of-two lane group per row
+  int sl_nfw, sl_nbw;
+  const int4 *sl_chunk;                // {slot0, slot1, log2 largest group, 0}
+  const int4 *sl_slot;                 // {position, column + 1 | log2 group, row or -1, pivot}
 };
 
 // Lanes that share one item (a task's pair list, a row's entries): a power of
@@ -1306,6 +1310,206 @@ __device__ __forceinline__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti,
   if (P.sw_nbw == 0 && tail) tail_rows<T>(A, Ti, X, rs, P, false);
 }
 
+// Slot-form sweeps (X in LDS).  A chunk's rows each own an aligned lane
+// group of 2^lg lanes, one lane per entry; a lane forms its entry's product
+// L_ij x_j (or U_ij x_j) in registers, the group sums its lanes' products
+// into its first lane with row shifts (DPP within 16 lanes, cross-lane
+// reads beyond), and that lane updates x_i (backward: times the pivot
+// inverse).  One barrier per chunk.  Loads run two stages ahead as in the
+// entry-parallel form (slot descriptors two chunks ahead, factor blocks one
+// chunk ahead); a wave skips the slot blocks it has no lane in with a scalar
+// branch, so load counts vary and one explicit wait at the top of each chunk
+// receives everything the previous chunk requested.
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum of the lanes of each aligned group of 2^lg lanes into the group's
+// first lane (lane i adds lane i + 2^s for s < lg); glg = the largest lg of
+// the wave's groups.  Every lane of the wave must take part.
+template <typename T>
+__device__ __forceinline__ void slot_group_sum(T &p0, T &p1, int lg, int glg) {
+  // DPP row_shl:n (0x100 + n): lane i reads lane i + n of its 16-lane row
+  if (glg > 0) {
+    const T a = dpp_row<0x101>(p0), b = dpp_row<0x101>(p1);
+    if (lg > 0) { p0 += a; p1 += b; }
+  }
+  if (glg > 1) {
+    const T a = dpp_row<0x102>(p0), b = dpp_row<0x102>(p1);
+    if (lg > 1) { p0 += a; p1 += b; }
+  }
+  if (glg > 2) {
+    const T a = dpp_row<0x104>(p0), b = dpp_row<0x104>(p1);
+    if (lg > 2) { p0 += a; p1 += b; }
+  }
+  if (glg > 3) {
+    const T a = dpp_row<0x108>(p0), b = dpp_row<0x108>(p1);
+    if (lg > 3) { p0 += a; p1 += b; }
+  }
+  if (glg > 4) {
+    const T a = __shfl_down(p0, 16, 64), b = __shfl_down(p1, 16, 64);
+    if (lg > 4) { p0 += a; p1 += b; }
+  }
+  if (glg > 5) {
+    const T a = __shfl_down(p0, 32, 64), b = __shfl_down(p1, 32, 64);
+    if (lg > 5) { p0 += a; p1 += b; }
+  }
+}
+
+template <int SP>
+struct SlotIdx {
+  int4 s[SP];       // the thread's slots of one chunk
+};
+
+// a prefetched slot field, opaque to the compiler from here on: arithmetic
+// on it (the 64-bit address of a block) is then not moved up to the load,
+// where it would wait for the load at once
+__device__ __forceinline__ int pinned(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+template <typename T, int SP>
+struct SlotVal {
+  B4<T> a[SP];      // entry blocks
+  B4<T> d[SP];      // pivot inverses (backward sweep, groups' first lanes)
+};
+
+// The slot descriptors are loaded by every wave (clamped to the chunk's last
+// slot): a load that a wave may skip leaves the register's previous value
+// on the skipping path, and the compiler waits for the load before merging
+// the two.  The blocks are loaded only by waves with a slot in the block
+// (their registers are not read otherwise, so nothing merges).
+template <int SP>
+__device__ __forceinline__ SlotIdx<SP> slot_index(const int4 *slot, const int4 &m) {
+  SlotIdx<SP> ix;
+#pragma unroll
+  for (int k = 0; k < SP; k++) {
+    const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
+    ix.s[k] = slot[min(q, m.y - 1)];
+  }
+  return ix;
+}
+
+template <typename T, int SP>
+__device__ __forceinline__ SlotVal<T, SP> slot_values(const B4<T> *A,
+                                                      const int4 &m, bool bw,
+                                                      const SlotIdx<SP> &ix) {
+  SlotVal<T, SP> v;
+#pragma unroll
+  for (int k = 0; k < SP; k++) {
+    const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
+    if (wave_any(q, m.y)) {
+      v.a[k] = A[pinned(ix.s[k].x)];
+      if (bw) v.d[k] = A[pinned(ix.s[k].w)];
+    }
+  }
+  return v;
+}
+
+// chunk c (slots cur, blocks val, both received); before its reduction the
+// blocks of chunk c + 1 are requested from its slots (nxt) and the slots of
+// chunk c + 2 (c_after, clamped to the last chunk) into cur
+template <typename T, int SP>
+__device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
+                                           const B4<T> *A, T *X, int c,
+                                           bool bw, SlotIdx<SP> &cur,
+                                           const SlotVal<T, SP> &val,
+                                           int c_next, bool next_bw,
+                                           const SlotIdx<SP> &nxt,
+                                           SlotVal<T, SP> &nval, int c_after) {
+  __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+  int4 m = swc[c];
+  m.x = __builtin_amdgcn_readfirstlane(m.x);
+  m.y = __builtin_amdgcn_readfirstlane(m.y);
+  m.z = __builtin_amdgcn_readfirstlane(m.z);
+  T p0[SP], p1[SP];
+  int lead[SP], lg[SP];
+#pragma unroll
+  for (int k = 0; k < SP; k++) {
+    const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
+    if (wave_any(q, m.y)) {
+      int4 sd = cur.s[k];
+      sd.y = pinned(sd.y);
+      sd.z = pinned(sd.z);
+      const int colp = sd.y & ((1 << SLOT_LG_SHIFT) - 1);
+      const int col = max(colp - 1, 0);
+      const T y0 = X[2 * col], y1 = X[2 * col + 1];
+      const B4<T> &a = val.a[k];
+      const bool ent = colp > 0 && q < m.y;
+      p0[k] = ent ? a.a * y0 + a.b * y1 : (T)0;
+      p1[k] = ent ? a.c * y0 + a.d * y1 : (T)0;
+      lead[k] = q < m.y ? sd.z : -1;
+      lg[k] = sd.y >> SLOT_LG_SHIFT;
+    }
+  }
+  const int4 mn = swc[c_next];
+  nval = slot_values<T, SP>(A, mn, next_bw, nxt);
+#pragma unroll
+  for (int k = 0; k < SP; k++) {
+    const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
+    if (wave_any(q, m.y)) {
+      slot_group_sum(p0[k], p1[k], lg[k], m.z);
+      const int i = lead[k];
+      if (i >= 0) {
+        const T x0 = X[2 * i] - p0[k], x1 = X[2 * i + 1] - p1[k];
+        if (bw) {
+          const B4<T> &D = val.d[k];
+          X[2 * i] = D.a * x0 + D.b * x1;
+          X[2 * i + 1] = D.c * x0 + D.d * x1;
+        } else {
+          X[2 * i] = x0;
+          X[2 * i + 1] = x1;
+        }
+      }
+    }
+  }
+  // the slots of chunk c + 2 replace those of chunk c, which are no longer
+  // read (a load into registers still in use would be waited for at once)
+  cur = slot_index<SP>(slot, swc[c_after]);
+  __syncthreads();
+}
+
+template <typename T, int SP>
+__device__ __forceinline__ void lu_sweeps_slot(const B4<T> *A, const B4<T> *Ti,
+                                               T *X, T *rs, const int4 *swc,
+                                               const SolvePlan &P) {
+  const int nc = P.sl_nfw + P.sl_nbw;
+  const bool tail = P.tn > 0;
+  if (nc == 0) {
+    if (tail) tail_rows<T>(A, Ti, X, rs, P, false);
+    return;
+  }
+  const int cl = nc - 1;
+  SlotIdx<SP> ia, ib;
+  SlotVal<T, SP> va, vb;
+  ia = slot_index<SP>(P.sl_slot, swc[0]);
+  __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+  va = slot_values<T, SP>(A, swc[0], 0 >= P.sl_nfw, ia);
+  ib = slot_index<SP>(P.sl_slot, swc[min(1, cl)]);
+  for (int c = 0; c < nc; c += 2) {
+    if (c == P.sl_nfw && tail) tail_rows<T>(A, Ti, X, rs, P, false);
+    slot_chunk<T, SP>(P.sl_slot, swc, A, X, c, c >= P.sl_nfw, ia, va,
+                      min(c + 1, cl), min(c + 1, cl) >= P.sl_nfw, ib, vb,
+                      min(c + 2, cl));
+    if (c + 1 >= nc) break;
+    if (c + 1 == P.sl_nfw && tail) tail_rows<T>(A, Ti, X, rs, P, false);
+    slot_chunk<T, SP>(P.sl_slot, swc, A, X, c + 1, c + 1 >= P.sl_nfw, ib, vb,
+                      min(c + 2, cl), min(c + 2, cl) >= P.sl_nfw, ia, va,
+                      min(c + 3, cl));
+  }
+  if (P.sl_nbw == 0 && tail) tail_rows<T>(A, Ti, X, rs, P, false);
+}
+
 // ---------------------------------------------------------------------------
 // Newton step solve: dx = J^-1 rhs with the factors in precision T (one pass;
 // mixed precision runs it again on the FP64 residual of k_residual and
@@ -1324,7 +1528,7 @@ __host__ __device__ constexpr int ep_offset(int nb) { return (2 * nb + 3) & ~3; 
 // X in LDS and once in global memory, so that the compiler addresses X
 // directly: through a generic pointer every access would be a flat
 // instruction, whose wait also drains the prefetched global loads.
-template <typename T, bool LDS>
+template <typename T, bool LDS, int SP>
 __device__ __forceinline__ void solve_pass(const DevTopo &g, const B4<T> *A,
                                            const B4<T> *Ti, T *X, T *prod,
                                            T *rs, const int4 *swc,
@@ -1350,7 +1554,8 @@ __device__ __forceinline__ void solve_pass(const DevTopo &g, const B4<T> *A,
     }
   }
   __syncthreads();
-  if (LDS && ep) lu_sweeps_ep<T>(A, Ti, X, prod, rs, swc, swm, P);
+  if (SP > 0) lu_sweeps_slot<T, (SP > 0 ? SP : 1)>(A, Ti, X, rs, swc, P);
+  else if (LDS && ep) lu_sweeps_ep<T>(A, Ti, X, prod, rs, swc, swm, P);
   else lu_sweeps<T>(g, A, Ti, X, rs, P);
   if (accumulate) {
     for (int k0 = threadIdx.x; k0 < n2; k0 += VU * stride) {
@@ -1368,13 +1573,16 @@ __device__ __forceinline__ void solve_pass(const DevTopo &g, const B4<T> *A,
   }
 }
 
-template <typename T>
-__global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
-                           const B4<T> *TL, int rhs_mode, int accumulate,
-                           SolvePlan P, int use_lds, int ep, T *xglob) {
+// SP > 0: slot form with SLOT_CHUNK / SP lanes per workgroup (the launch
+// bound lets the larger slot counts of narrower workgroups stay in registers)
+template <typename T, int SP>
+__global__ void __launch_bounds__(SP > 0 ? SLOT_CHUNK / SP : 1024)
+k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, const B4<T> *TL,
+           int rhs_mode, int accumulate, SolvePlan P, int use_lds, int ep,
+           T *xglob) {
   __shared__ T smem[SOLVE_LDS_BYTES / sizeof(T)];
   __shared__ T rs[2 * TAIL_MAX];
-  // the entry-parallel chunk table in LDS (see the factorization)
+  // the chunk table in LDS (see the factorization)
   __shared__ int4 s_swc[MAX_CHUNKS];
   __shared__ int s_swm[MAX_CHUNKS];
   const int t = blockIdx.x;
@@ -1382,18 +1590,25 @@ __global__ void k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU,
   const B4<T> *A = LU + (size_t)t * g.nnzLU;
   const B4<T> *Ti = TL + (size_t)t * P.tn * P.tn;
   const double *src = (rhs_mode == 3 ? d.rhs : d.F) + (size_t)t * 2 * g.nb;
-  if (ep)
+  if (SP > 0)
+    for (int q = threadIdx.x; q < P.sl_nfw + P.sl_nbw; q += blockDim.x)
+      s_swc[q] = P.sl_chunk[q];
+  else if (ep == 1)
     for (int q = threadIdx.x; q < P.sw_nfw + P.sw_nbw; q += blockDim.x) {
       s_swc[q] = P.sw_chunk[q];
       s_swm[q] = P.sw_cmax[q];
     }
   double *dx = d.dx + (size_t)t * 2 * g.nb;
-  if (use_lds)
-    solve_pass<T, true>(g, A, Ti, smem, smem + ep_offset(g.nb), rs, s_swc,
-                        s_swm, P, ep, src, rhs_mode, accumulate, dx);
+  T *prod = smem + ep_offset(g.nb);
+  if (SP > 0)
+    solve_pass<T, true, SP>(g, A, Ti, smem, prod, rs, s_swc, s_swm, P, 0, src,
+                            rhs_mode, accumulate, dx);
+  else if (use_lds)
+    solve_pass<T, true, 0>(g, A, Ti, smem, prod, rs, s_swc, s_swm, P, ep,
+                           src, rhs_mode, accumulate, dx);
   else
-    solve_pass<T, false>(g, A, Ti, xglob + (size_t)t * 2 * g.nb, nullptr, rs,
-                         s_swc, s_swm, P, 0, src, rhs_mode, accumulate, dx);
+    solve_pass<T, false, 0>(g, A, Ti, xglob + (size_t)t * 2 * g.nb, nullptr, rs,
+                            s_swc, s_swm, P, 0, src, rhs_mode, accumulate, dx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1822,7 +2037,13 @@ template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *, int);
 template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *, int);
 template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *);
 template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *, int, int *);
-template __global__ void k_lu_solve<float>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
-template __global__ void k_lu_solve<double>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
+template __global__ void k_lu_solve<float, 0>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
+template __global__ void k_lu_solve<float, 2>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
+template __global__ void k_lu_solve<float, 4>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
+template __global__ void k_lu_solve<float, 8>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
+template __global__ void k_lu_solve<double, 0>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
+template __global__ void k_lu_solve<double, 2>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
+template __global__ void k_lu_solve<double, 4>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
+template __global__ void k_lu_solve<double, 8>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
 
 }  // namespace bpf

@@ -31,6 +31,7 @@ EXPORTS = [
     "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_adjoint",
     "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
     "bpf_analyse", "bpf_analyse_sweeps", "bpf_analyse_fill", "bpf_analyse_tail",
+    "bpf_analyse_slots",
     "bpf_load_matpower",
     "bpf_case_error", "bpf_free_case",
 ]
@@ -72,7 +73,9 @@ class BpfInfo(C.Structure):
                 ("tail_pairs", C.c_int32), ("tail_row_entries", C.c_int32),
                 ("sweep_fw_chunks", C.c_int32), ("sweep_bw_chunks", C.c_int32),
                 ("sweep_rows", C.c_int32), ("sweep_entries", C.c_int32),
-                ("fill_level0_chunks", C.c_int32), ("tail_stage", C.c_int32)]
+                ("fill_level0_chunks", C.c_int32), ("tail_stage", C.c_int32),
+                ("slot_fw_chunks", C.c_int32), ("slot_bw_chunks", C.c_int32),
+                ("slots", C.c_int32)]
 
 
 _LIB = None
@@ -120,6 +123,7 @@ def load_library(path: str = LIB_PATH):
     L.bpf_analyse_sweeps.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 3
     L.bpf_analyse_fill.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
     L.bpf_analyse_tail.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
+    L.bpf_analyse_slots.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
     L.bpf_load_matpower.argtypes = [C.c_char_p, C.c_int32, C.POINTER(BpfCase),
                                     C.POINTER(_pd), C.POINTER(C.c_void_p)]
     L.bpf_case_error.argtypes = [C.c_void_p]
@@ -271,6 +275,13 @@ def analyse(case):
     if st != 0:
         raise EngineError(f"bpf_analyse_tail failed ({st})")
     out.update(tl)
+    sl = dict(sl_chunk=np.zeros((i["slot_fw_chunks"] + i["slot_bw_chunks"], 4), np.int32),
+              sl_slot=np.zeros((i["slots"], 4), np.int32))
+    st = L.bpf_analyse_slots(C.byref(s), C.byref(info), _ptr(sl["sl_chunk"], _pi),
+                             _ptr(sl["sl_slot"], _pi))
+    if st != 0:
+        raise EngineError(f"bpf_analyse_slots failed ({st})")
+    out.update(sl)
     return out
 
 

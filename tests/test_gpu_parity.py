@@ -214,3 +214,31 @@ def test_instances_can_be_set_again_with_another_count():
             np.testing.assert_array_equal(r[k], rf[k], err_msg=f"T={T} {k}")
         np.testing.assert_array_equal(r["V"], rf["V"])
     e.close()
+
+
+@pytest.mark.parametrize("form,threads", [("slot", 1024), ("slot", 512), ("slot", 256),
+                                          ("entry", 512), ("rows", 512)])
+def test_sweep_forms_match_oracle(monkeypatch, form, threads):
+    """Every form of the triangular sweeps (slot form with 1024, 512 and 256
+    lanes, entry-parallel, level rows) solves pf9241x200's first contingencies
+    with the oracle's iterations and verdicts, FP64 and mixed precision."""
+    print(f"[sweeps] form={form} threads={threads}", flush=True)
+    monkeypatch.setenv("BPF_SWEEP_ENTRY", "1" if form == "entry" else "0")
+    monkeypatch.setenv("BPF_SWEEP_ROWS", "1" if form == "rows" else "0")
+    case, cont = synth.workload("pf9241x200")
+    T = 4
+    orc = oracle.Oracle(case)
+    ref = [orc.solve(oracle.Oracle.opts(enforce_qlim=True), br_on=cont.br_on[t],
+                     gen_on=cont.gen_on[t]) for t in range(T)]
+    for precision in (BPF_FP64, BPF_MIXED):
+        e = Engine(case, precision=precision, enforce_qlim=True)
+        e.set_launch(solve=threads)
+        e.set_instances(br_on=cont.br_on[:T], gen_on=cont.gen_on[:T])
+        e.solve()
+        r = e.results()
+        e.close()
+        for t in range(T):
+            assert r["verdict"][t] == ref[t]["verdict"], t
+            assert r["iterations"][t] == ref[t]["iterations"], t
+            if ref[t]["converged"]:
+                assert np.abs(r["V"][t] - ref[t]["V"]).max() <= DV_TOL

@@ -195,6 +195,49 @@ def _sweeps_ep(a, A, Sinv, b):
     return x.ravel()
 
 
+def _sweeps_slot(a, A, Sinv, b):
+    """the slot-form sweeps: per chunk, every lane's entry product, then each
+    row's lane group summed pairwise into its first lane (lane i adds lane
+    i + 2^s for s < g), which updates x; the tail rows between the forward
+    and the backward chunks"""
+    x = b.reshape(-1, 2).copy()
+    nfw = a["info"]["slot_fw_chunks"]
+    ch, sl = a["sl_chunk"], a["sl_slot"]
+    LG = 27
+
+    def tail():
+        s = a["info"]["tail_n"]
+        piv = a["tail_piv"]
+        rs = np.zeros((s, 2))
+        for k in range(s):
+            rs[k] = x[piv[k]]
+            for q in range(a["ts_ptr"][k], a["ts_ptr"][k + 1]):
+                rs[k] -= A[a["ts_pos"][q]] @ x[a["ts_col"][q]]
+        for k in range(s):
+            x[piv[k]] = sum(Sinv[k, j] @ rs[j] for j in range(s))
+
+    for c in range(len(ch)):
+        if c == nfw:
+            tail()
+        s0, s1 = ch[c, 0], ch[c, 1]
+        colp = sl[s0:s1, 1] & ((1 << LG) - 1)
+        prod = np.array([A[p] @ x[max(cp - 1, 0)] if cp > 0 else np.zeros(2)
+                         for p, cp in zip(sl[s0:s1, 0], colp)])
+        q = 0
+        while q < s1 - s0:
+            pos, w, row, piv = sl[s0 + q]
+            g = w >> LG
+            v = prod[q:q + (1 << g)]
+            while len(v) > 1:
+                v = v[0::2] + v[1::2]
+            xi = x[row] - v[0]
+            x[row] = A[piv] @ xi if c >= nfw else xi
+            q += 1 << g
+    if nfw == len(ch):
+        tail()
+    return x.ravel()
+
+
 def _system(name, seed=0):
     case, cont = synth.workload(name)
     a = engine.analyse(case)
@@ -219,6 +262,10 @@ def test_schedule_replay_gives_the_newton_step(name):
     # the entry-parallel chunks give the same step
     xe = _sweeps_ep(a, LU, Sinv, rhs)
     assert np.abs(xe - x).max() <= 1e-12 * max(1, np.abs(x).max())
+    # and so do the slot-form chunks
+    assert a["info"]["slots"] > 0 or a["info"]["tail_n"] == a["info"]["nblocks"]
+    xs = _sweeps_slot(a, LU, Sinv, rhs)
+    assert np.abs(xs - x).max() <= 1e-12 * max(1, np.abs(x).max())
     # the PV magnitude unknowns of the superset system come out exactly zero
     pv_blocks = [r[0] for r in rowmap[:int((case.bus_type == 2).sum())]]
     assert np.all(x[2 * np.array(pv_blocks, int) + 1] == 0.0)
@@ -386,3 +433,49 @@ def test_sweep_chunks_cover_the_levels():
             assert np.array_equal(ent[f0:f1, 0], np.arange(p0, p1))
             assert np.array_equal(ent[f0:f1, 1], ci[p0:p1])
             assert dpos == (-1 if fw else diag[r])
+
+
+@pytest.mark.parametrize("name", ["pf1354x200", "pf9241x200"])
+def test_slot_chunks_cover_the_levels(name):
+    """the slot-form chunks hold every non-tail row once per sweep (the
+    forward sweep only rows with L entries), in level order; a row's lane
+    group is the power of two at least its entry count, starts at a multiple
+    of its size within the chunk, holds the row's own entries in order and
+    then padding slots; at most 2048 slots per chunk"""
+    case, _ = synth.workload(name)
+    a = engine.analyse(case)
+    i = a["info"]
+    rp, ci = a["lu_rowptr"], a["lu_colind"]
+    diag = _diag(a)
+    nfw = i["slot_fw_chunks"]
+    ch, sl = a["sl_chunk"], a["sl_slot"]
+    assert i["slots"] == len(sl) > 0 and len(ch) == nfw + i["slot_bw_chunks"]
+    assert np.all(ch[:, 1] - ch[:, 0] <= 2048) and np.all(ch[:, 1] > ch[:, 0])
+    assert np.array_equal(ch[1:, 0], ch[:-1, 1]) and ch[0, 0] == 0 and ch[-1, 1] == len(sl)
+    LG = 27
+    for lo, hi, order, fw in ((0, nfw, a["fw_rows"], True),
+                              (nfw, len(ch), a["bw_rows"], False)):
+        if fw:
+            order = np.array([r for r in order if diag[r] > rp[r]])
+        seen = []
+        for c in range(lo, hi):
+            s0, s1 = ch[c, 0], ch[c, 1]
+            q, gmax = s0, 0
+            while q < s1:
+                pos, w, row, piv = sl[q]
+                g = w >> LG
+                gmax = max(gmax, g)
+                assert row >= 0 and (q - s0) % (1 << g) == 0 and q + (1 << g) <= s1
+                p0, p1 = (rp[row], diag[row]) if fw else (diag[row] + 1, rp[row + 1])
+                assert (1 << g) >= p1 - p0 and (g == 0 or (1 << (g - 1)) < p1 - p0)
+                grp = sl[q:q + (1 << g)]
+                assert np.all(grp[:, 1] >> LG == g) and np.all(grp[1:, 2] == -1)
+                n = p1 - p0
+                assert np.array_equal(grp[:n, 0], np.arange(p0, p1))
+                assert np.array_equal((grp[:n, 1] & ((1 << LG) - 1)) - 1, ci[p0:p1])
+                assert np.all(grp[n:, 1] & ((1 << LG) - 1) == 0)
+                assert piv == (diag[row] if not fw else pos)
+                seen.append(row)
+                q += 1 << g
+            assert ch[c, 2] == gmax
+        assert np.array_equal(np.array(seen, int), np.asarray(order, int))
