@@ -644,10 +644,10 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   // compiler and any wait for one of them waits for all (global loads
   // complete in order).  The order below makes each such wait fall where
   // everything in flight is needed anyway: one explicit wait at the top of
-  // the chunk (the pair indices and the previous chunk's stores), then
-  // operands and descriptors, then the Jacobian values (once the descriptors
-  // arrive, in flight while the products form), and the next chunk's pair
-  // indices only once the Jacobian values are in registers.
+  // the chunk (the pair indices and the previous chunk's stores), one after
+  // the operands and descriptors; the next chunk's pair indices are then
+  // requested and stay in flight through the reduction, which reads only
+  // LDS and registers.
   for (int c = c_start; c < P.nchunk; c++) {
     __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     const int q0 = nq0, q1 = nq1;
@@ -685,7 +685,6 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     load_operands(0);
     // the thread's first batch of outputs: descriptors
     int4 ta[NB], tb[NB];
-    B4<T> av[NB];
 #pragma unroll
     for (int u = 0; u < NB; u++) {
       const int k = kf + u * stride;
@@ -695,12 +694,6 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       }
     }
     __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
-    // Jacobian values of the first batch; outputs without one read block 0
-    // (cached) and drop it
-#pragma unroll
-    for (int u = 0; u < NB; u++)
-      if (wave_any(kf + u * stride, k1))
-        av[u] = A[(ta[u].x & TASK_HAS_A) ? (ta[u].x & TASK_POS_MASK) : 0];
     form_products(0);
 #pragma unroll
     for (int j0 = PR; j0 < PF; j0 += PR) {
@@ -719,37 +712,19 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       buf[q - q0] = v;
     }
     __syncthreads();
-    B4<T> ev[NB];
-#pragma unroll
-    for (int u = 0; u < NB; u++)
-      ev[u] = wave_any(kf + u * stride, k1)
-                  ? blk_sel(lane == 0 && (ta[u].x & TASK_HAS_A), av[u], blk_zero<T>())
-                  : blk_zero<T>();
     if (c + 1 < P.nchunk) prefetch_pairs(c + 1);
-    // outputs in batches of NB per thread: a later batch's descriptors, then
-    // its Jacobian values, are requested together before any is reduced
-    for (int kb = kf; kb < k1; kb += NB * stride) {
-      if (kb != kf) {
-#pragma unroll
-        for (int u = 0; u < NB; u++) {
-          const int k = kb + u * stride;
-          if (wave_any(k, k1)) {
-            ta[u] = P.tk_a[min(k, klast)];
-            tb[u] = P.tk_b[min(k, klast)];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < NB; u++)
-          if (wave_any(kb + u * stride, k1))
-            ev[u] = blk_sel(lane == 0 && (ta[u].x & TASK_HAS_A),
-                            A[(ta[u].x & TASK_HAS_A) ? (ta[u].x & TASK_POS_MASK) : 0],
-                            blk_zero<T>());
-      }
+    // outputs in batches of NB per thread; every output's Jacobian value is
+    // an init entry of its own product range.  The first batch uses the
+    // descriptors received with the operands (its own code path, so that no
+    // wait for a later batch's descriptors, which would also wait for the
+    // pair indices just requested, is merged into it); a later batch's
+    // descriptors are requested together before any is reduced.
+    auto reduce_batch = [&](int kb) {
 #pragma unroll
       for (int u = 0; u < NB; u++) {
         if (kb + u * stride >= k1) break;
         const int p = ta[u].x & TASK_POS_MASK;
-        B4<T> e = ev[u];
+        B4<T> e = blk_zero<T>();
         for (int q = ta[u].y + lane; q < ta[u].z; q += gsz) blk_acc(e, buf[q - q0]);
         group_sum4(e, gsz);
         if (ta[u].x & TASK_LCOL) {
@@ -771,6 +746,18 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
           }
         }
       }
+    };
+    if (kf < k1) reduce_batch(kf);
+    for (int kb = kf + NB * stride; kb < k1; kb += NB * stride) {
+#pragma unroll
+      for (int u = 0; u < NB; u++) {
+        const int k = kb + u * stride;
+        if (wave_any(k, k1)) {
+          ta[u] = P.tk_a[min(k, klast)];
+          tb[u] = P.tk_b[min(k, klast)];
+        }
+      }
+      reduce_batch(kb);
     }
     __syncthreads();
   }

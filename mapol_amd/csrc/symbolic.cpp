@@ -300,6 +300,17 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
           out[1 + sk.size() + a].U.push_back(pmk);
         }
       }
+      // every other output with a Jacobian value also starts from an init
+      // entry, so the value arrives with the products' operands (a U block
+      // without products keeps its value and is left out below)
+      for (size_t q = 1; q < out.size(); q++) {
+        Task &t = out[q];
+        if (!(t.dst & TASK_HAS_A)) continue;
+        if (t.L.empty() && !(t.dst & TASK_LCOL)) continue;
+        t.L.insert(t.L.begin(), t.dst & TASK_POS_MASK);
+        t.U.insert(t.U.begin(), -1);
+        t.dst &= ~TASK_HAS_A;
+      }
       groups.push_back(std::move(out));
     }
     auto gpairs = [](const std::vector<Task> &g) {

@@ -403,7 +403,7 @@ def test_schedule_invariants():
             if d & LCOL:
                 assert q0 <= a["tk_b"][k][1] <= a["tk_b"][k][2] <= q1
     # every block of the Jacobian pattern outside the tail holds a Jacobian
-    # value: directly (HAS_A) or as its pivot's init entry
+    # value: directly (HAS_A) or as an init entry
     n_init = int((a["pairs"][:, 1] < 0).sum())
     assert n_init + int((a["tk_a"][:, 0] & HAS_A > 0).sum()) + len(unwritten) \
         + int((a["tl_dense"] & HAS_A > 0).sum()) == i["nnz_jac_blocks"]
