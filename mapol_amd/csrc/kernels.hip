@@ -469,13 +469,48 @@ __device__ __forceinline__ T group_sum(T v, int gsz) {
   return v;
 }
 
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum of each aligned group of gsz lanes (a power of two, the same for the
+// whole wave) into the group's first lane: lane i adds lane i + 2^s for
+// 2^s < gsz, by DPP row shifts within 16 lanes and cross-lane reads beyond.
+// Every lane of a group must be active; only the first lane's sum is whole.
 template <typename T>
-__device__ __forceinline__ void group_sum4(B4<T> &v, int gsz) {
-  for (int off = gsz >> 1; off > 0; off >>= 1) {
-    v.a += __shfl_xor(v.a, off, 64);
-    v.b += __shfl_xor(v.b, off, 64);
-    v.c += __shfl_xor(v.c, off, 64);
-    v.d += __shfl_xor(v.d, off, 64);
+__device__ __forceinline__ void group_sum4_first(B4<T> &v, int gsz) {
+  if (gsz > 1) {
+    v.a += dpp_row<0x101>(v.a); v.b += dpp_row<0x101>(v.b);
+    v.c += dpp_row<0x101>(v.c); v.d += dpp_row<0x101>(v.d);
+  }
+  if (gsz > 2) {
+    v.a += dpp_row<0x102>(v.a); v.b += dpp_row<0x102>(v.b);
+    v.c += dpp_row<0x102>(v.c); v.d += dpp_row<0x102>(v.d);
+  }
+  if (gsz > 4) {
+    v.a += dpp_row<0x104>(v.a); v.b += dpp_row<0x104>(v.b);
+    v.c += dpp_row<0x104>(v.c); v.d += dpp_row<0x104>(v.d);
+  }
+  if (gsz > 8) {
+    v.a += dpp_row<0x108>(v.a); v.b += dpp_row<0x108>(v.b);
+    v.c += dpp_row<0x108>(v.c); v.d += dpp_row<0x108>(v.d);
+  }
+  if (gsz > 16) {
+    v.a += __shfl_down(v.a, 16, 64); v.b += __shfl_down(v.b, 16, 64);
+    v.c += __shfl_down(v.c, 16, 64); v.d += __shfl_down(v.d, 16, 64);
+  }
+  if (gsz > 32) {
+    v.a += __shfl_down(v.a, 32, 64); v.b += __shfl_down(v.b, 32, 64);
+    v.c += __shfl_down(v.c, 32, 64); v.d += __shfl_down(v.d, 32, 64);
   }
 }
 
@@ -497,6 +532,28 @@ __device__ __forceinline__ bool blk_inv(const B4<T> &a, B4<T> &inv) {
   bool bad = !(det != (T)0) || !isfinite((double)det);
   T r = (T)1 / det;
   inv.a = a.d * r; inv.b = -a.b * r; inv.c = -a.c * r; inv.d = a.a * r;
+  return bad;
+}
+
+// inverse of a 2x2 matrix of 2x2 blocks [[A, B], [C, D]] through the Schur
+// complement S = D - C A^-1 B (the two pivots of Gauss-Jordan in sequence):
+// [[A^-1 + A^-1 B S^-1 C A^-1, -A^-1 B S^-1], [-S^-1 C A^-1, S^-1]].  True
+// when A or S is singular.
+template <typename T>
+__device__ __forceinline__ bool pair_inv(const B4<T> (&m)[2][2], B4<T> (&r)[2][2]) {
+  B4<T> ai, si;
+  bool bad = blk_inv(m[0][0], ai);
+  const B4<T> aib = blk_mul(ai, m[0][1]), cai = blk_mul(m[1][0], ai);
+  B4<T> sc = m[1][1];
+  blk_fms(sc, m[1][0], aib);
+  bad |= blk_inv(sc, si);
+  const B4<T> u = blk_mul(aib, si);      // A^-1 B S^-1
+  const B4<T> w = blk_mul(si, cai);      // S^-1 C A^-1
+  const B4<T> x = blk_mul(u, cai);
+  r[0][0] = {ai.a + x.a, ai.b + x.b, ai.c + x.c, ai.d + x.d};
+  r[0][1] = {-u.a, -u.b, -u.c, -u.d};
+  r[1][0] = {-w.a, -w.b, -w.c, -w.d};
+  r[1][1] = si;
   return bad;
 }
 
@@ -597,6 +654,11 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   union TabStage {
     struct { int ct[MAX_CHUNKS + 1], cp[MAX_CHUNKS + 1], cm[MAX_CHUNKS]; } tab;
     B4<T> stage[STAGE];
+    // blocked Gauss-Jordan of the dense tail (after the stage is read): per
+    // step parity the next pivot pair's columns, rows and pair inverse
+    struct {
+      B4<T> col[2][TAIL_MAX][2], row[2][2][TAIL_MAX], q[2][2][2];
+    } gj;
   };
   __shared__ TabStage sh;
   int *const s_ct = sh.tab.ct, *const s_cp = sh.tab.cp, *const s_cm = sh.tab.cm;
@@ -652,7 +714,7 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     const int q0 = nq0, q1 = nq1;
     const int k0 = s_ct[c], k1 = s_ct[c + 1];
-    const int gsz = lane_group(k1 - k0, s_cm[c]);
+    const int gsz = __builtin_amdgcn_readfirstlane(lane_group(k1 - k0, s_cm[c]));
     const int lane = threadIdx.x & (gsz - 1);
     const int stride = (int)blockDim.x / gsz;
     const int kf = k0 + (int)threadIdx.x / gsz;
@@ -726,11 +788,11 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
         const int p = ta[u].x & TASK_POS_MASK;
         B4<T> e = blk_zero<T>();
         for (int q = ta[u].y + lane; q < ta[u].z; q += gsz) blk_acc(e, buf[q - q0]);
-        group_sum4(e, gsz);
+        group_sum4_first(e, gsz);
         if (ta[u].x & TASK_LCOL) {
           B4<T> pv = blk_zero<T>();
           for (int q = tb[u].y + lane; q < tb[u].z; q += gsz) blk_acc(pv, buf[q - q0]);
-          group_sum4(pv, gsz);
+          group_sum4_first(pv, gsz);
           if (lane == 0) {
             B4<T> inv;
             blk_inv(pv, inv);   // a singular pivot is flagged by its own output
@@ -785,7 +847,7 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
           if (q0 + u * bd < ns) sh.stage[q0 + u * bd] = sv[u];
       }
       __syncthreads();
-      const int gsz = lane_group(P.tnent, P.tmax);
+      const int gsz = __builtin_amdgcn_readfirstlane(lane_group(P.tnent, P.tmax));
       const int lane = threadIdx.x & (gsz - 1);
       constexpr int TU = 8;
       for (int k = (int)threadIdx.x / gsz; k < P.tnent; k += (int)blockDim.x / gsz) {
@@ -803,11 +865,11 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
             blk_fms(acc, blk_sel(q + u * gsz < q1, sh.stage[ps[u] & 0xFFFF], blk_zero<T>()),
                     sh.stage[ps[u] >> 16]);
         }
-        group_sum4(acc, gsz);
+        group_sum4_first(acc, gsz);
         if (lane == 0) D[dd & TASK_POS_MASK] = acc;
       }
     } else {
-      const int gsz = lane_group(P.tnent, P.tmax);
+      const int gsz = __builtin_amdgcn_readfirstlane(lane_group(P.tnent, P.tmax));
       const int lane = threadIdx.x & (gsz - 1);
       constexpr int TU = 4;  // products of one lane requested together
       for (int k = (int)threadIdx.x / gsz; k < P.tnent; k += (int)blockDim.x / gsz) {
@@ -836,72 +898,109 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
           for (int u = 0; u < TU; u++)
             blk_fms(acc, blk_sel(q + u * gsz < q1, lv[u], blk_zero<T>()), uv[u]);
         }
-        group_sum4(acc, gsz);
+        group_sum4_first(acc, gsz);
         if (lane == 0) D[dd & TASK_POS_MASK] = acc;
       }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < s; q += blockDim.x) {
-      colb[0][q] = D[q * s];
-      rowb[0][q] = D[q];
-    }
-    if (threadIdx.x == 0) {
-      B4<T> iv;
-      if (blk_inv(D[0], iv)) sing = 1;
-      pvb[0] = iv;
-    }
-    __syncthreads();
-    constexpr int GJR = 4;  // entries a thread keeps in registers
     B4<T> *Ti = TL + (size_t)t * ss;
-    const int nt = (s + 1) / 2;  // 2x2 tiles per side
-    if (nt * nt <= (int)blockDim.x) {
-      // each thread owns a 2x2 tile of S for the whole elimination: per
-      // pivot it reads two row and two column blocks (shared by its four
-      // entries) and forms P*row once per column
+    // 2x2 tiles of blocks per side; an odd size gets an identity pivot
+    // appended (zero coupling: every other value comes out unchanged)
+    const int se = s + (s & 1), nt = se / 2;
+    if (sizeof(T) == 4 && nt * nt <= (int)blockDim.x) {
+      // FP32: Gauss-Jordan by pivot pairs, each thread owning a 2x2 tile for
+      // the whole elimination: per step the pair's inverse Q (formed by the
+      // owner of the next diagonal tile), its two columns and two rows pass
+      // through LDS; one barrier per pivot pair (in FP64 the sixteen blocks
+      // a step holds would spill: it eliminates one pivot per step)
+      auto dget = [&](int i, int j) {
+        if (i < s && j < s) return D[i * s + j];
+        B4<T> z = blk_zero<T>();
+        if (i == j) { z.a = (T)1; z.d = (T)1; }
+        return z;
+      };
+      for (int q = threadIdx.x; q < se; q += blockDim.x) {
+        sh.gj.col[0][q][0] = dget(q, 0);
+        sh.gj.col[0][q][1] = dget(q, 1);
+        sh.gj.row[0][0][q] = dget(0, q);
+        sh.gj.row[0][1][q] = dget(1, q);
+      }
+      if (threadIdx.x == 0) {
+        const B4<T> m[2][2] = {{dget(0, 0), dget(0, 1)}, {dget(1, 0), dget(1, 1)}};
+        B4<T> r[2][2];
+        if (pair_inv(m, r)) sing = 1;
+        sh.gj.q[0][0][0] = r[0][0]; sh.gj.q[0][0][1] = r[0][1];
+        sh.gj.q[0][1][0] = r[1][0]; sh.gj.q[0][1][1] = r[1][1];
+      }
       const int ti = (int)threadIdx.x / nt, tj = (int)threadIdx.x - ti * nt;
       const bool own = ti < nt;
       const int i0 = 2 * ti, j0 = 2 * tj;
-      const bool vi1 = i0 + 1 < s, vj1 = j0 + 1 < s;
       B4<T> dv[2][2];
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
-          const int i = min(i0 + a, s - 1), j = min(j0 + b, s - 1);
-          dv[a][b] = own ? D[i * s + j] : blk_zero<T>();
-        }
-      for (int k = 0; k < s; k++) {
-        const int cur = k & 1, nxt = cur ^ 1;
+        for (int b = 0; b < 2; b++) dv[a][b] = own ? dget(i0 + a, j0 + b) : blk_zero<T>();
+      __syncthreads();
+      for (int p = 0; p < nt; p++) {
+        const int cur = p & 1, nxt = cur ^ 1;
         if (own) {
-          const B4<T> Pk = pvb[cur];
-          const B4<T> col[2] = {colb[cur][i0], colb[cur][min(i0 + 1, s - 1)]};
-          const B4<T> pr[2] = {blk_mul(Pk, rowb[cur][j0]),
-                               blk_mul(Pk, rowb[cur][min(j0 + 1, s - 1)])};
+          B4<T> Q[2][2], cl[2][2], QR[2][2];
 #pragma unroll
           for (int a = 0; a < 2; a++)
 #pragma unroll
             for (int b = 0; b < 2; b++) {
-              if ((a && !vi1) || (b && !vj1)) continue;
-              const int i = i0 + a, j = j0 + b;
+              Q[a][b] = sh.gj.q[cur][a][b];
+              cl[a][b] = sh.gj.col[cur][i0 + a][b];
+            }
+#pragma unroll
+          for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+              const B4<T> r0 = sh.gj.row[cur][0][j0 + b], r1 = sh.gj.row[cur][1][j0 + b];
+              B4<T> v = blk_mul(Q[m][0], r0);
+              const B4<T> v1 = blk_mul(Q[m][1], r1);
+              v.a += v1.a; v.b += v1.b; v.c += v1.c; v.d += v1.d;
+              QR[m][b] = v;
+            }
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
               B4<T> v;
-              if (i == k) {
-                v = (j == k) ? Pk : pr[b];
-              } else if (j == k) {
+              if (ti == p && tj == p) {
+                v = Q[a][b];
+              } else if (ti == p) {
+                v = QR[a][b];
+              } else if (tj == p) {
                 v = blk_zero<T>();
-                blk_fms(v, col[a], Pk);
+                blk_fms(v, cl[a][0], Q[0][b]);
+                blk_fms(v, cl[a][1], Q[1][b]);
               } else {
                 v = dv[a][b];
-                blk_fms(v, col[a], pr[b]);
+                blk_fms(v, cl[a][0], QR[0][b]);
+                blk_fms(v, cl[a][1], QR[1][b]);
               }
               dv[a][b] = v;
-              if (j == k + 1) colb[nxt][i] = v;
-              if (i == k + 1) rowb[nxt][j] = v;
-              if (i == k + 1 && j == k + 1) {
-                B4<T> iv;
-                if (blk_inv(v, iv)) sing = 1;
-                pvb[nxt] = iv;
-              }
             }
+          // the next pair (tile p + 1): its columns, rows and inverse
+          if (tj == p + 1)
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int b = 0; b < 2; b++) sh.gj.col[nxt][i0 + a][b] = dv[a][b];
+          if (ti == p + 1)
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int b = 0; b < 2; b++) sh.gj.row[nxt][a][j0 + b] = dv[a][b];
+          if (ti == p + 1 && tj == p + 1) {
+            B4<T> r[2][2];
+            if (pair_inv(dv, r)) sing = 1;
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int b = 0; b < 2; b++) sh.gj.q[nxt][a][b] = r[a][b];
+          }
         }
         __syncthreads();
       }
@@ -910,79 +1009,150 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
         for (int a = 0; a < 2; a++)
 #pragma unroll
           for (int b = 0; b < 2; b++)
-            if ((!a || vi1) && (!b || vj1)) Ti[(i0 + a) * s + j0 + b] = dv[a][b];
-    } else if (ss <= GJR * (int)blockDim.x) {
-      // each thread owns up to GJR entries of S for the whole elimination:
-      // only the pivot row, column and inverse pass through LDS
-      B4<T> dv[GJR];
-      int di[GJR], dj[GJR];
-#pragma unroll
-      for (int m = 0; m < GJR; m++) {
-        const int q = threadIdx.x + m * blockDim.x;
-        di[m] = q < ss ? q / s : -1;
-        dj[m] = q < ss ? q - di[m] * s : -1;
-        dv[m] = q < ss ? D[q] : blk_zero<T>();
+            if (i0 + a < s && j0 + b < s) Ti[(i0 + a) * s + j0 + b] = dv[a][b];
+    } else {
+      for (int q = threadIdx.x; q < s; q += blockDim.x) {
+        colb[0][q] = D[q * s];
+        rowb[0][q] = D[q];
       }
-      for (int k = 0; k < s; k++) {
-        const int cur = k & 1, nxt = cur ^ 1;
-        const B4<T> Pk = pvb[cur];
+      if (threadIdx.x == 0) {
+        B4<T> iv;
+        if (blk_inv(D[0], iv)) sing = 1;
+        pvb[0] = iv;
+      }
+      __syncthreads();
+      constexpr int GJR = 4;  // entries a thread keeps in registers
+      const int nt1 = (s + 1) / 2;  // 2x2 tiles per side
+      if (nt1 * nt1 <= (int)blockDim.x) {
+        // each thread owns a 2x2 tile of S for the whole elimination: per
+        // pivot it reads two row and two column blocks (shared by its four
+        // entries) and forms P*row once per column
+        const int ti = (int)threadIdx.x / nt1, tj = (int)threadIdx.x - ti * nt1;
+        const bool own = ti < nt1;
+        const int i0 = 2 * ti, j0 = 2 * tj;
+        const bool vi1 = i0 + 1 < s, vj1 = j0 + 1 < s;
+        B4<T> dv[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++) {
+            const int i = min(i0 + a, s - 1), j = min(j0 + b, s - 1);
+            dv[a][b] = own ? D[i * s + j] : blk_zero<T>();
+          }
+        for (int k = 0; k < s; k++) {
+          const int cur = k & 1, nxt = cur ^ 1;
+          if (own) {
+            const B4<T> Pk = pvb[cur];
+            const B4<T> col[2] = {colb[cur][i0], colb[cur][min(i0 + 1, s - 1)]};
+            const B4<T> pr[2] = {blk_mul(Pk, rowb[cur][j0]),
+                                 blk_mul(Pk, rowb[cur][min(j0 + 1, s - 1)])};
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int b = 0; b < 2; b++) {
+                if ((a && !vi1) || (b && !vj1)) continue;
+                const int i = i0 + a, j = j0 + b;
+                B4<T> v;
+                if (i == k) {
+                  v = (j == k) ? Pk : pr[b];
+                } else if (j == k) {
+                  v = blk_zero<T>();
+                  blk_fms(v, col[a], Pk);
+                } else {
+                  v = dv[a][b];
+                  blk_fms(v, col[a], pr[b]);
+                }
+                dv[a][b] = v;
+                if (j == k + 1) colb[nxt][i] = v;
+                if (i == k + 1) rowb[nxt][j] = v;
+                if (i == k + 1 && j == k + 1) {
+                  B4<T> iv;
+                  if (blk_inv(v, iv)) sing = 1;
+                  pvb[nxt] = iv;
+                }
+              }
+          }
+          __syncthreads();
+        }
+        if (own)
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+              if ((!a || vi1) && (!b || vj1)) Ti[(i0 + a) * s + j0 + b] = dv[a][b];
+      } else if (ss <= GJR * (int)blockDim.x) {
+        // each thread owns up to GJR entries of S for the whole elimination:
+        // only the pivot row, column and inverse pass through LDS
+        B4<T> dv[GJR];
+        int di[GJR], dj[GJR];
 #pragma unroll
         for (int m = 0; m < GJR; m++) {
-          const int i = di[m], j = dj[m];
-          if (i < 0) continue;
-          B4<T> v;
-          if (i == k) {
-            v = (j == k) ? Pk : blk_mul(Pk, rowb[cur][j]);
-          } else if (j == k) {
-            v = blk_zero<T>();
-            blk_fms(v, colb[cur][i], Pk);
-          } else {
-            v = dv[m];
-            blk_fms(v, colb[cur][i], blk_mul(Pk, rowb[cur][j]));
-          }
-          dv[m] = v;
-          if (j == k + 1) colb[nxt][i] = v;
-          if (i == k + 1) rowb[nxt][j] = v;
-          if (i == k + 1 && j == k + 1) {
-            B4<T> iv;
-            if (blk_inv(v, iv)) sing = 1;
-            pvb[nxt] = iv;
-          }
+          const int q = threadIdx.x + m * blockDim.x;
+          di[m] = q < ss ? q / s : -1;
+          dj[m] = q < ss ? q - di[m] * s : -1;
+          dv[m] = q < ss ? D[q] : blk_zero<T>();
         }
-        __syncthreads();
-      }
+        for (int k = 0; k < s; k++) {
+          const int cur = k & 1, nxt = cur ^ 1;
+          const B4<T> Pk = pvb[cur];
 #pragma unroll
-      for (int m = 0; m < GJR; m++)
-        if (di[m] >= 0) Ti[threadIdx.x + m * blockDim.x] = dv[m];
-    } else {
-      // more entries than registers allow: S stays in LDS
-      for (int k = 0; k < s; k++) {
-        const int cur = k & 1, nxt = cur ^ 1;
-        const B4<T> Pk = pvb[cur];
-        for (int q = threadIdx.x; q < ss; q += blockDim.x) {
-          const int i = q / s, j = q - i * s;
-          B4<T> v;
-          if (i == k) {
-            v = (j == k) ? Pk : blk_mul(Pk, rowb[cur][j]);
-          } else if (j == k) {
-            v = blk_zero<T>();
-            blk_fms(v, colb[cur][i], Pk);
-          } else {
-            v = D[q];
-            blk_fms(v, colb[cur][i], blk_mul(Pk, rowb[cur][j]));
+          for (int m = 0; m < GJR; m++) {
+            const int i = di[m], j = dj[m];
+            if (i < 0) continue;
+            B4<T> v;
+            if (i == k) {
+              v = (j == k) ? Pk : blk_mul(Pk, rowb[cur][j]);
+            } else if (j == k) {
+              v = blk_zero<T>();
+              blk_fms(v, colb[cur][i], Pk);
+            } else {
+              v = dv[m];
+              blk_fms(v, colb[cur][i], blk_mul(Pk, rowb[cur][j]));
+            }
+            dv[m] = v;
+            if (j == k + 1) colb[nxt][i] = v;
+            if (i == k + 1) rowb[nxt][j] = v;
+            if (i == k + 1 && j == k + 1) {
+              B4<T> iv;
+              if (blk_inv(v, iv)) sing = 1;
+              pvb[nxt] = iv;
+            }
           }
-          D[q] = v;
-          if (j == k + 1) colb[nxt][i] = v;
-          if (i == k + 1) rowb[nxt][j] = v;
-          if (i == k + 1 && j == k + 1) {
-            B4<T> iv;
-            if (blk_inv(v, iv)) sing = 1;
-            pvb[nxt] = iv;
-          }
+          __syncthreads();
         }
-        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < GJR; m++)
+          if (di[m] >= 0) Ti[threadIdx.x + m * blockDim.x] = dv[m];
+      } else {
+        // more entries than registers allow: S stays in LDS
+        for (int k = 0; k < s; k++) {
+          const int cur = k & 1, nxt = cur ^ 1;
+          const B4<T> Pk = pvb[cur];
+          for (int q = threadIdx.x; q < ss; q += blockDim.x) {
+            const int i = q / s, j = q - i * s;
+            B4<T> v;
+            if (i == k) {
+              v = (j == k) ? Pk : blk_mul(Pk, rowb[cur][j]);
+            } else if (j == k) {
+              v = blk_zero<T>();
+              blk_fms(v, colb[cur][i], Pk);
+            } else {
+              v = D[q];
+              blk_fms(v, colb[cur][i], blk_mul(Pk, rowb[cur][j]));
+            }
+            D[q] = v;
+            if (j == k + 1) colb[nxt][i] = v;
+            if (i == k + 1) rowb[nxt][j] = v;
+            if (i == k + 1 && j == k + 1) {
+              B4<T> iv;
+              if (blk_inv(v, iv)) sing = 1;
+              pvb[nxt] = iv;
+            }
+          }
+          __syncthreads();
+        }
+        for (int q = threadIdx.x; q < ss; q += blockDim.x) Ti[q] = D[q];
       }
-      for (int q = threadIdx.x; q < ss; q += blockDim.x) Ti[q] = D[q];
     }
   }
   if (threadIdx.x == 0 && sing) {
@@ -1307,19 +1477,6 @@ __device__ __forceinline__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti,
 // chunk ahead); a wave skips the slot blocks it has no lane in with a scalar
 // branch, so load counts vary and one explicit wait at the top of each chunk
 // receives everything the previous chunk requested.
-template <int CTRL>
-__device__ __forceinline__ float dpp_row(float v) {
-  return __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_row(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
 // sum of the lanes of each aligned group of 2^lg lanes into the group's
 // first lane (lane i adds lane i + 2^s for s < lg); glg = the largest lg of
 // the wave's groups.  Every lane of the wave must take part.
