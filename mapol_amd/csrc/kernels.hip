@@ -482,6 +482,31 @@ __device__ __forceinline__ double dpp_row(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// the value of lane i + 16 (lanes 0-15 and 32-47) and of lane i + 32 (lanes
+// 0-31), by gfx950's lane swaps within one register (no LDS round trip)
+__device__ __forceinline__ float lane_up16(float v) {
+  const unsigned u = __float_as_uint(v);
+  return __uint_as_float(__builtin_amdgcn_permlane16_swap(u, u, false, false)[1]);
+}
+__device__ __forceinline__ float lane_up32(float v) {
+  const unsigned u = __float_as_uint(v);
+  return __uint_as_float(__builtin_amdgcn_permlane32_swap(u, u, false, false)[1]);
+}
+__device__ __forceinline__ double lane_up16(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+  const unsigned l2 = __builtin_amdgcn_permlane16_swap(lo, lo, false, false)[1];
+  const unsigned h2 = __builtin_amdgcn_permlane16_swap(hi, hi, false, false)[1];
+  return __longlong_as_double(((long long)h2 << 32) | l2);
+}
+__device__ __forceinline__ double lane_up32(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+  const unsigned l2 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false)[1];
+  const unsigned h2 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false)[1];
+  return __longlong_as_double(((long long)h2 << 32) | l2);
+}
+
 // sum of each aligned group of gsz lanes (a power of two, the same for the
 // whole wave) into the group's first lane: lane i adds lane i + 2^s for
 // 2^s < gsz, by DPP row shifts within 16 lanes and cross-lane reads beyond.
@@ -505,12 +530,12 @@ __device__ __forceinline__ void group_sum4_first(B4<T> &v, int gsz) {
     v.c += dpp_row<0x108>(v.c); v.d += dpp_row<0x108>(v.d);
   }
   if (gsz > 16) {
-    v.a += __shfl_down(v.a, 16, 64); v.b += __shfl_down(v.b, 16, 64);
-    v.c += __shfl_down(v.c, 16, 64); v.d += __shfl_down(v.d, 16, 64);
+    v.a += lane_up16(v.a); v.b += lane_up16(v.b);
+    v.c += lane_up16(v.c); v.d += lane_up16(v.d);
   }
   if (gsz > 32) {
-    v.a += __shfl_down(v.a, 32, 64); v.b += __shfl_down(v.b, 32, 64);
-    v.c += __shfl_down(v.c, 32, 64); v.d += __shfl_down(v.d, 32, 64);
+    v.a += lane_up32(v.a); v.b += lane_up32(v.b);
+    v.c += lane_up32(v.c); v.d += lane_up32(v.d);
   }
 }
 
@@ -1478,35 +1503,41 @@ __device__ __forceinline__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti,
 // branch, so load counts vary and one explicit wait at the top of each chunk
 // receives everything the previous chunk requested.
 // sum of the lanes of each aligned group of 2^lg lanes into the group's
-// first lane (lane i adds lane i + 2^s for s < lg); glg = the largest lg of
-// the wave's groups.  Every lane of the wave must take part.
+// first lane.  An unconditional tree (step s: lane i adds lane i + 2^s) leaves
+// in the first lane of every aligned window of 2^(s+1) lanes that window's
+// sum, whatever the groups; a group's first lane keeps the value after step
+// lg - 1, its own window.  glg = the largest lg of the chunk.  Every lane of
+// the wave must take part.
 template <typename T>
 __device__ __forceinline__ void slot_group_sum(T &p0, T &p1, int lg, int glg) {
+  T r0 = p0, r1 = p1;
   // DPP row_shl:n (0x100 + n): lane i reads lane i + n of its 16-lane row
   if (glg > 0) {
-    const T a = dpp_row<0x101>(p0), b = dpp_row<0x101>(p1);
-    if (lg > 0) { p0 += a; p1 += b; }
+    p0 += dpp_row<0x101>(p0); p1 += dpp_row<0x101>(p1);
+    r0 = lg == 1 ? p0 : r0; r1 = lg == 1 ? p1 : r1;
   }
   if (glg > 1) {
-    const T a = dpp_row<0x102>(p0), b = dpp_row<0x102>(p1);
-    if (lg > 1) { p0 += a; p1 += b; }
+    p0 += dpp_row<0x102>(p0); p1 += dpp_row<0x102>(p1);
+    r0 = lg == 2 ? p0 : r0; r1 = lg == 2 ? p1 : r1;
   }
   if (glg > 2) {
-    const T a = dpp_row<0x104>(p0), b = dpp_row<0x104>(p1);
-    if (lg > 2) { p0 += a; p1 += b; }
+    p0 += dpp_row<0x104>(p0); p1 += dpp_row<0x104>(p1);
+    r0 = lg == 3 ? p0 : r0; r1 = lg == 3 ? p1 : r1;
   }
   if (glg > 3) {
-    const T a = dpp_row<0x108>(p0), b = dpp_row<0x108>(p1);
-    if (lg > 3) { p0 += a; p1 += b; }
+    p0 += dpp_row<0x108>(p0); p1 += dpp_row<0x108>(p1);
+    r0 = lg == 4 ? p0 : r0; r1 = lg == 4 ? p1 : r1;
   }
   if (glg > 4) {
-    const T a = __shfl_down(p0, 16, 64), b = __shfl_down(p1, 16, 64);
-    if (lg > 4) { p0 += a; p1 += b; }
+    p0 += lane_up16(p0); p1 += lane_up16(p1);
+    r0 = lg == 5 ? p0 : r0; r1 = lg == 5 ? p1 : r1;
   }
   if (glg > 5) {
-    const T a = __shfl_down(p0, 32, 64), b = __shfl_down(p1, 32, 64);
-    if (lg > 5) { p0 += a; p1 += b; }
+    p0 += lane_up32(p0); p1 += lane_up32(p1);
+    r0 = lg == 6 ? p0 : r0; r1 = lg == 6 ? p1 : r1;
   }
+  p0 = r0;
+  p1 = r1;
 }
 
 template <int SP>
@@ -1576,28 +1607,38 @@ __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
   m.x = __builtin_amdgcn_readfirstlane(m.x);
   m.y = __builtin_amdgcn_readfirstlane(m.y);
   m.z = __builtin_amdgcn_readfirstlane(m.z);
-  T p0[SP], p1[SP];
-  int lead[SP], lg[SP];
+  const int4 mn = swc[c_next], ma = swc[c_after];
+  // the fields of this chunk's slots, so that the slots of chunk c + 2 can
+  // be requested into the same registers at once (a load into registers
+  // still in use would be waited for), then the blocks of chunk c + 1: both
+  // have the whole chunk to arrive
+  int col[SP], lead[SP], lg[SP];
 #pragma unroll
   for (int k = 0; k < SP; k++) {
     const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
     if (wave_any(q, m.y)) {
-      int4 sd = cur.s[k];
-      sd.y = pinned(sd.y);
-      sd.z = pinned(sd.z);
-      const int colp = sd.y & ((1 << SLOT_LG_SHIFT) - 1);
-      const int col = max(colp - 1, 0);
-      const T y0 = X[2 * col], y1 = X[2 * col + 1];
-      const B4<T> &a = val.a[k];
-      const bool ent = colp > 0 && q < m.y;
-      p0[k] = ent ? a.a * y0 + a.b * y1 : (T)0;
-      p1[k] = ent ? a.c * y0 + a.d * y1 : (T)0;
-      lead[k] = q < m.y ? sd.z : -1;
-      lg[k] = sd.y >> SLOT_LG_SHIFT;
+      const int y = pinned(cur.s[k].y), z = pinned(cur.s[k].z);
+      const int colp = y & ((1 << SLOT_LG_SHIFT) - 1);
+      col[k] = q < m.y ? colp - 1 : -1;
+      lead[k] = q < m.y ? z : -1;
+      lg[k] = y >> SLOT_LG_SHIFT;
     }
   }
-  const int4 mn = swc[c_next];
   nval = slot_values<T, SP>(A, mn, next_bw, nxt);
+  cur = slot_index<SP>(slot, ma);
+  T p0[SP], p1[SP];
+#pragma unroll
+  for (int k = 0; k < SP; k++) {
+    const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
+    if (wave_any(q, m.y)) {
+      const int cc = max(col[k], 0);
+      const T y0 = X[2 * cc], y1 = X[2 * cc + 1];
+      const B4<T> &a = val.a[k];
+      const bool ent = col[k] >= 0;
+      p0[k] = ent ? a.a * y0 + a.b * y1 : (T)0;
+      p1[k] = ent ? a.c * y0 + a.d * y1 : (T)0;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < SP; k++) {
     const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
@@ -1617,9 +1658,6 @@ __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
       }
     }
   }
-  // the slots of chunk c + 2 replace those of chunk c, which are no longer
-  // read (a load into registers still in use would be waited for at once)
-  cur = slot_index<SP>(slot, swc[c_after]);
   __syncthreads();
 }
 
