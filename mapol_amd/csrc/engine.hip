@@ -635,6 +635,10 @@ static void launch_iteration(bpf_engine *e) {
 }
 
 
+// Newton iterations per captured graph (each replay is followed by one read
+// of the active-instance count)
+constexpr int NEWTON_PER_GRAPH = 4;
+
 static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
   bpf_status s = ensure_lu(e, e->opt.precision == BPF_FP64 ? 1 : 0);
   if (s) return s;
@@ -649,7 +653,7 @@ static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
     hipGraph_t gr;
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     e->capturing = true;
-    launch_iteration(e);
+    for (int j = 0; j < NEWTON_PER_GRAPH; j++) launch_iteration(e);
     e->capturing = false;
     HIPCHK(hipStreamEndCapture(e->stream, &gr));
     hipError_t r = hipGraphInstantiate(&e->graph, gr, nullptr, nullptr, 0);
@@ -671,7 +675,12 @@ static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
   if (stale != hipSuccess && e->graph_err.empty())
     e->graph_err = std::string("error pending before the Newton loop: ") +
                    hipGetErrorName(stale);
-  for (int it = 0; it < cap; it++) {
+  // The graph holds NEWTON_PER_GRAPH iterations and the active count is read
+  // back once per replay: every kernel of an iteration returns at once for a
+  // finished instance, so the iterations replayed past the last one change
+  // nothing (the count stays 0), and the host waits once per replay.
+  const int ahead = use_graph ? NEWTON_PER_GRAPH : 1;
+  for (int it = 0; it < cap; it += ahead) {
     hipError_t r = hipSuccess;
     const char *where = "";
     if (use_graph) {
