@@ -242,3 +242,49 @@ def test_sweep_forms_match_oracle(monkeypatch, form, threads):
             assert r["iterations"][t] == ref[t]["iterations"], t
             if ref[t]["converged"]:
                 assert np.abs(r["V"][t] - ref[t]["V"]).max() <= DV_TOL
+
+
+@pytest.mark.parametrize("lu", [1024, 512, 256])
+def test_factorization_workgroup_sizes_match_oracle(lu):
+    """Every workgroup size of the factorization: 1024 lanes invert the
+    61-pivot dense tail by pivot pairs in 2x2 tiles (FP32) or one pivot per
+    step (FP64); 512 and 256 lanes keep several entries per thread.  Same
+    iterations, verdicts and voltages as the oracle."""
+    print(f"[factor] lu threads={lu}", flush=True)
+    case, cont = synth.workload("pf9241x200")
+    T = 4
+    orc = oracle.Oracle(case)
+    ref = [orc.solve(oracle.Oracle.opts(enforce_qlim=True), br_on=cont.br_on[t],
+                     gen_on=cont.gen_on[t]) for t in range(T)]
+    for precision in (BPF_FP64, BPF_MIXED):
+        e = Engine(case, precision=precision, enforce_qlim=True)
+        e.set_launch(lu=lu)
+        e.set_instances(br_on=cont.br_on[:T], gen_on=cont.gen_on[:T])
+        e.solve()
+        r = e.results()
+        e.close()
+        for t in range(T):
+            assert r["verdict"][t] == ref[t]["verdict"], t
+            assert r["iterations"][t] == ref[t]["iterations"], t
+            if ref[t]["converged"]:
+                assert np.abs(r["V"][t] - ref[t]["V"]).max() <= DV_TOL
+
+
+def test_graph_replay_matches_direct_launches():
+    """The replayed graph (four Newton iterations per replay, the active
+    count read once per replay; iterations past the last are empty) gives
+    bit for bit what direct launches, one iteration at a time, give."""
+    case, cont = synth.workload("pf1354x200")
+    T = 24
+    out = []
+    for graph in (True, False):
+        e = Engine(case, precision=BPF_MIXED, enforce_qlim=True, use_graph=graph)
+        e.set_instances(br_on=cont.br_on[:T], gen_on=cont.gen_on[:T])
+        e.solve()
+        assert e.info()["graph_used"] == (1 if graph else 0)
+        out.append(e.results())
+        e.close()
+    a, b = out
+    np.testing.assert_array_equal(a["iterations"], b["iterations"])
+    np.testing.assert_array_equal(a["verdict"], b["verdict"])
+    np.testing.assert_array_equal(a["V"], b["V"])
