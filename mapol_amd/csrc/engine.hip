@@ -557,7 +557,8 @@ static void mismatch_pair(bpf_engine *e, int mode, int maxit, int qlim,
   hipLaunchKernelGGL(k_mismatch_mb, xgrid(e->T, e->n, e->mis_threads),
                      dim3(e->mis_threads), 0, e->stream, e->g, e->d, mode,
                      fd_norm, e->opt.qlim_tol, qlim);
-  hipLaunchKernelGGL(k_verdict, dim3(e->T), dim3(256), 0, e->stream, e->g, e->d,
+  // 1024 lanes: the reactive-limit switching walks every bus of an instance
+  hipLaunchKernelGGL(k_verdict, dim3(e->T), dim3(1024), 0, e->stream, e->g, e->d,
                      (int)nparts, mode, maxit, e->opt.tol, qlim,
                      e->opt.max_qlim_rounds, e->opt.qlim_tol, fd_norm);
 }
