@@ -544,6 +544,26 @@ __device__ __forceinline__ void blk_acc(B4<T> &acc, const B4<T> &x) {
   acc.a += x.a; acc.b += x.b; acc.c += x.c; acc.d += x.d;
 }
 
+// a lane's share of a product range in LDS (q0 + lane, step gsz): four
+// independent partial sums, so that four LDS reads are in flight at a time
+// (a single running sum waits for every read), combined pairwise at the end
+template <typename T>
+__device__ __forceinline__ B4<T> lane_range_sum(const B4<T> *buf, int q, int q1,
+                                                int gsz) {
+  B4<T> e0 = {0, 0, 0, 0}, e1 = e0, e2 = e0, e3 = e0;
+  for (; q + 3 * gsz < q1; q += 4 * gsz) {
+    blk_acc(e0, buf[q]);
+    blk_acc(e1, buf[q + gsz]);
+    blk_acc(e2, buf[q + 2 * gsz]);
+    blk_acc(e3, buf[q + 3 * gsz]);
+  }
+  for (; q < q1; q += gsz) blk_acc(e0, buf[q]);
+  blk_acc(e0, e1);
+  blk_acc(e2, e3);
+  blk_acc(e0, e2);
+  return e0;
+}
+
 template <typename T>
 __device__ __forceinline__ B4<T> blk_zero() {
   B4<T> z;
@@ -811,12 +831,10 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       for (int u = 0; u < NB; u++) {
         if (kb + u * stride >= k1) break;
         const int p = ta[u].x & TASK_POS_MASK;
-        B4<T> e = blk_zero<T>();
-        for (int q = ta[u].y + lane; q < ta[u].z; q += gsz) blk_acc(e, buf[q - q0]);
+        B4<T> e = lane_range_sum(buf, ta[u].y + lane - q0, ta[u].z - q0, gsz);
         group_sum4_first(e, gsz);
         if (ta[u].x & TASK_LCOL) {
-          B4<T> pv = blk_zero<T>();
-          for (int q = tb[u].y + lane; q < tb[u].z; q += gsz) blk_acc(pv, buf[q - q0]);
+          B4<T> pv = lane_range_sum(buf, tb[u].y + lane - q0, tb[u].z - q0, gsz);
           group_sum4_first(pv, gsz);
           if (lane == 0) {
             B4<T> inv;
