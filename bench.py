@@ -33,7 +33,9 @@ def log(*a):
 
 
 def algorithmic_bytes(info, case, T, its_total, s, refine):
-    """Algorithmic HBM bytes per kernel over one solve.
+    """Algorithmic HBM bytes per kernel over one solve: the compulsory bytes
+    of one instance-launch (every array a kernel must read or write, once)
+    times the launches.
 
     ``its_total`` = sum over instances of Newton iterations (instance-
     iterations); ``s`` = bytes per factor scalar (4 mixed, 8 FP64)."""
@@ -170,8 +172,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: contingencies are independent "
-              "and would shard across GPUs with no exchange during Newton "
-              "(see DESIGN.md); multi-GPU runs are not part of this build.",
+              "and would shard across GPUs with no exchange during the Newton "
+              "iterations; runs on several GPUs are not supported.",
               file=sys.stderr)
         sys.exit(2)
 
