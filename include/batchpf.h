@@ -251,12 +251,14 @@ bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
  * reference solves with a library, newtonpf.h:312-363).  The same levels as
  * fw/bw_rows; each row owns a lane group of 2^g lanes (the power of two at
  * least its entry count, at most 64) that starts at a multiple of its size,
- * one lane per entry.  sl_chunk[4*(slot_fw_chunks+slot_bw_chunks)] = {first
- * slot, end slot, largest g, 0}, at most 2048 slots each;
- * sl_slot[4*slots] = {L or U position (padding: the row's first entry),
- * column block + 1 (0: padding) | g << 27, block row (a group's first slot)
- * or -1, pivot position (a group's first slot, backward) or the position}.
- * slots = 0 when a row has more than 64 entries.  Arrays may be NULL. */
+ * one lane per entry; a row of more than 64 entries comes in segments of 64
+ * in successive chunks of its level.  sl_chunk[4*(slot_fw_chunks+
+ * slot_bw_chunks)] = {first slot, end slot, largest g, 0}, at most 2048
+ * slots each; sl_slot[4*slots] = {L or U position (padding: the segment's
+ * first entry), column block + 1 (0: padding) | g << 27, block row (a
+ * group's first slot; | 1 << 30 for a segment before the row's last) or -1,
+ * pivot position (first slot of a row's last segment, backward) or the
+ * position}.  Arrays may be NULL. */
 bpf_status bpf_analyse_slots(const bpf_case *c, bpf_info *info,
                              int32_t *sl_chunk, int32_t *sl_slot);
 

@@ -1662,10 +1662,12 @@ __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
     const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
     if (wave_any(q, m.y)) {
       slot_group_sum(p0[k], p1[k], lg[k], m.z);
-      const int i = lead[k];
-      if (i >= 0) {
+      if (lead[k] >= 0) {
+        // a row's earlier segments only subtract; its last one (or its only
+        // one) also applies the pivot inverse in the backward sweep
+        const int i = lead[k] & (SLOT_NOT_LAST - 1);
         const T x0 = X[2 * i] - p0[k], x1 = X[2 * i + 1] - p1[k];
-        if (bw) {
+        if (bw && !(lead[k] & SLOT_NOT_LAST)) {
           const B4<T> &D = val.d[k];
           X[2 * i] = D.a * x0 + D.b * x1;
           X[2 * i + 1] = D.c * x0 + D.d * x1;

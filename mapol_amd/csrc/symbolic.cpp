@@ -573,49 +573,70 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   for (size_t c = 0; c < S.sw_cmax.size(); c++)
     if (S.sw_chunk[4 * c + 3] - S.sw_chunk[4 * c + 2] > SWEEP_CHUNK) S.sw_ok = 0;
   // slot form: rows of a level (longest first, so lane groups shrink and
-  // stay aligned) packed into chunks of at most SLOT_CHUNK slots
-  bool slot_fit = true;
+  // stay aligned) packed into chunks of at most SLOT_CHUNK slots.  A row of
+  // more than 64 entries is cut into segments of 64: segment j of every
+  // such row goes into the chunks of pass j of the level (a new chunk starts
+  // with each pass, so no chunk holds two segments of one row); each
+  // segment's first lane subtracts its partial sum from x_i, and only the
+  // last one applies the pivot inverse (backward).
+  struct Piece { int row, p0, p1, last; };
   auto slots = [&](const std::vector<int> &ptr, const std::vector<int> &rows,
                    int nl, bool fw) {
     int nch = 0;
     for (int l = 0; l < nl; l++) {
-      int r = ptr[l];
-      while (r < ptr[l + 1]) {
-        const int s0 = (int)S.sl_slot.size() / 4;
-        int glg = -1;
-        while (r < ptr[l + 1]) {
-          const int i = rows[r];
-          const int p0 = fw ? S.lrow[i] : S.ldiag[i] + 1;
-          const int p1 = fw ? S.ldiag[i] : S.lrow[i + 1];
-          if (fw && p1 == p0) { r++; continue; }
-          if (p1 - p0 > 64) { slot_fit = false; r++; continue; }
-          int lg = 0;
-          while ((1 << lg) < p1 - p0) lg++;
-          const int ns = (int)S.sl_slot.size() / 4;
-          if (ns > s0 && ns - s0 + (1 << lg) > SLOT_CHUNK) break;
-          const int first = p1 > p0 ? p0 : std::max(S.ldiag[i], 0);
-          for (int k = 0; k < (1 << lg); k++) {
-            const int p = p0 + k;
-            const bool ent = p < p1;
-            const int pos = ent ? p : first;
-            S.sl_slot.insert(S.sl_slot.end(),
-                             {pos, (ent ? S.lcol[p] + 1 : 0) | (lg << SLOT_LG_SHIFT),
-                              k == 0 ? i : -1, (k == 0 && !fw) ? S.ldiag[i] : pos});
+      std::vector<std::vector<Piece>> pass;
+      for (int r = ptr[l]; r < ptr[l + 1]; r++) {
+        const int i = rows[r];
+        const int p0 = fw ? S.lrow[i] : S.ldiag[i] + 1;
+        const int p1 = fw ? S.ldiag[i] : S.lrow[i + 1];
+        if (fw && p1 == p0) continue;   // x = b: nothing to do
+        const int nseg = std::max(1, (p1 - p0 + 63) / 64);
+        if ((int)pass.size() < nseg) pass.resize(nseg);
+        for (int j = 0; j < nseg; j++)
+          pass[j].push_back({i, p0 + 64 * j, std::min(p0 + 64 * (j + 1), p1),
+                             j == nseg - 1 ? 1 : 0});
+      }
+      for (auto &pieces : pass) {
+        std::stable_sort(pieces.begin(), pieces.end(), [](const Piece &a, const Piece &b) {
+          return a.p1 - a.p0 > b.p1 - b.p0;
+        });
+        size_t r = 0;
+        while (r < pieces.size()) {
+          const int s0 = (int)S.sl_slot.size() / 4;
+          int glg = -1;
+          while (r < pieces.size()) {
+            const Piece &pc = pieces[r];
+            const int i = pc.row, len = pc.p1 - pc.p0;
+            int lg = 0;
+            while ((1 << lg) < len) lg++;
+            const int ns = (int)S.sl_slot.size() / 4;
+            if (ns > s0 && ns - s0 + (1 << lg) > SLOT_CHUNK) break;
+            const int first = len > 0 ? pc.p0 : std::max(S.ldiag[i], 0);
+            const int lead = i | (pc.last ? 0 : SLOT_NOT_LAST);
+            for (int k = 0; k < (1 << lg); k++) {
+              const int p = pc.p0 + k;
+              const bool ent = p < pc.p1;
+              const int pos = ent ? p : first;
+              S.sl_slot.insert(S.sl_slot.end(),
+                               {pos, (ent ? S.lcol[p] + 1 : 0) | (lg << SLOT_LG_SHIFT),
+                                k == 0 ? lead : -1,
+                                (k == 0 && !fw && pc.last) ? S.ldiag[i] : pos});
+            }
+            glg = std::max(glg, lg);
+            r++;
           }
-          glg = std::max(glg, lg);
-          r++;
+          const int s1 = (int)S.sl_slot.size() / 4;
+          if (s1 == s0) continue;
+          S.sl_chunk.insert(S.sl_chunk.end(), {s0, s1, glg, 0});
+          nch++;
         }
-        const int s1 = (int)S.sl_slot.size() / 4;
-        if (s1 == s0) continue;
-        S.sl_chunk.insert(S.sl_chunk.end(), {s0, s1, glg, 0});
-        nch++;
       }
     }
     return nch;
   };
   S.sl_nfw = slots(S.fw_ptr, S.fw_rows, S.nfw, true);
   S.sl_nbw = slots(S.bw_ptr, S.bw_rows, S.nbw, false);
-  S.sl_ok = slot_fit && S.sl_nfw + S.sl_nbw <= MAX_CHUNKS;
+  S.sl_ok = S.sl_nfw + S.sl_nbw <= MAX_CHUNKS;
   return true;
 }
 

@@ -20,6 +20,7 @@ constexpr int CHUNK_PAIRS = 4096;    // products per factorization chunk (LDS)
 constexpr int SWEEP_CHUNK = 2048;    // entries per sweep chunk (LDS)
 constexpr int SLOT_CHUNK = 2048;     // lane slots per sweep chunk, slot form
 constexpr int SLOT_LG_SHIFT = 27;    // slot column word: column + 1 | log2 group << 27
+constexpr int SLOT_NOT_LAST = 1 << 30;  // slot row word: a segment before the row's last
 constexpr int MAX_CHUNKS = 512;      // chunk tables the kernels keep in LDS
 constexpr int TAIL_MAX = 64;         // dense tail: at most 64 blocks square
 // FP32 factorization: distinct blocks of the tail products staged in LDS
@@ -103,13 +104,16 @@ struct Symbolic {
   // slot form of the same sweeps: every row of a chunk owns a lane group
   // (the power of two at least its length, at most 64 lanes), groups packed
   // largest first so each starts at a multiple of its size; one slot per
-  // entry, padding slots carry none.  A chunk holds at most SLOT_CHUNK
+  // entry, padding slots carry none.  A row of more than 64 entries comes in
+  // segments of 64 in successive chunks of its level (all but the last
+  // marked SLOT_NOT_LAST in the row word).  A chunk holds at most SLOT_CHUNK
   // slots; chunks 0 .. sl_nfw-1 are the forward sweep.
   int sl_nfw = 0, sl_nbw = 0, sl_ok = 0;
   std::vector<int> sl_chunk;   // {first slot, end slot, log2 of the largest group, 0}
   // per slot {position (padding: the row's first entry), column + 1 (0:
   // padding) | log2 group << SLOT_LG_SHIFT, block row (the group's first
-  // slot) or -1, pivot position (first slot, backward) or the position}
+  // slot; | SLOT_NOT_LAST for an earlier segment) or -1, pivot position
+  // (first slot of a row's last segment, backward) or the position}
   std::vector<int> sl_slot;
 };
 

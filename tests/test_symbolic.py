@@ -14,6 +14,7 @@ import oracle
 from mapol_amd import engine, synth
 
 HAS_A, DIAG, LCOL, MASK = 1 << 30, 1 << 29, 1 << 28, (1 << 28) - 1
+NOT_LAST = 1 << 30      # slot row word: a segment before the row's last
 
 pytestmark = pytest.mark.skipif(not __import__("os").path.exists(engine.LIB_PATH),
                                 reason="libbatchpf.so not built")
@@ -225,21 +226,23 @@ def _sweeps_slot(a, A, Sinv, b):
                          for p, cp in zip(sl[s0:s1, 0], colp)])
         q = 0
         while q < s1 - s0:
-            pos, w, row, piv = sl[s0 + q]
+            pos, w, lead, piv = sl[s0 + q]
             g = w >> LG
+            row, last = lead & (NOT_LAST - 1), not lead & NOT_LAST
             v = prod[q:q + (1 << g)]
             while len(v) > 1:
                 v = v[0::2] + v[1::2]
             xi = x[row] - v[0]
-            x[row] = A[piv] @ xi if c >= nfw else xi
+            x[row] = A[piv] @ xi if c >= nfw and last else xi
             q += 1 << g
     if nfw == len(ch):
         tail()
     return x.ravel()
 
 
-def _system(name, seed=0):
-    case, cont = synth.workload(name)
+def _system(name, seed=0, case=None):
+    if case is None:
+        case, cont = synth.workload(name)
     a = engine.analyse(case)
     orc = oracle.Oracle(case)
     V = orc.initial_voltage() * (1 + 0.01 * np.random.default_rng(seed)
@@ -444,6 +447,10 @@ def test_slot_chunks_cover_the_levels(name):
     then padding slots; at most 2048 slots per chunk"""
     case, _ = synth.workload(name)
     a = engine.analyse(case)
+    _check_slots(a, a["info"]["tail_n"])
+
+
+def _check_slots(a, tail_n):
     i = a["info"]
     rp, ci = a["lu_rowptr"], a["lu_colind"]
     diag = _diag(a)
@@ -453,29 +460,74 @@ def test_slot_chunks_cover_the_levels(name):
     assert np.all(ch[:, 1] - ch[:, 0] <= 2048) and np.all(ch[:, 1] > ch[:, 0])
     assert np.array_equal(ch[1:, 0], ch[:-1, 1]) and ch[0, 0] == 0 and ch[-1, 1] == len(sl)
     LG = 27
+    long_rows = 0
     for lo, hi, order, fw in ((0, nfw, a["fw_rows"], True),
                               (nfw, len(ch), a["bw_rows"], False)):
         if fw:
             order = np.array([r for r in order if diag[r] > rp[r]])
-        seen = []
+        got = {}            # row -> its entries so far, in segment order
+        done = []           # rows in the order their last segment comes
         for c in range(lo, hi):
             s0, s1 = ch[c, 0], ch[c, 1]
-            q, gmax = s0, 0
+            q, gmax, rows_here = s0, 0, set()
             while q < s1:
-                pos, w, row, piv = sl[q]
+                pos, w, lead, piv = sl[q]
                 g = w >> LG
                 gmax = max(gmax, g)
-                assert row >= 0 and (q - s0) % (1 << g) == 0 and q + (1 << g) <= s1
-                p0, p1 = (rp[row], diag[row]) if fw else (diag[row] + 1, rp[row + 1])
-                assert (1 << g) >= p1 - p0 and (g == 0 or (1 << (g - 1)) < p1 - p0)
+                row, last = lead & (NOT_LAST - 1), not lead & NOT_LAST
+                assert lead >= 0 and (q - s0) % (1 << g) == 0 and q + (1 << g) <= s1
+                assert row not in rows_here         # one segment of a row per chunk
+                rows_here.add(row)
                 grp = sl[q:q + (1 << g)]
+                colp = grp[:, 1] & ((1 << LG) - 1)
+                n = int((colp > 0).sum())
+                assert np.all(colp[:n] > 0) and np.all(colp[n:] == 0)
+                assert (1 << g) >= n and (g == 0 or (1 << (g - 1)) < n) and n <= 64
                 assert np.all(grp[:, 1] >> LG == g) and np.all(grp[1:, 2] == -1)
-                n = p1 - p0
-                assert np.array_equal(grp[:n, 0], np.arange(p0, p1))
-                assert np.array_equal((grp[:n, 1] & ((1 << LG) - 1)) - 1, ci[p0:p1])
-                assert np.all(grp[n:, 1] & ((1 << LG) - 1) == 0)
-                assert piv == (diag[row] if not fw else pos)
-                seen.append(row)
+                got.setdefault(row, ([], []))
+                got[row][0].extend(grp[:n, 0])
+                got[row][1].extend(colp[:n] - 1)
+                assert piv == (diag[row] if not fw and last else pos)
+                if last:
+                    p0, p1 = (rp[row], diag[row]) if fw else (diag[row] + 1, rp[row + 1])
+                    assert np.array_equal(got[row][0], np.arange(p0, p1))
+                    assert np.array_equal(got[row][1], ci[p0:p1])
+                    long_rows += p1 - p0 > 64
+                    done.append(row)
                 q += 1 << g
             assert ch[c, 2] == gmax
-        assert np.array_equal(np.array(seen, int), np.asarray(order, int))
+        # every row once, in level order (rows of one level may come in any
+        # order of their last segments)
+        assert sorted(done) == sorted(order.tolist())
+        lv = {}
+        for l, (b0, b1) in enumerate(zip(*(lambda p: (p[:-1], p[1:]))(a["fw_ptr" if fw else "bw_ptr"]))):
+            for r in (a["fw_rows"] if fw else a["bw_rows"])[b0:b1]:
+                lv[int(r)] = l
+        levels = [lv[r] for r in done]
+        assert levels == sorted(levels)
+    return long_rows
+
+
+def test_slot_form_splits_long_rows():
+    """case9241pegase has rows of more than 64 entries below its dense tail:
+    they come in segments of 64, one per chunk, the last one applying the
+    pivot inverse; the slot-form replay gives the sparse direct solve's step"""
+    import os
+    from mapol_amd.engine import load_matpower
+    from tests.test_matpower import write_case
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "matpower_cases.npz"))
+    import tempfile
+    d = tempfile.mkdtemp()
+    path = os.path.join(d, "case9241pegase.m")
+    write_case(path, float(z["case9241pegase/baseMVA"]), z["case9241pegase/bus"],
+               z["case9241pegase/gen"], z["case9241pegase/branch"])
+    case = load_matpower(path)[0]
+    case, a, J, F, A, rhs, rowmap = _system(None, case=case)
+    assert _check_slots(a, a["info"]["tail_n"]) > 0
+    LU, Sinv = _crout(a, A)
+    x = _sweeps(a, LU, Sinv, rhs)
+    xs = _sweeps_slot(a, LU, Sinv, rhs)
+    assert np.abs(xs - x).max() <= 1e-11 * max(1, np.abs(x).max())
+    ref = sla.spsolve(J.tocsc(), F)
+    got = np.array([x[2 * b + c] for b, c in rowmap])
+    assert np.abs(got - ref).max() <= 1e-9 * max(1, np.abs(ref).max())
