@@ -594,7 +594,7 @@ static void launch_newton_step(bpf_engine *e) {
   // the fill also finishes level 0 when the analysis allows it
   const int c0 = e->S.l0_chunks;
   tbegin(e, "jacobian");
-  hipLaunchKernelGGL(k_jacobian<T>, xgrid(e->T, e->g.nJb, 256),
+  hipLaunchKernelGGL(k_jacobian<T>, xgrid(e->T, ((size_t)e->g.nJb + 1) / 2, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU, c0 > 0 ? 1 : 0);
   tend(e);
   tbegin(e, "lu_factor");
@@ -1146,7 +1146,7 @@ extern "C" bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V,
   HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
   B4<double> *LU = (B4<double> *)e->d_LU;
   HIPCHK(hipMemset(LU, 0, sizeof(B4<double>) * (size_t)e->T * e->g.nnzLU));
-  hipLaunchKernelGGL(k_jacobian<double>, xgrid(e->T, e->g.nJb, 256),
+  hipLaunchKernelGGL(k_jacobian<double>, xgrid(e->T, ((size_t)e->g.nJb + 1) / 2, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU, 0);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
@@ -1256,7 +1256,7 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   B4<double> *LU = (B4<double> *)e->d_LU;
   const unsigned TT = (unsigned)T;
   tbegin(e, "adjoint");
-  hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, e->g.nJb, 256),
+  hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, ((size_t)e->g.nJb + 1) / 2, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU, 0);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
                      e->stream, e->g, e->d.st, 0, e->fplan, LU,

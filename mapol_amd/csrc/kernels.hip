@@ -646,23 +646,32 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *d.active_count = 0;
   // one thread per Jacobian block, in block-position order: the writes into
   // the factor slab are nearly contiguous, the reads of Ybus are gathered
-  const XSlot sl = xcd_slot(parts_of(g.nJb));
-  const int t = sl.t, r = sl.part * blockDim.x + threadIdx.x;
-  if (t >= d.T || r >= g.nJb) return;
-  if (d.st[t].done) return;
-  const int4 jb = g.jac4[r];
-  B4<T> v = jac_block<T>(g, d, t, jb);
-  const int role = elim0 ? g.jac_l0[r] : -1;
-  if (role == -2) {
-    B4<T> inv;
-    if (blk_inv(v, inv)) d.jsing[t] = 1;
-    v = inv;
-  } else if (role >= 0) {
-    B4<T> inv;
-    blk_inv(jac_block<T>(g, d, t, g.jac4[role]), inv);  // flagged by the pivot's own block
-    v = blk_mul(v, inv);
+  // JU blocks per thread (rows r and r + blockDim.x of a part of JU *
+  // blockDim.x blocks), their loads issued together: half the workgroups
+  // of one block per thread, which matters once few instances remain
+  constexpr int JU = 2;
+  const int bd = (int)blockDim.x;
+  const XSlot sl = xcd_slot((g.nJb + JU * bd - 1) / (JU * bd));
+  const int t = sl.t;
+  if (t >= d.T || d.st[t].done) return;
+#pragma unroll
+  for (int u = 0; u < JU; u++) {
+    const int r = sl.part * JU * bd + u * bd + (int)threadIdx.x;
+    if (r >= g.nJb) break;
+    const int4 jb = g.jac4[r];
+    B4<T> v = jac_block<T>(g, d, t, jb);
+    const int role = elim0 ? g.jac_l0[r] : -1;
+    if (role == -2) {
+      B4<T> inv;
+      if (blk_inv(v, inv)) d.jsing[t] = 1;
+      v = inv;
+    } else if (role >= 0) {
+      B4<T> inv;
+      blk_inv(jac_block<T>(g, d, t, g.jac4[role]), inv);  // flagged by the pivot's own block
+      v = blk_mul(v, inv);
+    }
+    LU[(size_t)t * g.nnzLU + jb.x] = v;
   }
-  LU[(size_t)t * g.nnzLU + jb.x] = v;
 }
 
 // ---------------------------------------------------------------------------
