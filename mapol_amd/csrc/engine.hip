@@ -313,6 +313,10 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(F.tdense, S.tl_dense); UP(F.tpos, S.tl_pos); UP(F.tpp, S.tl_pp);
   UP(F.tpairL, S.tl_pairL); UP(F.tpairU, S.tl_pairU);
   F.tstage_n = (int)S.tl_stage.size();
+  {
+    const char *tv = getenv("BPF_TAIL_VALU");
+    F.tail_mfma = !(tv && tv[0] == '1');
+  }
   UP(F.tstage, nonempty(S.tl_stage, 1)); UP(F.tpairS, nonempty(S.tl_pairS, 1));
   SolvePlan &SP = e->splan;
   SP.nfw = S.nfw;

@@ -422,6 +422,7 @@ struct FactorPlan {
   const int *tdense, *tpos, *tpp, *tpairL, *tpairU;
   int tstage_n;                        // staged tail blocks (0: no staging)
   const int *tstage, *tpairS;          // their positions; pair -> L | U << 16
+  int tail_mfma;                       // FP32 tail inverse on matrix cores
 };
 
 struct SolvePlan {
@@ -602,6 +603,156 @@ __device__ __forceinline__ bool pair_inv(const B4<T> (&m)[2][2], B4<T> (&r)[2][2
   return bad;
 }
 
+// Dense-tail inverse on matrix cores (FP32, workgroups of 1024 lanes).  S,
+// 2s scalars per side, is taken as one 128 x 128 matrix (identity beyond
+// S, zero coupling); wave w holds the 32 x 32 tile (w / 4, w % 4) in the
+// accumulators of v_mfma_f32_32x32x2f32 for the whole elimination
+// (accumulator v of lane l: tile row 8 (v / 4) + 4 (l / 32) + v % 4, tile
+// column l % 32; operand lane l: A row / B column l % 32, k = l / 32).
+// Gauss-Jordan by pivot quads, the block pivots 2p and 2p + 1 (scalars
+// 4p..4p+3), in the order of the block form.  A step updates every entry by
+// one rank-4 product,
+//   new = M' + A B,  A(i, k) = -col(i, k), or [i = 4p + k] on a pivot row,
+//                    B(k, j) = (Q row)(k, j), or Q(k, j - 4p) in a pivot column,
+//                    M' = M, or 0 on a pivot row or column,
+// which leaves Q (the pivot block's inverse) on the pivot block, Q row on the
+// pivot rows, -col Q in the pivot columns and M - col Q row elsewhere: two
+// MFMAs per wave.  The next quad's columns and rows, and its inverse
+// (pair_inv by lane 0 of the wave holding its pivot block, after gathering
+// the block with shuffles), pass through LDS double buffers: one barrier per
+// step.  A singular pivot block sets sing.
+constexpr int TAIL_NS = 2 * TAIL_MAX;   // scalars per side of the dense tail
+struct TailQuads {
+  float col[2][TAIL_NS][4], row[2][4][TAIL_NS], q[2][4][4];
+};
+
+__device__ __forceinline__ void tail_inverse_mfma(const B4<float> *D, int s, int nt,
+                                                  TailQuads &mq, B4<float> *Ti,
+                                                  int &sing) {
+  typedef float f32x16 __attribute__((ext_vector_type(16)));
+  const int n2 = 2 * s;
+  auto mget = [&](int i, int j) -> float {
+    if (i < n2 && j < n2)
+      return (&D[(i >> 1) * s + (j >> 1)].a)[2 * (i & 1) + (j & 1)];
+    return i == j ? 1.f : 0.f;
+  };
+  auto put_q = [&](int buf, const B4<float> (&r)[2][2]) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        mq.q[buf][2 * a][2 * b] = r[a][b].a;
+        mq.q[buf][2 * a][2 * b + 1] = r[a][b].b;
+        mq.q[buf][2 * a + 1][2 * b] = r[a][b].c;
+        mq.q[buf][2 * a + 1][2 * b + 1] = r[a][b].d;
+      }
+  };
+  // the wave's tile in scalar registers: every choice below that depends
+  // only on the tile and the step is a scalar branch
+  const int lane = (int)threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int rb = (w >> 2) * 32, cb = (w & 3) * 32;
+  const int half = lane >> 5;
+  const int ia = rb + (lane & 31);   // row of the lane's A operands
+  const int jc = cb + (lane & 31);   // column of its B operands and accumulators
+  auto acc_row = [&](int v) { return rb + 8 * (v >> 2) + 4 * half + (v & 3); };
+  f32x16 acc;
+#pragma unroll
+  for (int v = 0; v < 16; v++) acc[v] = mget(acc_row(v), jc);
+  if (threadIdx.x < TAIL_NS) {
+    const int q = (int)threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      mq.col[0][q][k] = mget(q, k);
+      mq.row[0][k][q] = mget(k, q);
+    }
+  }
+  if (threadIdx.x == 0) {
+    B4<float> m[2][2], r[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+        m[a][b] = {mget(2 * a, 2 * b), mget(2 * a, 2 * b + 1),
+                   mget(2 * a + 1, 2 * b), mget(2 * a + 1, 2 * b + 1)};
+    if (pair_inv(m, r)) sing = 1;
+    put_q(0, r);
+  }
+  __syncthreads();
+  for (int p = 0; p < nt; p++) {
+    const int cur = p & 1, nxt = cur ^ 1;
+    const bool prow = (ia >> 2) == p, pcol = (jc >> 2) == p;
+    float a[2], b[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int k = 2 * h + half;
+      a[h] = prow ? ((ia & 3) == k ? 1.f : 0.f) : -mq.col[cur][ia][k];
+      const float qr = mq.q[cur][k][0] * mq.row[cur][0][jc] + mq.q[cur][k][1] * mq.row[cur][1][jc] +
+                       mq.q[cur][k][2] * mq.row[cur][2][jc] + mq.q[cur][k][3] * mq.row[cur][3][jc];
+      b[h] = pcol ? mq.q[cur][k][jc & 3] : qr;
+    }
+#pragma unroll
+    for (int v = 0; v < 16; v++)
+      if (pcol || (acc_row(v) >> 2) == p) acc[v] = 0.f;
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[0], b[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[1], b[1], acc, 0, 0, 0);
+    if (p + 1 < nt) {
+      const int nb = 4 * (p + 1);
+      if (nb >= rb && nb < rb + 32) {   // the next quad's rows
+        const int qq = (nb - rb) >> 2, g = qq >> 1;
+        float rv[4];
+        if (g == 0) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) rv[k] = acc[k];
+        } else if (g == 1) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) rv[k] = acc[4 + k];
+        } else if (g == 2) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) rv[k] = acc[8 + k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; k++) rv[k] = acc[12 + k];
+        }
+        if (half == (qq & 1))
+#pragma unroll
+          for (int k = 0; k < 4; k++) mq.row[nxt][k][jc] = rv[k];
+        if (nb >= cb && nb < cb + 32) {   // and its pivot block
+          const int src = 32 * (qq & 1) + (nb - cb);
+          float e[4][4];
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) e[k][m] = __shfl(rv[k], src + m, 64);
+          if (lane == 0) {
+            B4<float> m[2][2], r[2][2];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+              for (int y = 0; y < 2; y++)
+                m[x][y] = {e[2 * x][2 * y], e[2 * x][2 * y + 1], e[2 * x + 1][2 * y],
+                           e[2 * x + 1][2 * y + 1]};
+            if (pair_inv(m, r)) sing = 1;
+            put_q(nxt, r);
+          }
+        }
+      }
+      if (nb >= cb && nb < cb + 32) {   // the next quad's columns
+        const int k = (lane & 31) - (nb - cb);
+        if (k >= 0 && k < 4)
+#pragma unroll
+          for (int v = 0; v < 16; v++) mq.col[nxt][acc_row(v)][k] = acc[v];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int i = acc_row(v);
+    if (i < n2 && jc < n2) (&Ti[(i >> 1) * s + (jc >> 1)].a)[2 * (i & 1) + (jc & 1)] = acc[v];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Jacobian fill into the L+U storage (compuateJacobianMatrix,
 // newtonpf.h:97-186): one thread per (Jacobian block, instance) writes one
@@ -713,6 +864,9 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     struct {
       B4<T> col[2][TAIL_MAX][2], row[2][2][TAIL_MAX], q[2][2][2];
     } gj;
+    // the same on matrix cores, in scalars: the next pivot quad's columns,
+    // rows and inverse
+    TailQuads mq;
   };
   __shared__ TabStage sh;
   int *const s_ct = sh.tab.ct, *const s_cp = sh.tab.cp, *const s_cm = sh.tab.cm;
@@ -959,7 +1113,9 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     // 2x2 tiles of blocks per side; an odd size gets an identity pivot
     // appended (zero coupling: every other value comes out unchanged)
     const int se = s + (s & 1), nt = se / 2;
-    if (sizeof(T) == 4 && nt * nt <= (int)blockDim.x) {
+    if (sizeof(T) == 4 && P.tail_mfma && blockDim.x == 1024) {
+      if constexpr (sizeof(T) == 4) tail_inverse_mfma(D, s, nt, sh.mq, Ti, sing);
+    } else if (sizeof(T) == 4 && nt * nt <= (int)blockDim.x) {
       // FP32: Gauss-Jordan by pivot pairs, each thread owning a 2x2 tile for
       // the whole elimination: per step the pair's inverse Q (formed by the
       // owner of the next diagonal tile), its two columns and two rows pass

@@ -244,13 +244,18 @@ def test_sweep_forms_match_oracle(monkeypatch, form, threads):
                 assert np.abs(r["V"][t] - ref[t]["V"]).max() <= DV_TOL
 
 
-@pytest.mark.parametrize("lu", [1024, 512, 256])
-def test_factorization_workgroup_sizes_match_oracle(lu):
-    """Every workgroup size of the factorization: 1024 lanes invert the
-    61-pivot dense tail by pivot pairs in 2x2 tiles (FP32) or one pivot per
-    step (FP64); 512 and 256 lanes keep several entries per thread.  Same
-    iterations, verdicts and voltages as the oracle."""
-    print(f"[factor] lu threads={lu}", flush=True)
+@pytest.mark.parametrize("lu,tail", [(1024, "mfma"), (1024, "valu"), (512, "valu"),
+                                     (256, "valu")])
+def test_factorization_workgroup_sizes_match_oracle(lu, tail, monkeypatch):
+    """Every workgroup size and dense-tail inverse of the factorization: with
+    1024 lanes the 61-pivot tail is inverted on matrix cores (FP32, one
+    32x32 tile per wave, pivot quads) or, with BPF_TAIL_VALU=1, by pivot
+    pairs in 2x2 tiles of blocks; FP64 takes one pivot per step; 512 and 256
+    lanes keep several entries per thread.  Same iterations, verdicts and
+    voltages as the oracle."""
+    if tail == "valu":
+        monkeypatch.setenv("BPF_TAIL_VALU", "1")
+    print(f"[factor] lu threads={lu} tail={tail}", flush=True)
     case, cont = synth.workload("pf9241x200")
     T = 4
     orc = oracle.Oracle(case)
