@@ -839,9 +839,11 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
 // output of the same chunk writes.
 // Dense tail (the top pivots, ancestors of everything else): its Schur
 // complement S = A_SS - L_SN U_NS is assembled in LDS and inverted in place
-// by block Gauss-Jordan with the same pivot order (one barrier per pivot:
-// the next pivot row, column and inverse are captured into double buffers
-// while the current step updates); S^-1 goes to TL for the sweeps.
+// by Gauss-Jordan with the same pivot order (one barrier per step: the next
+// pivot rows, columns and inverse are captured into double buffers while
+// the current step updates): FP32 on 1024 lanes by pivot quads on the matrix
+// cores (tail_inverse_mfma), otherwise by pivot pairs (FP32) or single
+// pivots (FP64) on the vector ALUs; S^-1 goes to TL for the sweeps.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
