@@ -1057,7 +1057,10 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       __syncthreads();
       const int gsz = __builtin_amdgcn_readfirstlane(lane_group(P.tnent, P.tmax));
       const int lane = threadIdx.x & (gsz - 1);
-      constexpr int TU = 8;
+      // pair indices of a lane requested together: one round trip per 12
+      // products (16 would spill; the entries come longest first, so the
+      // lanes of a wave have lists of about the same length)
+      constexpr int TU = 12;
       for (int k = (int)threadIdx.x / gsz; k < P.tnent; k += (int)blockDim.x / gsz) {
         const int dd = P.tdense[k];
         const int pos = P.tpos[k];

@@ -461,6 +461,35 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
         S.tl_pp.push_back((int)S.tl_pairL.size());
       }
     }
+    // longest product lists first (stable): the factorization takes the
+    // entries one per lane in passes over the workgroup, and a wave runs as
+    // long as its longest list, so the long lists share the first waves of
+    // the first pass and the later passes hold only short ones; every entry
+    // keeps its own list and order, so its sum is the same
+    const int ne = (int)S.tl_pos.size();
+    std::vector<int> ord(ne);
+    for (int q = 0; q < ne; q++) ord[q] = q;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+      return S.tl_pp[x + 1] - S.tl_pp[x] > S.tl_pp[y + 1] - S.tl_pp[y];
+    });
+    std::vector<int> dense(ne), tpos(ne), pp(1, 0), pl, pu;
+    pl.reserve(S.tl_pairL.size());
+    pu.reserve(S.tl_pairU.size());
+    for (int q = 0; q < ne; q++) {
+      const int e = ord[q];
+      dense[q] = S.tl_dense[e];
+      tpos[q] = S.tl_pos[e];
+      for (int r = S.tl_pp[e]; r < S.tl_pp[e + 1]; r++) {
+        pl.push_back(S.tl_pairL[r]);
+        pu.push_back(S.tl_pairU[r]);
+      }
+      pp.push_back((int)pl.size());
+    }
+    S.tl_dense.swap(dense);
+    S.tl_pos.swap(tpos);
+    S.tl_pp.swap(pp);
+    S.tl_pairL.swap(pl);
+    S.tl_pairU.swap(pu);
   }
   for (int q = 0; q + 1 < (int)S.tl_pp.size(); q++)
     S.tl_max = std::max(S.tl_max, S.tl_pp[q + 1] - S.tl_pp[q]);

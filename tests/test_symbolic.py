@@ -323,6 +323,22 @@ def test_tail_staging_reproduces_the_pair_lists(name):
     np.testing.assert_array_equal(stg[ps >> 16], U)
 
 
+@pytest.mark.parametrize("name", ["pf118x200", "pf1354x200", "pf9241x200"])
+def test_tail_entries_come_longest_first(name):
+    """The tail entries are ordered by the length of their product lists,
+    longest first (one lane per entry, a wave as slow as its longest list),
+    and still cover every output of the dense tail once."""
+    case, _ = synth.workload(name)
+    a = engine.analyse(case)
+    s = a["info"]["tail_n"]
+    n = np.diff(a["tl_ptr"])
+    assert (np.diff(n) <= 0).all()
+    pos = a["tl_dense"] & MASK
+    assert len(np.unique(pos)) == len(pos) and pos.max() < s * s
+    # every diagonal block of the tail is an output
+    assert set(range(0, s * s, s + 1)) <= set(pos.tolist())
+
+
 @pytest.mark.parametrize("name", ["pf118x200", "pf1354x200"])
 def test_transposed_replay_solves_the_adjoint_system(name):
     """k_adj_solve's schedule: (superset block matrix)^T lambda = g against a
