@@ -1910,8 +1910,11 @@ __device__ __forceinline__ void solve_pass(const DevTopo &g, const B4<T> *A,
                                            double *dx) {
   // rhs_mode 0: (P, Q) mismatch; 1: (P, 0) and 2: (0, Q) for the decoupled
   // half steps; 3: the refinement residual of k_residual.  Vector loads in
-  // groups of VU per thread, all requested before the first is used.
-  constexpr int VU = 8;
+  // groups of VU per thread, all requested before the first is used: with
+  // 1024 lanes one round trip covers 20480 scalars (10240 blocks), the
+  // right-hand side and the accumulated dx of a 9241-bus grid each in one
+  // (24 would spill).
+  constexpr int VU = 20;
   const int n2 = 2 * g.nb;
   const int stride = (int)blockDim.x;
   for (int k0 = threadIdx.x; k0 < n2; k0 += VU * stride) {
