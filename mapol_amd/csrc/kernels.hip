@@ -1043,6 +1043,20 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       // from LDS in the order of its pair list (the sums of the plain loop)
       constexpr int SU = 6;
       const int ns = P.tstage_n, bd = (int)blockDim.x;
+      const int gsz = __builtin_amdgcn_readfirstlane(lane_group(P.tnent, P.tmax));
+      const int lane = threadIdx.x & (gsz - 1);
+      // an entry's descriptor is requested a pass ahead: the first pass's
+      // with the staging loads, the next pass's while a pass runs (clamped
+      // to the last entry; every tail has at least one)
+      int dd_n, pos_n, qa_n, q1_n;
+      auto descriptor = [&](int k) {
+        k = min(k, P.tnent - 1);
+        dd_n = P.tdense[k];
+        pos_n = P.tpos[k];
+        qa_n = P.tpp[k];
+        q1_n = P.tpp[k + 1];
+      };
+      descriptor((int)threadIdx.x / gsz);
       for (int q0 = threadIdx.x; q0 < ns; q0 += SU * bd) {
         int sp[SU];
 #pragma unroll
@@ -1055,16 +1069,13 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
           if (q0 + u * bd < ns) sh.stage[q0 + u * bd] = sv[u];
       }
       __syncthreads();
-      const int gsz = __builtin_amdgcn_readfirstlane(lane_group(P.tnent, P.tmax));
-      const int lane = threadIdx.x & (gsz - 1);
       // pair indices of a lane requested together: one round trip per 12
       // products (16 would spill; the entries come longest first, so the
       // lanes of a wave have lists of about the same length)
       constexpr int TU = 12;
       for (int k = (int)threadIdx.x / gsz; k < P.tnent; k += (int)blockDim.x / gsz) {
-        const int dd = P.tdense[k];
-        const int pos = P.tpos[k];
-        const int qa = P.tpp[k], q1 = P.tpp[k + 1];
+        const int dd = dd_n, pos = pos_n, qa = qa_n, q1 = q1_n;
+        descriptor(k + (int)blockDim.x / gsz);
         const B4<T> a0 = A[(dd & TASK_HAS_A) ? pos : 0];
         B4<T> acc = blk_sel(lane == 0 && (dd & TASK_HAS_A), a0, blk_zero<T>());
         for (int q = qa + lane; q < q1; q += TU * gsz) {
