@@ -90,6 +90,9 @@ def lib(which: str = "own"):
                                 C.POINTER(OrcOpts), C.POINTER(OrcResult)]
         L.orc_make_ybus.argtypes = [C.POINTER(OrcCase), C.POINTER(OrcInst),
                                     _pi, _pi, _pd, C.c_int]
+        L.orc_make_bmat.argtypes = [C.POINTER(OrcCase), C.POINTER(OrcInst),
+                                    C.c_int, C.c_int, _pi, _pi, _pi, _pd,
+                                    C.c_int]
         L.orc_mismatch.argtypes = [C.POINTER(OrcCase), C.POINTER(OrcInst),
                                    _pd, _pd, _pd]
         L.orc_jacobian.argtypes = [C.POINTER(OrcCase), C.POINTER(OrcInst),
@@ -213,6 +216,25 @@ class Oracle:
         import scipy.sparse as sp
         return sp.csr_matrix((v[0:2 * nnz:2] + 1j * v[1:2 * nnz:2],
                               ci[:nnz], rp), shape=(n, n))
+
+    def bmat(self, alg, which, **inst):
+        """B' (which 1, over pv then pq) or B'' (which 2, over pq) of the
+        fast-decoupled variant ``alg`` as a scipy CSR matrix"""
+        import scipy.sparse as sp
+        n = self.case.nbus
+        cap = n + 2 * self.case.nbranch
+        rp = np.zeros(n + 1, np.int32)
+        ci = np.zeros(cap, np.int32)
+        v = np.zeros(cap)
+        rows = np.zeros(1, np.int32)
+        i, keep = self._inst(**inst)
+        nnz = lib(self.which).orc_make_bmat(C.byref(self.s), C.byref(i), alg,
+                                            which, _p(rows, _pi), _p(rp, _pi),
+                                            _p(ci, _pi), _p(v, _pd), cap)
+        assert nnz >= 0
+        rows = int(rows[0])
+        return sp.csr_matrix((v[:nnz], ci[:nnz], rp[:rows + 1]),
+                             shape=(rows, rows))
 
     def initial_voltage(self, **inst):
         n = self.case.nbus

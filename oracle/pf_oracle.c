@@ -864,42 +864,74 @@ static void make_bmat(const ybus_t *Y, const int *sel, int nsel, const int *idx,
   *rp_o = rp; *ci_o = ci; *v_o = v;
 }
 
-static int fdpf(const ybus_t *Y, const inst_t *I, const int *type,
-                const int *from, const int *to, cpx *V, const orc_opts *o,
-                int *iters, real *norm_out) {
+/* B' (over pv then pq, fdpf.h:481-484) and B'' (over pq) of mkl_makeB
+ * (makeB.h:392-501) from two modified Ybus builds:
+ *   B':  line charging 0, taps 1, bus shunts 0 (and controls off,
+ *        makeB.h:311-324); XB also R = 0; phase shifts kept;
+ *   B'': phase shifts 0 (makeB.h:383-390); BX also R = 0; taps, charging,
+ *        shunts and controls kept.
+ * (mkl_zeroBusesAndBranches clears bus shunts inside its branch loop, so a
+ * grid with fewer branches than buses keeps the shunts of the last buses;
+ * the reference's own Ybus build, makeYbus.h:54-59, leaves those buses'
+ * shunts unset in that case, so only nbranch >= nbus is defined, and there
+ * every shunt is cleared.) */
+typedef struct {
+  int npv, npq, np;
+  int *rpP, *ciP, *rpQ, *ciQ;
+  real *vP, *vQ;
+} bmat_t;
+
+static void bmat_free(bmat_t *B) {
+  free(B->rpP); free(B->ciP); free(B->vP);
+  free(B->rpQ); free(B->ciQ); free(B->vQ);
+}
+
+static void bmat_build(const inst_t *I, const order_t *O, const int *from,
+                       const int *to, int alg, bmat_t *Bm) {
   int n = I->n, m = I->m;
-  order_t O;
-  order_build(type, n, &O);
-  int npv = O.npv, npq = O.npq, np = npv + npq;
-  /* B': zero line charging, taps -> 1, bus shunts -> 0; XB also R -> 0 */
+  int npv = O->npv, npq = O->npq, np = npv + npq;
+  Bm->npv = npv; Bm->npq = npq; Bm->np = np;
   inst_t T = *I;
-  real *zB = (real *)calloc(m, sizeof(real));
-  real *zT = (real *)calloc(m, sizeof(real));
-  real *zR = (real *)calloc(m, sizeof(real));
+  real *zB = (real *)calloc(m > 0 ? m : 1, sizeof(real));
+  real *zT = (real *)calloc(m > 0 ? m : 1, sizeof(real));
+  real *zR = (real *)calloc(m > 0 ? m : 1, sizeof(real));
   real *zBs = (real *)calloc(n, sizeof(real));
   T.B = zB; T.tap = zT; T.Bs = zBs;
-  if (o->alg == ORC_FDXB) T.R = zR;
+  if (alg == ORC_FDXB) T.R = zR;
   ybus_t Yp;
   memset(&Yp, 0, sizeof(Yp));
   ybus_build(&T, from, to, &Yp);
   int *sel_p = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
-  for (int k = 0; k < npv; k++) sel_p[k] = O.pv[k];
-  for (int k = 0; k < npq; k++) sel_p[npv + k] = O.pq[k];
-  int *rpP, *ciP; real *vP;
-  make_bmat(&Yp, sel_p, np, O.idx, &rpP, &ciP, &vP);
-  /* B'': zero phase shifters; BX also R -> 0 (taps, charging, shunts kept) */
+  for (int k = 0; k < npv; k++) sel_p[k] = O->pv[k];
+  for (int k = 0; k < npq; k++) sel_p[npv + k] = O->pq[k];
+  make_bmat(&Yp, sel_p, np, O->idx, &Bm->rpP, &Bm->ciP, &Bm->vP);
   inst_t T2 = *I;
-  real *zS = (real *)calloc(m, sizeof(real));
+  real *zS = (real *)calloc(m > 0 ? m : 1, sizeof(real));
   T2.shift = zS;
-  if (o->alg == ORC_FDBX) T2.R = zR;
+  if (alg == ORC_FDBX) T2.R = zR;
   ybus_t Ypp;
   memset(&Ypp, 0, sizeof(Ypp));
   ybus_build(&T2, from, to, &Ypp);
   int *pqidx = (int *)malloc(sizeof(int) * n);
   for (int i = 0; i < n; i++) pqidx[i] = -1;
-  for (int k = 0; k < npq; k++) pqidx[O.pq[k]] = k;
-  int *rpQ, *ciQ; real *vQ;
-  make_bmat(&Ypp, O.pq, npq, pqidx, &rpQ, &ciQ, &vQ);
+  for (int k = 0; k < npq; k++) pqidx[O->pq[k]] = k;
+  make_bmat(&Ypp, O->pq, npq, pqidx, &Bm->rpQ, &Bm->ciQ, &Bm->vQ);
+  free(sel_p); free(pqidx);
+  ybus_free(&Yp); ybus_free(&Ypp);
+  free(zB); free(zT); free(zR); free(zBs); free(zS);
+}
+
+static int fdpf(const ybus_t *Y, const inst_t *I, const int *type,
+                const int *from, const int *to, cpx *V, const orc_opts *o,
+                int *iters, real *norm_out) {
+  int n = I->n;
+  order_t O;
+  order_build(type, n, &O);
+  int npv = O.npv, npq = O.npq, np = npv + npq;
+  bmat_t Bm;
+  bmat_build(I, &O, from, to, o->alg, &Bm);
+  int *rpP = Bm.rpP, *ciP = Bm.ciP, *rpQ = Bm.rpQ, *ciQ = Bm.ciQ;
+  real *vP = Bm.vP, *vQ = Bm.vQ;
 
   real *P = (real *)malloc(sizeof(real) * (np > 0 ? np : 1));
   real *Q = (real *)malloc(sizeof(real) * (npq > 0 ? npq : 1));
@@ -948,10 +980,7 @@ static int fdpf(const ybus_t *Y, const inst_t *I, const int *type,
   *norm_out = err;
   lu_free(&LP); lu_free(&LQ);
   free(qP); free(qQ); free(P); free(Q); free(dVa); free(dVm);
-  free(rpP); free(ciP); free(vP); free(rpQ); free(ciQ); free(vQ);
-  free(sel_p); free(pqidx);
-  ybus_free(&Yp); ybus_free(&Ypp);
-  free(zB); free(zT); free(zR); free(zBs); free(zS);
+  bmat_free(&Bm);
   order_free(&O);
   return verdict;
 }
@@ -1023,6 +1052,33 @@ int orc_make_ybus(const orc_case *c, const orc_inst *inst, int *rowptr,
     nnz = -nnz;
   }
   ybus_free(&Y);
+  inst_free(&I);
+  return nnz;
+}
+
+int orc_make_bmat(const orc_case *c, const orc_inst *inst, int alg, int which,
+                  int *rows_out, int *rowptr, int *colind, double *val, int cap) {
+  inst_t I;
+  inst_build(c, inst, &I);
+  order_t O;
+  order_build(I.type, I.n, &O);
+  bmat_t Bm;
+  bmat_build(&I, &O, c->br_from, c->br_to, alg, &Bm);
+  int rows = which == 1 ? Bm.np : Bm.npq;
+  const int *rp = which == 1 ? Bm.rpP : Bm.rpQ;
+  const int *ci = which == 1 ? Bm.ciP : Bm.ciQ;
+  const real *v = which == 1 ? Bm.vP : Bm.vQ;
+  int nnz = rp[rows];
+  *rows_out = rows;
+  if (nnz <= cap) {
+    memcpy(rowptr, rp, sizeof(int) * (rows + 1));
+    memcpy(colind, ci, sizeof(int) * nnz);
+    for (int k = 0; k < nnz; k++) val[k] = (double)v[k];
+  } else {
+    nnz = -nnz;
+  }
+  bmat_free(&Bm);
+  order_free(&O);
   inst_free(&I);
   return nnz;
 }

@@ -106,6 +106,12 @@ int orc_mismatch(const orc_case *c, const orc_inst *inst, const double *V,
 /* Jacobian in the reference layout (CSR); returns nnzJ or -needed. */
 int orc_jacobian(const orc_case *c, const orc_inst *inst, const double *V,
                  int *rowptr, int *colind, double *val, int cap);
+/* B' (which = 1: rows/columns pv then pq) or B'' (which = 2: pq) of the
+ * fast-decoupled variant alg (ORC_FDXB / ORC_FDBX), CSR with sorted
+ * columns, as mkl_makeB builds them (makeB.h:311-501); *rows = its order;
+ * returns nnz or -needed. */
+int orc_make_bmat(const orc_case *c, const orc_inst *inst, int alg, int which,
+                  int *rows, int *rowptr, int *colind, double *val, int cap);
 /* Initial voltage (runpf.h:211-226). */
 void orc_initial_voltage(const orc_case *c, const orc_inst *inst, double *V);
 /* Branch complex flows Sf, St (pu, interleaved) and the objective
