@@ -172,6 +172,23 @@ bpf_status bpf_get_branch_loading(bpf_engine *e, const double *rate,
                                   double *loading, int32_t *overloads,
                                   double *max_loading);
 
+/* Allocation-free variant of bpf_get_branch_loading: same arguments, same
+ * results.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc/hipFree: the
+ * ratings and the per-instance outputs are staged in a device array of
+ * BPF_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs on
+ * until the process ends, in batches of instances that fit it.  Limit: the
+ * ratings plus one instance must fit, i.e. nbranch <= 262 142 with
+ * loading != NULL (524 256 without).  Larger cases return
+ * BPF_ERR_INVALID_ARG.  All engines on one GPU share that array: calls that
+ * overlap in time on one GPU (several host threads) race; calls in sequence
+ * do not. */
+#define BPF_NOALLOC_SCRATCH_BYTES (4u << 20)
+bpf_status bpf_get_branch_loading_noalloc(bpf_engine *e, const double *rate,
+                                          double *loading, int32_t *overloads,
+                                          double *max_loading);
+
 /* hybrid_free (runpf.h:984-1078), mkl_clean (runpf.h:452). */
 void bpf_destroy(bpf_engine *e);
 

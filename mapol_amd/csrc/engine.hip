@@ -8,6 +8,7 @@
 
 #include <chrono>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -105,7 +106,10 @@ struct bpf_engine {
   bool capturing = false;            // recording the iteration graph
   std::map<std::string, std::pair<int, double>> ktime;
   bool solved = false;
+  double *scratch = nullptr;         // this device's allocation-free scratch
 };
+
+static bpf_status scratch_setup(bpf_engine *e);
 
 #define HIPCHK(expr)                                                        \
   do {                                                                      \
@@ -253,6 +257,10 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   const char *dbg = getenv("BPF_DEBUG_SYNC");
   e->debug_sync = (dbg && dbg[0] == '1') ? 1 : 0;
   HIPCHK(hipGetDevice(&e->dev));
+  {
+    bpf_status s0 = scratch_setup(e);
+    if (s0) return s0;
+  }
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   HIPCHK(hipDeviceGetAttribute(&e->max_lds,
                                hipDeviceAttributeMaxSharedMemoryPerBlock, e->dev));
@@ -907,6 +915,96 @@ extern "C" bpf_status bpf_get_branch_loading(bpf_engine *e, const double *rate,
   if (overloads) for (int t = 0; t < T; t++) overloads[t] = ov[t];
   if (max_loading)
     HIPCHK(hipMemcpy(max_loading, d_max, sizeof(double) * T, hipMemcpyDeviceToHost));
+  return BPF_OK;
+}
+
+// ---- allocation-free branch report ------------------------------------------
+// The same report as bpf_get_branch_loading for callers that cannot allocate
+// device memory: the ratings and the per-instance outputs are staged in one
+// file-scope device array (each GPU holds its own copy for the life of the
+// process), the instances are reported in batches that fit it, and each
+// batch is copied to the caller before the next one reuses the space.
+// Layout in doubles: rate [round_up(m, 32)], then per batch of tb instances
+// loading [tb * m] (when requested), max [tb], overloads [tb] (int32 stored
+// in round_up(tb / 2) doubles).
+static __device__ __align__(256) double scratch[BPF_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+// hipGetSymbolAddress(scratch) of each device, set by bpf_create with the
+// device current; the host-side name `scratch` is never a device address
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;
+
+static bpf_status scratch_setup(bpf_engine *e) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(e->dev);
+  if (it == g_scratch_ptr.end()) {
+    double *p = nullptr;
+    hipError_t r = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (r != hipSuccess || !p) {
+      e->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(r);
+      return BPF_ERR_HIP;
+    }
+    it = g_scratch_ptr.emplace(e->dev, p).first;
+  }
+  e->scratch = it->second;
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_get_branch_loading_noalloc(bpf_engine *e,
+                                                     const double *rate,
+                                                     double *loading,
+                                                     int32_t *overloads,
+                                                     double *max_loading) {
+  if (!e || !rate) return BPF_ERR_INVALID_ARG;
+  if (!e->solved) { e->err = "no solution yet"; return BPF_ERR_STATE; }
+  if (!e->scratch) { e->err = "scratch address not set up"; return BPF_ERR_STATE; }
+  const int T = e->T, m = e->m;
+  for (int b = 0; b < m; b++)
+    if (!(rate[b] >= 0.0)) {
+      e->err = "branch ratings must be >= 0 (0: no rating)";
+      return BPF_ERR_INVALID_ARG;
+    }
+  const size_t cap = BPF_NOALLOC_SCRATCH_BYTES / sizeof(double);
+  const size_t rate_d = ((size_t)m + 31) / 32 * 32;
+  // doubles per instance: loading row, max, half a double of overload count
+  const size_t per = (loading ? (size_t)m : 0) + 2;
+  if (rate_d + per > cap) {
+    // the largest branch count that fits, for the message
+    size_t mmax = (cap - 2) / (loading ? 2 : 1);
+    while (mmax > 0 && (mmax + 31) / 32 * 32 + (loading ? mmax : 0) + 2 > cap) mmax--;
+    e->err = "bpf_get_branch_loading_noalloc: " + std::to_string(m) +
+             " branches exceed its " + std::to_string(BPF_NOALLOC_SCRATCH_BYTES) +
+             "-byte device scratch (at most " + std::to_string(mmax) +
+             " branches" + (loading ? " with per-branch loading" : "") + ")";
+    return BPF_ERR_INVALID_ARG;
+  }
+  const int tb = (int)std::min<size_t>((size_t)T, (cap - rate_d) / per);
+  double *d_rate = e->scratch;
+  double *d_load = loading ? d_rate + rate_d : nullptr;
+  double *d_max = d_rate + rate_d + (loading ? (size_t)tb * m : 0);
+  int *d_over = (int *)(d_max + tb);
+  if (m)
+    HIPCHK(hipMemcpyAsync(d_rate, rate, sizeof(double) * m,
+                          hipMemcpyHostToDevice, e->stream));
+  for (int t0 = 0; t0 < T; t0 += tb) {
+    const int nt = std::min(tb, T - t0);
+    const size_t off = (size_t)t0 * m;
+    hipLaunchKernelGGL(k_branch_report, dim3(nt), dim3(256), 0, e->stream, e->g,
+                       nt, e->d_Sf + off, e->d_St + off, (const double *)d_rate,
+                       d_load, d_over, d_max);
+    HIPCHK(hipGetLastError());
+    // stream-ordered copies: the next batch's kernel starts after them
+    if (loading)
+      HIPCHK(hipMemcpyAsync(loading + off, d_load, sizeof(double) * nt * m,
+                            hipMemcpyDeviceToHost, e->stream));
+    if (overloads)
+      HIPCHK(hipMemcpyAsync(overloads + t0, d_over, sizeof(int32_t) * nt,
+                            hipMemcpyDeviceToHost, e->stream));
+    if (max_loading)
+      HIPCHK(hipMemcpyAsync(max_loading + t0, d_max, sizeof(double) * nt,
+                            hipMemcpyDeviceToHost, e->stream));
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
   return BPF_OK;
 }
 

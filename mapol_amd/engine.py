@@ -26,7 +26,7 @@ _pu8 = C.POINTER(C.c_uint8)
 EXPORTS = [
     "bpf_default_options", "bpf_create", "bpf_set_instances", "bpf_solve",
     "bpf_get_results", "bpf_get_branch_flows", "bpf_get_branch_loading",
-    "bpf_destroy",
+    "bpf_get_branch_loading_noalloc", "bpf_destroy",
     "bpf_last_error", "bpf_get_info", "bpf_get_ybus", "bpf_stage_mismatch",
     "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_adjoint",
     "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
@@ -105,6 +105,7 @@ def load_library(path: str = LIB_PATH):
     L.bpf_get_results.argtypes = [V, _pd, _pi, _pi, _pd, _pd, _pi, _pi]
     L.bpf_get_branch_flows.argtypes = [V, _pd, _pd]
     L.bpf_get_branch_loading.argtypes = [V, _pd, _pd, _pi, _pd]
+    L.bpf_get_branch_loading_noalloc.argtypes = [V, _pd, _pd, _pi, _pd]
     L.bpf_destroy.argtypes = [V]
     L.bpf_destroy.restype = None
     L.bpf_last_error.argtypes = [V]
@@ -420,6 +421,22 @@ class Engine:
                                                   _ptr(load, _pd), _ptr(over, _pi),
                                                   _ptr(mx, _pd)),
                     "bpf_get_branch_loading")
+        return dict(loading=load, overloads=over, max_loading=mx)
+
+    def branch_loading_noalloc(self, rate, per_branch=True):
+        """The same report through ``bpf_get_branch_loading_noalloc``, which
+        allocates no device memory (``loading`` is None when ``per_branch``
+        is False)."""
+        T, m = self.T, self.case.nbranch
+        rt = np.ascontiguousarray(rate, np.float64)
+        if rt.shape != (m,):
+            raise EngineError(f"rate must have shape ({m},)")
+        load = np.zeros((T, m)) if per_branch else None
+        over = np.zeros(T, np.int32)
+        mx = np.zeros(T)
+        self._check(self.L.bpf_get_branch_loading_noalloc(
+            self.h, _ptr(rt, _pd), _ptr(load, _pd), _ptr(over, _pi),
+            _ptr(mx, _pd)), "bpf_get_branch_loading_noalloc")
         return dict(loading=load, overloads=over, max_loading=mx)
 
     def info(self):
