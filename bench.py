@@ -64,19 +64,35 @@ def algorithmic_bytes(info, case, T, its_total, s, refine):
 
 
 def run_gpu(case, cont, precision, refine, qlim, warmup, steps, timing=True,
-            tune=True, graph=True, fixed=None):
+            tune=True, graph=True, fixed=None, cache=None):
     import torch
+    from mapol_amd import tune as tuner
     from mapol_amd.engine import Engine
-    from mapol_amd.tune import autotune
 
-    e = Engine(case, precision=precision, refine_steps=refine,
-               enforce_qlim=qlim, use_graph=graph)
-    e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+    def make(**tiles):
+        e = Engine(case, precision=precision, refine_steps=refine,
+                   enforce_qlim=qlim, use_graph=graph, **tiles)
+        e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+        return e
+
+    entry = None
     if fixed:
+        e = make()
         e.set_launch(*fixed)
         launch = dict(zip(("lu", "solve", "mismatch"), fixed))
+    elif tune:
+        # tiles, launch shapes and schedule from the autotuner (its JSON cache
+        # when it holds this workload; tuning is outside the timed steps)
+        entry = tuner.tune(make, tuner.signature(case, cont.T, precision, refine, qlim),
+                           cache_path=cache or tuner.DEFAULT_CACHE)
+        e = make(**entry["tiles"])
+        tuner.apply(e, entry)
+        launch = entry["launch"]
+        log(f"[bench] tuning ({'cached' if entry['cached'] else 'measured'}): "
+            f"tiles {entry['tiles']} schedule {entry['schedule']}")
     else:
-        launch = autotune(e) if tune else None
+        e = make()
+        launch = None
     if launch:
         log(f"[bench] launch shapes: lu {launch['lu']} solve {launch['solve']} "
             f"mismatch {launch['mismatch']}")
@@ -91,6 +107,7 @@ def run_gpu(case, cont, precision, refine, qlim, warmup, steps, timing=True,
     dt = time.perf_counter() - t0
     r = e.results()
     info = e.info()
+    message = e.last_error()
     # per-kernel durations: the same steps again with HIP events around
     # direct launches on the engine's stream (events inside replayed graphs
     # do not time reliably)
@@ -107,7 +124,9 @@ def run_gpu(case, cont, precision, refine, qlim, warmup, steps, timing=True,
         e.set_timing(False)
     e.close()
     return dict(seconds=dt, seconds_timed=dt_ev, kt=kt, res=r, info=info,
-                launch=launch)
+                launch=launch, message=message,
+                tuning=(None if entry is None else
+                        {k: entry[k] for k in ("tiles", "launch", "schedule", "cached")}))
 
 
 def copy_calibration(mib):
@@ -124,10 +143,30 @@ def copy_calibration(mib):
     log(f"[bench] copied {mib} MiB ({n * 4} bytes)")
 
 
-def cpu_baseline(case, cont, qlim, threads, limit_s=30.0):
-    """Oracle (own sparse LU, FP64) over the contingencies on host threads;
-    stops starting new instances after ``limit_s`` seconds of wall time."""
+def cpu_info():
+    """host CPU as the box reports it: logical CPUs and the model name"""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_baseline(case, cont, qlim, threads, runs=5, warmup=2, single_s=2.5):
+    """The oracle (FP64 restatement of the reference's CPU Newton with its own
+    sparse LU, symbolic analysis once per case) on the host, as the
+    reference's exec modes run it (src/pso/run.h:86-95): all contingencies on
+    ``threads`` threads (one contingency per thread), and one core solving a
+    sample of the contingencies in sequence.  Each is the median of ``runs``
+    timed runs after ``warmup`` untimed ones (run.sh:6 repeats 5 times).
+    Returns the all-thread results of every contingency (parity) and the
+    figures."""
     from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
     import oracle
 
     orc = oracle.Oracle(case)
@@ -136,17 +175,179 @@ def cpu_baseline(case, cont, qlim, threads, limit_s=30.0):
     def one(t):
         return t, orc.solve(opts, br_on=cont.br_on[t], gen_on=cont.gen_on[t])
 
+    def all_threads():
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            out = dict(ex.map(one, range(cont.T)))
+        return out, time.perf_counter() - t0
+
+    out = None
+    times = []
+    for k in range(warmup + runs):
+        res, dt = all_threads()
+        if out is None:
+            out = res
+        if k >= warmup:
+            times.append(dt)
+    multi = cont.T / float(np.median(times))
+    # one core, contingencies in sequence: every k-th contingency, as many
+    # as one core solves in about single_s seconds (timed once first)
     t0 = time.perf_counter()
+    for t in range(4):
+        one(t)
+    per = (time.perf_counter() - t0) / 4
+    ns = int(max(4, min(cont.T, round(single_s / max(per, 1e-6)))))
+    sample = np.linspace(0, cont.T - 1, ns).round().astype(int)
+    st = []
+    for k in range(warmup + runs):
+        t0 = time.perf_counter()
+        for t in sample:
+            one(int(t))
+        if k >= warmup:
+            st.append(time.perf_counter() - t0)
+    single = ns / float(np.median(st))
+    return out, dict(multi=multi, multi_runs=times, single=single, single_n=ns,
+                     single_runs=st)
+
+
+REAL_GRIDS = {
+    # the reference's own 9241-bus grid (src/datasets/case9241pegase.m, its
+    # tables kept as a fixture), 200 N-1 branch and generator outages
+    "case9241pegase": dict(T=200, seed=9241, gen_frac=0.25, qlim=True),
+}
+
+
+def load_workload(name):
+    """(case, contingencies, reactive limits on, data description)"""
+    from mapol_amd import synth
+    if name not in REAL_GRIDS:
+        case, cont = synth.workload(name)
+        return (case, cont, bool(synth.WORKLOADS[name]["qlim"]),
+                "synthetic (seeded generator, mapol_amd/synth.py)")
+    import tempfile
+    import numpy as np
+    from mapol_amd import matpower
+    from mapol_amd.engine import load_matpower
+    w = REAL_GRIDS[name]
+    z = np.load(os.path.join(ROOT, "tests", "golden", "matpower_cases.npz"))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, name + ".m")
+        matpower.write(path, float(z[f"{name}/baseMVA"]), z[f"{name}/bus"],
+                       z[f"{name}/gen"], z[f"{name}/branch"])
+        case = load_matpower(path)[0]
+    cont = synth.n1_contingencies(case, w["T"], w["seed"], gen_frac=w["gen_frac"])
+    return (case, cont, w["qlim"],
+            f"the reference's {name}.m (bpf_load_matpower), {w['T']} N-1 outages")
+
+
+def rates_from_base(case):
+    """branch ratings U(0.8, 1.5) x the larger end flow of the base case, as
+    the 2869-bus adjoint workload defines them (solved on the GPU)"""
+    import numpy as np
+    from mapol_amd.engine import Engine
+    e = Engine(case)
+    e.set_instances(T=1)
+    e.solve()
+    Sf, St = e.branch_flows()
+    e.close()
+    base = np.maximum(np.abs(Sf[0]), np.abs(St[0]))
+    rng = np.random.default_rng(3)
+    return np.maximum(base, 1e-3) * rng.uniform(0.8, 1.5, case.nbranch)
+
+
+def adjoint_leg(case, cont, qlim, refine, tuning, reps=3):
+    """The adjoint of losses + branch-flow penalty for every contingency after
+    the mixed-precision solve (FP64 factors of its own); device time by HIP
+    events around its kernels, and the whole call (host copies of the
+    gradients included)."""
+    import torch
+    rate = rates_from_base(case)
+    e = _tuned_engine(case, qlim, refine, tuning)
+    e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+    e.solve()
+    e.adjoint(rate=rate, w=1.0)            # warm-up: allocates its buffers
+    e.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e.adjoint(rate=rate, w=1.0)
+    wall = (time.perf_counter() - t0) / reps
+    n, ms = e.kernel_time("adjoint")
+    e.set_timing(False)
+    captures = e.info()["graph_captures"]
+    e.solve()                              # the solve after it: same graph
+    info = e.info()
+    e.close()
+    return dict(device_ms=round(ms / max(n, 1), 3), call_ms=round(wall * 1e3, 3),
+                contingencies=cont.T,
+                solve_after_adjoint_recaptured=bool(info["graph_captures"] != captures))
+
+
+def fd_leg(name, steps, warmup):
+    """Fast decoupled XB and BX (B' and B'' factored once in FP64) and the
+    mixed-precision Newton on the 1354-bus workload: solves/s of each."""
+    import torch
+    from mapol_amd import synth
+    from mapol_amd.engine import BPF_FDBX, BPF_FDXB, BPF_MIXED, BPF_NR, CONVERGED, Engine
+    case, cont = synth.workload(name)
+    qlim = bool(synth.WORKLOADS[name]["qlim"])
     out = {}
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        futs = []
-        for t in range(cont.T):
-            futs.append(ex.submit(one, t))
-        for f in futs:
-            t, r = f.result()
-            out[t] = r
+    for label, alg, prec in (("nr_mixed", BPF_NR, BPF_MIXED), ("fdxb", BPF_FDXB, 0),
+                             ("fdbx", BPF_FDBX, 0)):
+        # reactive-limit switching is a Newton feature
+        e = Engine(case, alg=alg, precision=prec, refine_steps=1,
+                   enforce_qlim=qlim and alg == BPF_NR)
+        e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+        for _ in range(warmup):
+            e.solve()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            e.solve()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        r = e.results()
+        e.close()
+        out[label] = dict(value=round(cont.T * steps / dt, 2),
+                          ms_per_step=round(dt / steps * 1e3, 3),
+                          converged=int((r["verdict"] == CONVERGED).sum()),
+                          max_iterations=int(r["iterations"].max()))
+    return out
+
+
+def _tuned_engine(case, qlim, refine, tuning):
+    from mapol_amd import tune as tuner
+    from mapol_amd.engine import BPF_MIXED, Engine
+    e = Engine(case, precision=BPF_MIXED, refine_steps=refine, enforce_qlim=qlim,
+               **(tuning["tiles"] if tuning else {}))
+    if tuning:
+        tuner.apply(e, tuning)
+    return e
+
+
+def setup_leg(case, cont, qlim, refine, tuning, steps):
+    """A new batch every step: bpf_set_instances (upload of the outage masks,
+    per-instance arrays built on the device) followed by the solve."""
+    import torch
+    e = _tuned_engine(case, qlim, refine, tuning)
+    e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+    e.solve()
+    torch.cuda.synchronize()
+    t_set = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+        t_set += time.perf_counter() - a
+        e.solve()
+    torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return out, dt
+    captures = e.info()["graph_captures"]
+    e.close()
+    mask_bytes = int(cont.br_on.nbytes + cont.gen_on.nbytes)
+    return dict(value=round(cont.T * steps / dt, 2), ms_per_step=round(dt / steps * 1e3, 3),
+                set_instances_ms=round(t_set / steps * 1e3, 3),
+                h2d_bytes_per_batch=mask_bytes, graph_captures=captures)
 
 
 def main():
@@ -169,6 +370,11 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain copy of MIB MiB (profiler "
                          "counter calibration) and exit")
+    ap.add_argument("--tune-cache", default=None, metavar="PATH",
+                    help="autotuner cache (default mapol_amd/tuned.json)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the other configuration legs (set-up + solve, "
+                         "adjoint, fast decoupled at 1354 buses)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: contingencies are independent "
@@ -187,20 +393,20 @@ def main():
     from mapol_amd.engine import BPF_FP64, BPF_MIXED, CONVERGED
 
     torch.cuda.set_device(0)
-    case, cont = synth.workload(args.workload)
-    qlim = bool(synth.WORKLOADS[args.workload]["qlim"])
+    case, cont, qlim, data_desc = load_workload(args.workload)
     T = cont.T
     log(f"[bench] workload {args.workload}: {case.counts()} T={T} qlim={qlim}")
 
     fixed = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
     mix = run_gpu(case, cont, BPF_MIXED, args.refine, qlim, args.warmup,
                   args.steps, tune=not args.no_tune, graph=not args.no_graph,
-                  fixed=fixed)
+                  fixed=fixed, cache=args.tune_cache)
     log(f"[bench] mixed: {mix['seconds'] / args.steps * 1e3:.2f} ms/step")
     fp = None
     if not args.no_fp64:
         fp = run_gpu(case, cont, BPF_FP64, 0, qlim, args.warmup, args.steps,
-                     tune=not args.no_tune, graph=not args.no_graph)
+                     tune=not args.no_tune, graph=not args.no_graph,
+                     cache=args.tune_cache)
         log(f"[bench] fp64: {fp['seconds'] / args.steps * 1e3:.2f} ms/step")
 
     value = T * args.steps / mix["seconds"]
@@ -241,17 +447,20 @@ def main():
         "ms_per_step": round(mix["seconds"] / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 LU+solve, f64 mismatch/refinement/update",
-        "data": "synthetic (seeded generator, mapol_amd/synth.py)",
+        "data": data_desc,
         "config": {"workload": args.workload, "buses": case.nbus,
                    "branches": case.nbranch, "contingencies": T,
                    "precision": "mixed", "refine_steps": args.refine,
                    "tol": 1e-8, "enforce_qlim": qlim,
                    "launch": ({k: mix["launch"][k] for k in ("lu", "solve", "mismatch")}
-                              if mix["launch"] else None)},
+                              if mix["launch"] else None),
+                   "tuning": mix["tuning"]},
         "roofline": roof,
         "kernels": kernels,
         "newton_instance_iterations": its_total,
         "graph_replayed": bool(info["graph_used"]),
+        "graph_fallback": bool(info["graph_fallback"]),
+        "engine_message": mix["message"],
         "ms_per_step_kernel_timing": (round(mix["seconds_timed"] / args.steps * 1e3, 3)
                                       if mix["seconds_timed"] else None),
         "converged": int((rm["verdict"] == CONVERGED).sum()),
@@ -272,9 +481,21 @@ def main():
             "verdicts_equal": int((rm["verdict"] == rf["verdict"]).sum()),
             "max_dV": float(np.abs(rm["V"][ok] - rf["V"][ok]).max()) if ok.any() else None,
         }
+    if not args.no_legs:
+        log("[bench] leg: set-up + solve per batch")
+        out["setup_plus_solve"] = setup_leg(case, cont, qlim, args.refine,
+                                            mix["tuning"], args.steps)
+        if args.workload == "pf9241x200":
+            log("[bench] leg: adjoint for every contingency")
+            out["adjoint"] = adjoint_leg(case, cont, qlim, args.refine, mix["tuning"])
+            log("[bench] leg: pf1354x200 Newton (mixed), FDXB, FDBX")
+            out["pf1354x200"] = fd_leg("pf1354x200", args.steps, args.warmup)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        res, dt = cpu_baseline(case, cont, qlim, threads)
+        # the box's CPU share for one GPU is 16 threads, whatever the host has
+        nproc, model = cpu_info()
+        threads = args.cpu_threads or min(16, nproc)
+        log(f"[bench] cpu baseline: {threads} threads, then 1 core")
+        res, cb = cpu_baseline(case, cont, qlim, threads)
         nsolve = len(res)
         its_eq = sum(int(rm["iterations"][t] == r["iterations"]) for t, r in res.items())
         ver_eq = sum(int(rm["verdict"][t] == r["verdict"]) for t, r in res.items())
@@ -283,13 +504,23 @@ def main():
                   if r["converged"] and rm["verdict"][t] == CONVERGED),
                  default=None)
         out["cpu_baseline"] = {
-            "value": round(nsolve / dt, 3), "unit": "solves/s",
+            "value": round(cb["multi"], 3), "unit": "solves/s",
             "cores": threads, "kind": "port",
             "sample": f"all {nsolve} contingencies of {args.workload} "
                       f"(oracle/pf_oracle.c, FP64, own sparse LU), one "
-                      f"contingency per thread, {dt:.1f} s wall",
+                      f"contingency per thread, median of "
+                      f"{len(cb['multi_runs'])} runs after 2 warm-up runs "
+                      f"({', '.join(f'{x:.2f}' for x in cb['multi_runs'])} s)",
+            "single_core": {
+                "value": round(cb["single"], 3), "cores": 1,
+                "sample": f"{cb['single_n']} contingencies spread over the "
+                          f"{nsolve} in sequence, "
+                          f"median of {len(cb['single_runs'])} runs after 2 "
+                          f"warm-up runs ({', '.join(f'{x:.2f}' for x in cb['single_runs'])} s)"},
+            "nproc": nproc, "cpu_model": model,
         }
-        out["speedup_vs_cpu"] = round(value / (nsolve / dt), 2)
+        out["speedup_vs_cpu"] = round(value / cb["multi"], 2)
+        out["speedup_vs_cpu_single_core"] = round(value / cb["single"], 2)
         out["parity_vs_oracle"] = {"instances": nsolve,
                                    "iterations_equal": its_eq,
                                    "verdicts_equal": ver_eq, "max_dV": dv}

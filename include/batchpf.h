@@ -87,6 +87,14 @@ typedef struct {
   int32_t use_graph;       /* replay each Newton iteration as one hipGraph */
   double tol;              /* 1e-8, runpf.h:47 */
   double qlim_tol;         /* pu */
+  /* tile sizes of the symbolic schedules (0: the largest the kernels hold;
+   * mapol_amd/tune.py picks them per case and instance count) */
+  int32_t tail_limit;      /* pivots of the dense tail, 1..64 */
+  int32_t chunk_products;  /* block products per factorization chunk, <= 4096 */
+  int32_t slot_chunk;      /* lane slots per slot-form sweep chunk, 64..2048 */
+  int32_t ordering;        /* 0 minimum degree; 1 the same, reordered to the
+                              least elimination-tree height (same fill);
+                              2 nested dissection */
 } bpf_options;
 
 /* one control of the reference's particle vector (Particula::Estrutura) */
@@ -121,6 +129,13 @@ typedef struct {
   int32_t slot_fw_chunks, slot_bw_chunks;    /* slot-form sweep chunks */
   int32_t slots;                   /* lane slots of the slot form (0: the
                                       slot form does not apply) */
+  int32_t graph_fallback;          /* the last solve abandoned graph replay
+                                      and ran with direct launches
+                                      (bpf_last_error says why) */
+  int32_t graph_captures;          /* Newton graphs captured since bpf_create
+                                      (a new capture follows a change of
+                                      instances, launch shapes or factor
+                                      precision) */
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */
@@ -193,8 +208,8 @@ bpf_status bpf_get_branch_loading_noalloc(bpf_engine *e, const double *rate,
 void bpf_destroy(bpf_engine *e);
 
 /* Last error message; when there is none but hipGraph replay had to be
- * abandoned (the solve then ran with direct launches), its reason, else a
- * note on what the runtime reported around replays (empty when nothing). */
+ * abandoned (the solve then ran with direct launches, bpf_info
+ * graph_fallback = 1), its reason; empty otherwise. */
 const char *bpf_last_error(const bpf_engine *e);
 bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info);
 
@@ -279,6 +294,17 @@ bpf_status bpf_analyse_sweeps(const bpf_case *c, bpf_info *info,
 bpf_status bpf_analyse_slots(const bpf_case *c, bpf_info *info,
                              int32_t *sl_chunk, int32_t *sl_slot);
 
+/* Host-only: the slot form of the adjoint's transposed sweeps (J^T = U^T
+ * L^T; the reference has no adjoint): U^T forward over the rows of L, every
+ * row applying its pivot inverse (transposed) at its last segment, then
+ * L^T backward over the rows of U that have entries.  The same slot layout
+ * as bpf_analyse_slots with every position replaced by its mirror (the
+ * block (k, i) for (i, k)), used transposed.  counts[3] = {forward chunks,
+ * backward chunks, slots} (zeros when the slot form does not apply);
+ * sl_chunk [4 * chunks], sl_slot [4 * slots]; arrays may be NULL. */
+bpf_status bpf_analyse_slots_transposed(const bpf_case *c, int32_t *counts,
+                                        int32_t *sl_chunk, int32_t *sl_slot);
+
 /* Host-only: the Jacobian fill of a Newton step, which also finishes level
  * 0 of the elimination tree (part of the factorization of newtonpf.h:312-363
  * moved into the fill).  jac_lu[nnz_jac_blocks] = L+U position of every
@@ -302,10 +328,35 @@ bpf_status bpf_analyse_tail(const bpf_case *c, bpf_info *info, int32_t *tl_stage
 
 /* ---- stage-level entry points (tests, benchmarks) ---------------------- */
 
-/* makeYbus (src/powerflow/makeYbus.h:372-510; CPU :113-274): Ybus of
- * instance t after bpf_solve, CSR with sorted columns; val interleaved. */
+/* hybrid_makeYbus / mkl_makeYbus (src/powerflow/makeYbus.h:372-510 and
+ * :113-274): assemble the Ybus of every instance from the instances set by
+ * bpf_set_instances, without solving (bpf_solve assembles it too). */
+bpf_status bpf_make_ybus(bpf_engine *e);
+
+/* Ybus of instance t after bpf_make_ybus or bpf_solve, CSR with sorted
+ * columns; val interleaved (re, im). */
 bpf_status bpf_get_ybus(bpf_engine *e, int32_t t, int32_t *rowptr,
                         int32_t *colind, double *val);
+
+/* hybrid_makeB / mkl_makeB (src/powerflow/makeB.h:109, :392): B' (which 1,
+ * rows and columns pv then pq) or B'' (which 2, over pq) of instance t for
+ * the engine's fast-decoupled variant, as COO triplets in that order
+ * (*nnz = entries; cap = capacity of the arrays).  Uses the instances' bus
+ * types as a solve starts them; overwrites the state of the last solve. */
+bpf_status bpf_stage_bmat(bpf_engine *e, int32_t t, int32_t which, int32_t cap,
+                          int32_t *row, int32_t *col, double *val,
+                          int32_t *nnz);
+
+/* One half-step of the fast-decoupled iteration (hybrid_fdpf / mkl_fdpf,
+ * src/powerflow/fdpf.h:99, :450; its P half-step is :480-520, the Q
+ * half-step :522-560) for every instance from the voltages V [T*nbus*2]:
+ * the mismatch divided by |V|, a solve with B' (which 1: angles of pv and pq
+ * buses) or B'' (which 2: magnitudes of pq buses), the update.  Vout
+ * receives the new voltages, norm [T] (may be NULL) the inf-norm of the
+ * normalised mismatch at V.  Needs bpf_make_ybus or bpf_solve first;
+ * overwrites the state of the last solve. */
+bpf_status bpf_stage_fd_halfstep(bpf_engine *e, const double *V, int32_t which,
+                                 double *Vout, double *norm);
 
 /* checkConvergence / computeDiagIbus (newtonpf.h:18-95): mismatch of every
  * instance at the given voltages V [T*nbus*2] for the instances' current bus
@@ -327,6 +378,17 @@ bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V, int32_t t,
  * and solve J dx = F; dx in the reference layout, stride 2*nbus. */
 bpf_status bpf_stage_newton_step(bpf_engine *e, const double *V, double *dx);
 
+/* The inverse of the dense tail's Schur complement of instance t, as the
+ * last factorization (bpf_stage_newton_step or bpf_solve) left it (no
+ * counterpart in the reference, whose LU is a library call; for tests).
+ * tail_bus [tail_n] = the bus of each tail pivot in elimination order;
+ * Sinv [(2 tail_n)^2] row-major, scalar index 2a + r for tail pivot a with
+ * r = 0 the angle / P component and r = 1 the magnitude / Q component, in
+ * the factorization's precision (FP32 values widened).  Either pointer may
+ * be NULL. */
+bpf_status bpf_stage_tail_inverse(bpf_engine *e, int32_t t, int32_t *tail_bus,
+                                  double *Sinv);
+
 /* Adjoint sensitivities at the converged point of the last bpf_solve.
  * Objective per instance: total active losses (runpf.h:228-257) plus
  * w * sum over branch ends of max(0, |S| - rate)^2.  Solves J^T lambda =
@@ -337,11 +399,12 @@ bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
                        double *grad_P, double *grad_Q, double *grad_x,
                        double *objective);
 
-/* Kernel timing on the engine's stream with hipEvents (enable before
- * bpf_solve).  With graph replay the events are event-record nodes of the
- * Newton-iteration graph, read after every replay.  Names: ybus, init,
- * mismatch, jacobian, lu_factor, lu_solve, update, loss, fd_solve,
- * adjoint. */
+/* Kernel timing on the engine's stream with hipEvents recorded around
+ * direct launches (enable before bpf_solve).  While timing is on, the
+ * Newton loop launches its kernels directly instead of replaying the
+ * iteration graph, so the events bracket single kernels.  Names: ybus, init,
+ * mismatch, jacobian, lu_factor, lu_solve, residual, update, loss, bmat,
+ * fd_solve, adjoint. */
 bpf_status bpf_set_timing(bpf_engine *e, int32_t enable);
 bpf_status bpf_get_kernel_time(const bpf_engine *e, const char *name,
                                int32_t *launches, double *total_ms);
@@ -350,6 +413,21 @@ bpf_status bpf_get_kernel_time(const bpf_engine *e, const char *name,
  * autotuner in mapol_amd/tune.py sets them.  0 keeps the current value. */
 bpf_status bpf_set_launch(bpf_engine *e, int32_t lu_threads,
                           int32_t solve_threads, int32_t mismatch_threads);
+
+/* Run-time schedule choices (the autotuner sets them; 0 keeps the current
+ * value): newton_per_graph = Newton iterations per captured graph (1..16;
+ * the host reads the active-instance counts once per round of replays);
+ * tail_inverse = 1 matrix cores, 2 vector ALUs (FP32 dense-tail inverse);
+ * sweep_form = 1 slot form, 2 entry-parallel, 3 level rows (the first that
+ * applies, in that order, from the chosen one); parts = 1..4: the Newton
+ * loop runs the batch as that many parts of consecutive instances, each on
+ * its own stream with its own graph, replayed side by side (results do not
+ * depend on it). */
+enum { BPF_TAIL_MFMA = 1, BPF_TAIL_VALU = 2 };
+enum { BPF_SWEEP_SLOT = 1, BPF_SWEEP_ENTRY = 2, BPF_SWEEP_ROWS = 3 };
+bpf_status bpf_set_schedule(bpf_engine *e, int32_t newton_per_graph,
+                            int32_t tail_inverse, int32_t sweep_form,
+                            int32_t parts);
 
 #ifdef __cplusplus
 }

@@ -79,37 +79,73 @@ struct bpf_engine {
   DevInst d{};
   const int8_t *d_type0 = nullptr;
   const double *d_Qinj0 = nullptr;
+  // case arrays of the per-instance expansion and its input staging
+  CaseArrays ca{};
+  uint8_t *in_bron = nullptr, *in_genon = nullptr;   // [T][m], [T][ng]
+  std::vector<DBuf> ctl_owned;
+  void *ctl_buf = nullptr;      // int2 [nctl] + double [T][nctl]
+  size_t ctl_bytes = 0;
   void *d_LU = nullptr;         // B4<float> or B4<double> [T][nnzLU]
   int lu_fp64 = -1;             // precision the LU buffer was sized for
   void *d_xglob = nullptr;      // solve vectors when LDS is too small
   void *d_TL = nullptr;         // inverse tail Schur complements [T][tn*tn]
+  // FP64 factor storage of the adjoint and the Jacobian export when the
+  // Newton factors are FP32 (so that neither disturbs the Newton buffers or
+  // the captured graph), and the adjoint's per-instance arrays; allocated on
+  // first use for the current instance count
+  struct Fp64Factors { void *LU = nullptr, *TL = nullptr, *xglob = nullptr; } f64;
+  struct AdjBufs {
+    double *rate = nullptr, *G = nullptr, *lam = nullptr, *gP = nullptr,
+           *gQ = nullptr, *gx = nullptr, *obj = nullptr;
+  } adj;
   cd *d_Sf = nullptr, *d_St = nullptr;
   double *d_loss = nullptr;
   int *h_count = nullptr;       // pinned
   // launch shapes
   int lu_threads = 1024, solve_threads = 1024, mis_threads = 256;
   int max_lds = 65536;
-  // graph of one Newton iteration
-  hipGraphExec_t graph = nullptr;
-  int graph_key = -1;
+  // graph of NEWTON_PER_GRAPH Newton iterations; destroyed whenever what it
+  // captured changes (launch shapes, factor storage, instances)
+  hipGraphExec_t graph = nullptr;   // part 0 (the whole batch when not split)
+  // the batch split into nsplit parts of consecutive instances, each with
+  // its own stream, graph and active-instance counter, replayed side by
+  // side so that one part's one-workgroup-per-instance kernels (factor,
+  // sweeps) overlap another part's many-workgroup kernels
+  static constexpr int MAX_SPLIT = 4;
+  int nsplit = 1;
+  hipStream_t sub[MAX_SPLIT] = {};   // sub[0] unused (the engine's stream)
+  hipGraphExec_t subgraph[MAX_SPLIT] = {};
+  int split_used = 1;                // parts of the captured graphs
   bool graph_broken = false;   // a replay failed: direct launches from then on
   std::string graph_err;
-  std::string graph_note;      // runtime oddities seen around replays
   int graph_used = 0;          // the last solve replayed graphs
+  int graph_fallback = 0;      // the last solve abandoned replay (graph_err says why)
+  int graph_captures = 0;      // captures since bpf_create
   // timing
   int timing = 0;
   std::vector<TimedEvent> events;
   std::string launch_err;            // first failed launch (kernel name)
   const char *cur_kernel = "";
   int debug_sync = 0;                // BPF_DEBUG_SYNC: sync after launches
-  int sweep_form = 0;                // sweep_mode's form (environment)
+  int sweep_form = 0;                // sweep_mode's form (0 slot, 1 entry, 2 rows)
+  int tail_form = 1;                 // BPF_TAIL_MFMA / BPF_TAIL_VALU
+  int per_graph = 4;                 // Newton iterations per captured graph
   bool capturing = false;            // recording the iteration graph
   std::map<std::string, std::pair<int, double>> ktime;
   bool solved = false;
+  bool ybus_ready = false;           // Ybus of the current instances assembled
   double *scratch = nullptr;         // this device's allocation-free scratch
 };
 
 static bpf_status scratch_setup(bpf_engine *e);
+
+// the captured Newton graphs (whole batch or its parts), dropped whenever
+// what they captured changes
+static void drop_graphs(bpf_engine *e) {
+  if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
+  for (int h = 1; h < bpf_engine::MAX_SPLIT; h++)
+    if (e->subgraph[h]) { hipGraphExecDestroy(e->subgraph[h]); e->subgraph[h] = nullptr; }
+}
 
 #define HIPCHK(expr)                                                        \
   do {                                                                      \
@@ -140,6 +176,13 @@ static bpf_status dalloc(bpf_engine *e, std::vector<DBuf> &pool, size_t bytes,
 // device (a zero element of `width` ints stands in for an empty one)
 static std::vector<int> nonempty(const std::vector<int> &v, int width) {
   return v.empty() ? std::vector<int>(width, 0) : v;
+}
+
+static std::vector<double> nonemptyd(const std::vector<double> &v) {
+  return v.empty() ? std::vector<double>(1, 0.0) : v;
+}
+static std::vector<uint8_t> nonempty8(const std::vector<uint8_t> &v) {
+  return v.empty() ? std::vector<uint8_t>(1, 0) : v;
 }
 
 template <typename V>
@@ -177,20 +220,26 @@ extern "C" void bpf_default_options(bpf_options *o) {
   o->use_graph = 1;
   o->tol = 1e-8;
   o->qlim_tol = 1e-6;
+  o->tail_limit = 0;
+  o->chunk_products = 0;
+  o->slot_chunk = 0;
+  o->ordering = 0;
 }
 
 extern "C" const char *bpf_last_error(const bpf_engine *e) {
   if (!e) return "null engine";
   if (e->err.empty() && e->graph_broken) return e->graph_err.c_str();
-  if (e->err.empty()) return e->graph_note.c_str();
   return e->err.c_str();
 }
 
 extern "C" void bpf_destroy(bpf_engine *e) {
   if (!e) return;
-  if (e->graph) hipGraphExecDestroy(e->graph);
+  drop_graphs(e);
+  for (int h = 1; h < bpf_engine::MAX_SPLIT; h++)
+    if (e->sub[h]) hipStreamDestroy(e->sub[h]);
   for (auto &ev : e->events) { hipEventDestroy(ev.a); hipEventDestroy(ev.b); }
   free_pool(e->inst_owned);
+  free_pool(e->ctl_owned);
   free_pool(e->owned);
   if (e->h_count) hipHostFree(e->h_count);
   if (e->stream) hipStreamDestroy(e->stream);
@@ -250,9 +299,16 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
       e->err = "generator bus out of range";
       return BPF_ERR_INVALID_ARG;
     }
-  if (!analyse(e->n, e->m, e->br_from.data(), e->br_to.data(),
-               e->bus_type.data(), e->S, &e->err))
-    return BPF_ERR_INVALID_ARG;
+  {
+    AnalysisParams prm;
+    prm.tail_limit = e->opt.tail_limit;
+    prm.chunk_pairs = e->opt.chunk_products;
+    prm.slot_chunk = e->opt.slot_chunk;
+    prm.ordering = e->opt.ordering;
+    if (!analyse(e->n, e->m, e->br_from.data(), e->br_to.data(),
+                 e->bus_type.data(), e->S, &e->err, prm))
+      return BPF_ERR_INVALID_ARG;
+  }
   // device
   const char *dbg = getenv("BPF_DEBUG_SYNC");
   e->debug_sync = (dbg && dbg[0] == '1') ? 1 : 0;
@@ -262,9 +318,12 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
     if (s0) return s0;
   }
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  for (int h = 1; h < bpf_engine::MAX_SPLIT; h++)
+    HIPCHK(hipStreamCreateWithFlags(&e->sub[h], hipStreamNonBlocking));
   HIPCHK(hipDeviceGetAttribute(&e->max_lds,
                                hipDeviceAttributeMaxSharedMemoryPerBlock, e->dev));
-  HIPCHK(hipHostMalloc((void **)&e->h_count, sizeof(int), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&e->h_count, sizeof(int) * bpf_engine::MAX_SPLIT,
+                       hipHostMallocDefault));
   const Symbolic &S = e->S;
   DevTopo &g = e->g;
   g.n = e->n; g.m = e->m; g.nb = S.nb;
@@ -304,15 +363,51 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(g.R, e->R); UP(g.X, e->X); UP(g.B, e->B); UP(g.shift, e->shift);
   UP(g.Gs, e->Gs); UP(g.Qd, e->Qd); UP(g.Va0, e->Va);
   UP(g.base_type, e->bus_type);
+  {
+    // case arrays of the per-instance expansion; generators grouped by bus
+    // in ascending generator order
+    CaseArrays &ca = e->ca;
+    std::vector<int> gptr(e->n + 1, 0), ggen(e->ng);
+    for (int k = 0; k < e->ng; k++) gptr[e->gen_bus[k] + 1]++;
+    for (int i = 0; i < e->n; i++) gptr[i + 1] += gptr[i];
+    {
+      std::vector<int> w(gptr.begin(), gptr.end() - 1);
+      for (int k = 0; k < e->ng; k++) ggen[w[e->gen_bus[k]]++] = k;
+    }
+    std::vector<uint8_t> bst(e->br_status), gst(e->gen_status);
+    for (auto &x : bst) x = x ? 1 : 0;
+    for (auto &x : gst) x = x ? 1 : 0;
+    UP(ca.Pd, e->Pd); UP(ca.Qd, e->Qd); UP(ca.Bs, e->Bs); UP(ca.Vm, e->Vm);
+    UP(ca.tap, nonemptyd(e->tap)); UP(ca.br_status, nonempty8(bst));
+    UP(ca.gb_ptr, gptr); UP(ca.gb_gen, nonempty(ggen, 1));
+    UP(ca.Pg, nonemptyd(e->Pg)); UP(ca.Qg, nonemptyd(e->Qg));
+    UP(ca.Qmax, nonemptyd(e->Qmax)); UP(ca.Qmin, nonemptyd(e->Qmin));
+    UP(ca.VG, nonemptyd(e->VG)); UP(ca.gen_status, nonempty8(gst));
+    ca.ng = e->ng;
+  }
   FactorPlan &F = e->fplan;
   F.nchunk = S.nchunk;
   UP(F.ch_task, S.ch_task); UP(F.ch_pair, S.ch_pair); UP(F.ch_max, S.ch_max);
   {
-    const int *ta, *tb, *pr;
-    UP(ta, nonempty(S.tk_a, 4)); UP(tb, nonempty(S.tk_b, 4));
+    // one descriptor per output: the pivot range of an L output packed into
+    // the fourth word (first product | length << TK_LEN_SHIFT)
+    if (S.npairs >= (int64_t)1 << TK_LEN_SHIFT) {
+      e->err = "factorization schedule has " + std::to_string(S.npairs) +
+               " products, more than the 2^19 its descriptors address";
+      return BPF_ERR_UNSUPPORTED;
+    }
+    std::vector<int> tk(S.tk_a.size());
+    for (size_t k = 0; k + 3 < S.tk_a.size(); k += 4) {
+      tk[k] = S.tk_a[k];
+      tk[k + 1] = S.tk_a[k + 1];
+      tk[k + 2] = S.tk_a[k + 2];
+      const unsigned v0 = (unsigned)S.tk_b[k + 1], len = (unsigned)(S.tk_b[k + 2] - S.tk_b[k + 1]);
+      tk[k + 3] = (int)(v0 | (len << TK_LEN_SHIFT));
+    }
+    const int *tp, *pr;
+    UP(tp, nonempty(tk, 4));
     UP(pr, nonempty(S.pairs, 2));
-    F.tk_a = (const int4 *)ta;
-    F.tk_b = (const int4 *)tb;
+    F.tk = (const int4 *)tp;
     F.pairs = (const int2 *)pr;
   }
   F.tn = S.tail_n;
@@ -350,12 +445,19 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   SP.sl_nbw = S.sl_nbw;
   {
     const int *ch, *sl;
+    const int *cht, *slt;
     UP(ch, nonempty(S.sl_chunk, 4)); UP(sl, nonempty(S.sl_slot, 4));
+    UP(cht, nonempty(S.sl_chunk_t, 4)); UP(slt, nonempty(S.sl_slot_t, 4));
     SP.sl_chunk = (const int4 *)ch;
     SP.sl_slot = (const int4 *)sl;
+    SP.sl_chunk_t = (const int4 *)cht;
+    SP.sl_slot_t = (const int4 *)slt;
+    SP.sl_nfw_t = S.sl_nfw_t;
+    SP.sl_nbw_t = S.sl_nbw_t;
   }
   const char *swr = getenv("BPF_SWEEP_ROWS"), *swe = getenv("BPF_SWEEP_ENTRY");
   e->sweep_form = (swr && swr[0] == '1') ? 2 : (swe && swe[0] == '1') ? 1 : 0;
+  e->tail_form = F.tail_mfma ? BPF_TAIL_MFMA : BPF_TAIL_VALU;
 #undef UP
   return BPF_OK;
 }
@@ -371,7 +473,29 @@ extern "C" bpf_status bpf_set_launch(bpf_engine *e, int32_t lu, int32_t sv,
   if (lu) e->lu_threads = lu;
   if (sv) e->solve_threads = sv;
   if (mis) e->mis_threads = mis;
-  if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
+  drop_graphs(e);
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_set_schedule(bpf_engine *e, int32_t per_graph,
+                                       int32_t tail_inverse, int32_t sweep_form,
+                                       int32_t parts) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  if (per_graph < 0 || per_graph > 16 || tail_inverse < 0 || tail_inverse > 2 ||
+      sweep_form < 0 || sweep_form > 3 || parts < 0 ||
+      parts > bpf_engine::MAX_SPLIT) {
+    e->err = "schedule out of range (newton_per_graph 1..16, tail_inverse 1..2, "
+             "sweep_form 1..3, parts 1..4)";
+    return BPF_ERR_INVALID_ARG;
+  }
+  if (per_graph) e->per_graph = per_graph;
+  if (parts) e->nsplit = parts;
+  if (tail_inverse) {
+    e->tail_form = tail_inverse;
+    e->fplan.tail_mfma = tail_inverse == BPF_TAIL_MFMA ? 1 : 0;
+  }
+  if (sweep_form) e->sweep_form = sweep_form - 1;
+  drop_graphs(e);
   return BPF_OK;
 }
 
@@ -392,104 +516,107 @@ extern "C" bpf_status bpf_set_instances(bpf_engine *e, int32_t T,
                : (k == BPF_CTL_OLTC) ? (id >= 0 && id < m) : false;
     if (!okk) { e->err = "control " + std::to_string(c) + " out of range"; return BPF_ERR_INVALID_ARG; }
   }
-  if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
-  free_pool(e->inst_owned);
-  // the factor storage lived in that pool too: forget it, so that ensure_lu
-  // allocates it for the new T instead of freeing the stale address again
-  // (which a new buffer of this call may have been given)
-  e->d_LU = e->d_xglob = e->d_TL = nullptr;
-  e->T = T;
   e->solved = false;
-  e->lu_fp64 = -1;
-  const size_t Tn = (size_t)T * n, Tm = (size_t)T * m;
-  std::vector<uint8_t> on(Tm);
-  std::vector<double> tapT(Tm), BsT(Tn), vset(Tn), P(Tn), Q(Tn), Qmx(Tn), Qmn(Tn);
-  std::vector<int8_t> ty(Tn);
-  std::vector<int> ngon(n);
-  std::vector<double> vg(n);
-  for (int t = 0; t < T; t++) {
-    const size_t on_ = (size_t)t * n, om = (size_t)t * m;
-    std::fill(ngon.begin(), ngon.end(), 0);
-    for (int i = 0; i < n; i++) {
-      ty[on_ + i] = (int8_t)e->bus_type[i];
-      P[on_ + i] = -e->Pd[i];
-      Q[on_ + i] = -e->Qd[i];
-      Qmx[on_ + i] = 0.0;
-      Qmn[on_ + i] = 0.0;
-      BsT[on_ + i] = e->Bs[i];
-      vg[i] = NAN;
-    }
-    for (int k = 0; k < ng; k++) {
-      int b = e->gen_bus[k];
-      bool g_on = e->gen_status[k] && (!gen_on || gen_on[(size_t)t * ng + k]);
-      if (!g_on) continue;
-      ngon[b]++;
-      P[on_ + b] += e->Pg[k];
-      Q[on_ + b] += e->Qg[k];
-      Qmx[on_ + b] += e->Qmax[k];
-      Qmn[on_ + b] += e->Qmin[k];
-      vg[b] = e->VG[k];   // the last in-service generator row sets VG
-    }
-    for (int i = 0; i < n; i++) {
-      if (e->bus_type[i] == BPF_PV && ngon[i] == 0) ty[on_ + i] = BPF_PQ;
-      double v = e->Vm[i];
-      if (e->bus_type[i] == BPF_PV && !std::isnan(vg[i])) v = vg[i];
-      vset[on_ + i] = v;
-    }
-    for (int b = 0; b < m; b++) {
-      on[om + b] = e->br_status[b] && (!br_on || br_on[om + b]);
-      tapT[om + b] = e->tap[b];
-    }
-    for (int c = 0; c < nctl; c++) {
-      double v = ctl_val[(size_t)t * nctl + c];
-      int id = ctl[c].id;
-      if (std::isnan(v)) continue;
-      if (ctl[c].kind == BPF_CTL_AVR) vset[on_ + id] = v;
-      else if (ctl[c].kind == BPF_CTL_SHC) BsT[on_ + id] = v;
-      else if (e->tap[id] != 0.0) tapT[om + id] = v;
-    }
-  }
-  DevInst &d = e->d;
-  d.T = T;
+  e->ybus_ready = false;
   bpf_status s;
-  const uint8_t *pu8;
-  const double *pd;
-  const int8_t *pi8;
-#define UPI(dst, vec, tmp) \
-  if ((s = upload(e, e->inst_owned, vec, &tmp))) return s; dst = tmp;
-  UPI(d.br_on, on, pu8);
-  UPI(d.tap, tapT, pd);
-  UPI(d.Bs, BsT, pd);
-  UPI(d.vset, vset, pd);
-  UPI(d.Qmax, Qmx, pd);
-  UPI(d.Qmin, Qmn, pd);
-  UPI(e->d_type0, ty, pi8);   // read-only after upload
-  UPI(e->d_Qinj0, Q, pd);
-  if ((s = upload(e, e->inst_owned, P, &pd))) return s;
-  d.Pinj = const_cast<double *>(pd);
-#undef UPI
   void *p;
-  const int nb = e->S.nb, nnzY = e->g.nnzY;
+  const size_t Tn = (size_t)T * n, Tm = (size_t)T * m;
+  DevInst &d = e->d;
+  // Per-instance buffers are kept while the instance count stays the same,
+  // so a new batch of contingencies costs the upload of its masks and one
+  // expansion kernel, and the captured Newton graph stays valid.
+  if (T != e->T || !d.Y) {
+    drop_graphs(e);
+    free_pool(e->inst_owned);
+    // the factor storage lived in that pool too: forget it, so that
+    // ensure_lu allocates it for the new T instead of freeing the stale
+    // address again (which a new buffer of this call may have been given)
+    e->d_LU = e->d_xglob = e->d_TL = nullptr;
+    e->f64 = bpf_engine::Fp64Factors();
+    e->adj = bpf_engine::AdjBufs();
+    e->lu_fp64 = -1;
+    e->T = 0;
+    d = DevInst();
+    d.T = T;
+    const int nb = e->S.nb, nnzY = e->g.nnzY;
 #define AL(dst, type, count) \
   if ((s = dalloc(e, e->inst_owned, sizeof(type) * (count), &p))) return s; dst = (type *)p;
-  AL(d.type, int8_t, Tn);
-  AL(d.Qinj, double, Tn);
-  AL(d.Y, cd, (size_t)T * nnzY);
-  AL(d.V, cd, Tn);
-  AL(d.Ib, cd, Tn);
-  AL(d.F, double, (size_t)T * 2 * nb);
-  AL(d.dx, double, (size_t)T * 2 * nb);
-  AL(d.rhs, double, (size_t)T * 2 * nb);
-  AL(d.st, InstState, T);
-  AL(d.active_count, int, 1);
-  AL(e->d_Sf, cd, Tm);
-  AL(e->d_St, cd, Tm);
-  AL(e->d_loss, double, T);
-  AL(d.part, double, (size_t)T * div_up(e->n, 64));
-  AL(d.partv, int, (size_t)T * div_up(e->n, 64));
-  AL(d.jsing, int, T);
+    AL(e->in_bron, uint8_t, std::max<size_t>(Tm, 1));
+    AL(e->in_genon, uint8_t, (size_t)T * std::max(ng, 1));
+    AL(d.br_on, uint8_t, std::max<size_t>(Tm, 1));
+    AL(d.tap, double, std::max<size_t>(Tm, 1));
+    AL(d.Bs, double, Tn);
+    AL(d.vset, double, Tn);
+    AL(d.Qmax, double, Tn);
+    AL(d.Qmin, double, Tn);
+    AL(e->d_type0, int8_t, Tn);
+    AL(e->d_Qinj0, double, Tn);
+    AL(d.Pinj, double, Tn);
+    AL(d.type, int8_t, Tn);
+    AL(d.Qinj, double, Tn);
+    AL(d.Y, cd, (size_t)T * nnzY);
+    AL(d.V, cd, Tn);
+    AL(d.Ib, cd, Tn);
+    AL(d.F, double, (size_t)T * 2 * nb);
+    AL(d.dx, double, (size_t)T * 2 * nb);
+    AL(d.rhs, double, (size_t)T * 2 * nb);
+    AL(d.st, InstState, T);
+    AL(d.active_count, int, bpf_engine::MAX_SPLIT);
+    AL(e->d_Sf, cd, std::max<size_t>(Tm, 1));
+    AL(e->d_St, cd, std::max<size_t>(Tm, 1));
+    AL(e->d_loss, double, T);
+    AL(d.part, double, (size_t)T * div_up(e->n, 64));
+    AL(d.partv, int, (size_t)T * div_up(e->n, 64));
+    AL(d.jsing, int, T);
 #undef AL
-  HIPCHK(hipMemset(d.jsing, 0, sizeof(int) * (size_t)T));
+    e->T = T;
+  }
+  // controls: descriptors and values, reallocated when they grow
+  if (nctl > 0) {
+    const size_t need = sizeof(int2) * nctl + sizeof(double) * (size_t)T * nctl;
+    if (need > e->ctl_bytes) {
+      free_pool(e->ctl_owned);
+      e->ctl_bytes = 0;
+      if ((s = dalloc(e, e->ctl_owned, need, &p))) return s;
+      e->ctl_buf = p;
+      e->ctl_bytes = need;
+    }
+  }
+  if (br_on && m)
+    HIPCHK(hipMemcpyAsync(e->in_bron, br_on, Tm, hipMemcpyHostToDevice, e->stream));
+  if (gen_on && ng)
+    HIPCHK(hipMemcpyAsync(e->in_genon, gen_on, (size_t)T * ng, hipMemcpyHostToDevice,
+                          e->stream));
+  int2 *d_ctl = nullptr;
+  double *d_ctlv = nullptr;
+  if (nctl > 0) {
+    std::vector<int2> cv(nctl);
+    for (int c = 0; c < nctl; c++) { cv[c].x = ctl[c].kind; cv[c].y = ctl[c].id; }
+    d_ctl = (int2 *)e->ctl_buf;
+    d_ctlv = (double *)(d_ctl + nctl);
+    HIPCHK(hipMemcpyAsync(d_ctl, cv.data(), sizeof(int2) * nctl,
+                          hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(d_ctlv, ctl_val, sizeof(double) * (size_t)T * nctl,
+                          hipMemcpyHostToDevice, e->stream));
+    // the host vector must outlive the copy
+    HIPCHK(hipStreamSynchronize(e->stream));
+  }
+  hipLaunchKernelGGL(k_expand, dim3(div_up(std::max(n, m), 256), T), dim3(256), 0,
+                     e->stream, e->g, d, e->ca, (br_on && m) ? e->in_bron : nullptr,
+                     (gen_on && ng) ? e->in_genon : nullptr, (int8_t *)e->d_type0,
+                     (double *)e->d_Qinj0, d.Pinj, (uint8_t *)d.br_on,
+                     (double *)d.tap, (double *)d.Bs, (double *)d.vset,
+                     (double *)d.Qmax, (double *)d.Qmin);
+  HIPCHK(hipGetLastError());
+  if (nctl > 0) {
+    hipLaunchKernelGGL(k_expand_ctl, dim3(div_up(T, 64)), dim3(64), 0, e->stream,
+                       e->g, d, e->ca.tap, nctl, (const int2 *)d_ctl,
+                       (const double *)d_ctlv, (double *)d.tap, (double *)d.Bs,
+                       (double *)d.vset);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemsetAsync(d.jsing, 0, sizeof(int) * (size_t)T, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
   return BPF_OK;
 }
 
@@ -517,7 +644,36 @@ static bpf_status ensure_lu(bpf_engine *e, int fp64) {
     if (s) return s;
   }
   e->lu_fp64 = fp64;
-  if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
+  drop_graphs(e);
+  return BPF_OK;
+}
+
+// FP64 factor storage for the adjoint and the Jacobian export: the Newton
+// buffers when they are FP64, else a second set kept for the instance count
+static bpf_status fp64_factors(bpf_engine *e, B4<double> **LU, B4<double> **TL,
+                               double **xglob) {
+  if (e->opt.precision == BPF_FP64) {
+    bpf_status s = ensure_lu(e, 1);
+    if (s) return s;
+    *LU = (B4<double> *)e->d_LU;
+    *TL = (B4<double> *)e->d_TL;
+    *xglob = (double *)e->d_xglob;
+    return BPF_OK;
+  }
+  auto &f = e->f64;
+  if (!f.LU) {
+    const size_t el = sizeof(B4<double>);
+    bpf_status s = dalloc(e, e->inst_owned, el * (size_t)e->T * e->g.nnzLU, &f.LU);
+    if (!s) s = dalloc(e, e->inst_owned,
+                       el * (size_t)e->T * e->S.tail_n * e->S.tail_n, &f.TL);
+    const size_t xs = sizeof(double) * 2 * (size_t)e->S.nb;
+    if (!s && xs > (size_t)SOLVE_LDS_BYTES)
+      s = dalloc(e, e->inst_owned, xs * e->T, &f.xglob);
+    if (s) { f = bpf_engine::Fp64Factors(); return s; }
+  }
+  *LU = (B4<double> *)f.LU;
+  *TL = (B4<double> *)f.TL;
+  *xglob = (double *)f.xglob;
   return BPF_OK;
 }
 
@@ -587,12 +743,14 @@ static void launch_mismatch(bpf_engine *e, int mode) {
 // workgroup size (SLOT_CHUNK / threads slots per lane)
 template <typename T>
 static void launch_solve(bpf_engine *e, int T_, const void *LU, int rhs_mode,
-                         int accumulate, int use_lds, int ep) {
+                         int accumulate, int use_lds, int ep,
+                         const void *TL = nullptr, void *xglob = nullptr) {
   const int th = e->solve_threads;
+  if (!TL) { TL = e->d_TL; xglob = e->d_xglob; }
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(T_), dim3(th), 0, e->stream, e->g, e->d,
-                       (const B4<T> *)LU, (const B4<T> *)e->d_TL, rhs_mode,
-                       accumulate, e->splan, use_lds, ep, (T *)e->d_xglob);
+                       (const B4<T> *)LU, (const B4<T> *)TL, rhs_mode,
+                       accumulate, e->splan, use_lds, ep, (T *)xglob);
   };
   if (ep == 2 && th == 1024) go(k_lu_solve<T, SLOT_CHUNK / 1024>);
   else if (ep == 2 && th == 512) go(k_lu_solve<T, SLOT_CHUNK / 512>);
@@ -648,86 +806,152 @@ static void launch_iteration(bpf_engine *e) {
 }
 
 
-// Newton iterations per captured graph (each replay is followed by one read
-// of the active-instance count)
-constexpr int NEWTON_PER_GRAPH = 4;
+
+// A part of the batch, the instances [t0, t1), as a view of the engine's
+// per-instance arrays: every pointer advanced by t0 instances, its own
+// active-instance counter and stream.
+struct PartView {
+  DevInst d;
+  int T;
+  void *LU, *TL, *xg;
+  hipStream_t s;
+  int *hc;
+};
+
+static PartView part_view(bpf_engine *e, int h, int K) {
+  const int t0 = (int)((long long)e->T * h / K), t1 = (int)((long long)e->T * (h + 1) / K);
+  const size_t n = e->n, m = e->m, nb2 = 2 * (size_t)e->S.nb, nY = e->g.nnzY;
+  const size_t np = div_up(e->n, e->mis_threads), o = t0;
+  PartView v;
+  DevInst d = e->d;
+  d.T = t1 - t0;
+  d.br_on += o * m; d.tap += o * m;
+  d.Bs += o * n; d.vset += o * n; d.type += o * n; d.Pinj += o * n;
+  d.Qinj += o * n; d.Qmax += o * n; d.Qmin += o * n;
+  d.Y += o * nY; d.V += o * n; d.Ib += o * n;
+  d.F += o * nb2; d.dx += o * nb2; d.rhs += o * nb2;
+  d.st += o; d.jsing += o; d.active_count += h;
+  d.part += o * np; d.partv += o * np;
+  v.d = d;
+  v.T = t1 - t0;
+  const size_t el = e->lu_fp64 ? sizeof(B4<double>) : sizeof(B4<float>);
+  const size_t tn = e->S.tail_n;
+  v.LU = (char *)e->d_LU + o * el * e->g.nnzLU;
+  v.TL = (char *)e->d_TL + o * el * tn * tn;
+  v.xg = e->d_xglob ? (char *)e->d_xglob + o * (e->lu_fp64 ? 8 : 4) * nb2 : nullptr;
+  v.s = h == 0 ? e->stream : e->sub[h];
+  v.hc = e->h_count + h;
+  return v;
+}
+
+// the engine's batch fields replaced by a part view for the launches made
+// in scope (they read e->d, e->T, the factor storage and e->stream)
+struct PartScope {
+  bpf_engine *e;
+  DevInst d;
+  int T;
+  void *LU, *TL, *xg;
+  hipStream_t s;
+  int *hc;
+  PartScope(bpf_engine *e_, const PartView &v)
+      : e(e_), d(e_->d), T(e_->T), LU(e_->d_LU), TL(e_->d_TL), xg(e_->d_xglob),
+        s(e_->stream), hc(e_->h_count) {
+    e->d = v.d; e->T = v.T; e->d_LU = v.LU; e->d_TL = v.TL; e->d_xglob = v.xg;
+    e->stream = v.s; e->h_count = v.hc;
+  }
+  ~PartScope() {
+    e->d = d; e->T = T; e->d_LU = LU; e->d_TL = TL; e->d_xglob = xg;
+    e->stream = s; e->h_count = hc;
+  }
+};
 
 static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
   bpf_status s = ensure_lu(e, e->opt.precision == BPF_FP64 ? 1 : 0);
   if (s) return s;
-  const int key = e->lu_fp64 * 10000000 +
-                  e->lu_threads * 10000 + e->mis_threads * 10 / 64 +
-                  e->solve_threads / 64;
   // per-kernel timing uses events around direct launches: elapsed times
   // between event nodes of a replayed graph are not reliable on this runtime
   bool use_graph = e->opt.use_graph != 0 && !e->graph_broken && !e->timing;
-  if (use_graph && (!e->graph || e->graph_key != key)) {
-    if (e->graph) { hipGraphExecDestroy(e->graph); e->graph = nullptr; }
-    hipGraph_t gr;
-    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    e->capturing = true;
-    for (int j = 0; j < NEWTON_PER_GRAPH; j++) launch_iteration(e);
-    e->capturing = false;
-    HIPCHK(hipStreamEndCapture(e->stream, &gr));
-    hipError_t r = hipGraphInstantiate(&e->graph, gr, nullptr, nullptr, 0);
-    hipGraphDestroy(gr);
-    if (r != hipSuccess) {
-      e->graph = nullptr;
-      e->graph_broken = true;
-      e->graph_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(r);
-      (void)hipGetLastError();
-      use_graph = false;
-    } else {
-      e->graph_key = key;
+  int K = use_graph ? std::max(1, std::min(e->nsplit, e->T)) : 1;
+  if (use_graph && e->graph && e->split_used != K) drop_graphs(e);
+  hipGraphExec_t *slot[bpf_engine::MAX_SPLIT] = {&e->graph};
+  for (int h = 1; h < bpf_engine::MAX_SPLIT; h++) slot[h] = &e->subgraph[h];
+  PartView view[bpf_engine::MAX_SPLIT];
+  for (int h = 0; h < K; h++) view[h] = part_view(e, h, K);
+  if (use_graph && !e->graph) {
+    for (int h = 0; h < K && use_graph; h++) {
+      PartScope ps(e, view[h]);
+      hipGraph_t gr;
+      HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      e->capturing = true;
+      for (int j = 0; j < e->per_graph; j++) launch_iteration(e);
+      e->capturing = false;
+      HIPCHK(hipStreamEndCapture(e->stream, &gr));
+      e->graph_captures++;
+      hipError_t r = hipGraphInstantiate(slot[h], gr, nullptr, nullptr, 0);
+      hipGraphDestroy(gr);
+      if (r != hipSuccess) {
+        *slot[h] = nullptr;
+        e->graph_broken = true;
+        e->graph_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(r);
+        e->graph_fallback = 1;
+        (void)hipGetLastError();
+        use_graph = false;
+      }
     }
+    if (!use_graph) {
+      // direct launches of the whole batch from here on
+      drop_graphs(e);
+      K = 1;
+      view[0] = part_view(e, 0, 1);
+    }
+    e->split_used = K;
   }
   e->graph_used = use_graph ? 1 : 0;
   const int cap = e->opt.maxit_nr * (e->opt.max_qlim_rounds + 1) + 2;
-  // an error left by an earlier call must not be read as this loop's
-  hipError_t stale = hipGetLastError();
-  if (stale != hipSuccess && e->graph_err.empty())
-    e->graph_err = std::string("error pending before the Newton loop: ") +
-                   hipGetErrorName(stale);
-  // The graph holds NEWTON_PER_GRAPH iterations and the active count is read
-  // back once per replay: every kernel of an iteration returns at once for a
-  // finished instance, so the iterations replayed past the last one change
-  // nothing (the count stays 0), and the host waits once per replay.
-  const int ahead = use_graph ? NEWTON_PER_GRAPH : 1;
+  // Each graph holds per_graph iterations (4 by default) and its part's
+  // active count is read back once per replay: every kernel of an iteration
+  // returns at once for a finished instance, so the iterations replayed
+  // past the last one change nothing (the count stays 0), and the host
+  // waits once per round of replays.  A part whose count reached 0 is not
+  // replayed again.
+  const int ahead = use_graph ? e->per_graph : 1;
+  bool active[bpf_engine::MAX_SPLIT];
+  for (int h = 0; h < K; h++) active[h] = true;
   for (int it = 0; it < cap; it += ahead) {
     hipError_t r = hipSuccess;
     const char *where = "";
     if (use_graph) {
-      where = "hipGraphLaunch";
-      r = hipGraphLaunch(e->graph, e->stream);
-      if (r == hipSuccess) {
-        where = "hipMemcpyAsync";
-        r = hipMemcpyAsync(e->h_count, e->d.active_count, sizeof(int),
-                           hipMemcpyDeviceToHost, e->stream);
+      for (int h = 0; h < K && r == hipSuccess; h++) {
+        if (!active[h]) continue;
+        where = "hipGraphLaunch";
+        r = hipGraphLaunch(*slot[h], view[h].s);
+        if (r == hipSuccess) {
+          where = "hipMemcpyAsync";
+          r = hipMemcpyAsync(view[h].hc, view[h].d.active_count, sizeof(int),
+                             hipMemcpyDeviceToHost, view[h].s);
+        }
       }
     } else {
       launch_iteration(e);
     }
-    if (r == hipSuccess) { where = "hipStreamSynchronize"; r = hipStreamSynchronize(e->stream); }
+    for (int h = 0; h < K && r == hipSuccess; h++) {
+      if (!active[h]) continue;
+      where = "hipStreamSynchronize";
+      r = hipStreamSynchronize(view[h].s);
+    }
     if (r == hipSuccess) {
+      // bpf_solve cleared the thread's error state on entry, so anything
+      // here was raised by this loop
       where = "hipGetLastError";
-      hipError_t le = hipGetLastError();
-      if (use_graph) {
-        // a replay's launch errors come back from hipGraphLaunch and its
-        // execution errors from the synchronization; the runtime has been
-        // seen to leave an unrelated error code behind after a replay that
-        // ran fine, which is noted, not taken as a failure
-        if (le != hipSuccess && e->graph_note.empty())
-          e->graph_note = std::string("after graph replay the runtime reported ") +
-                          hipGetErrorName(le) + " (ignored)";
-      } else {
-        r = le;
-      }
+      r = hipGetLastError();
     }
     if (r != hipSuccess) {
       if (use_graph) {
         // the runtime refused the replay: remember why and let bpf_solve
-        // start the solve again with direct launches
+        // start the solve again with direct launches; the fallback is
+        // reported by bpf_get_info and bpf_last_error
         e->graph_broken = true;
+        e->graph_fallback = 1;
         e->graph_err = std::string("graph replay (") + where + ", iteration " +
                        std::to_string(it) + "): " + hipGetErrorName(r) + " " +
                        hipGetErrorString(r);
@@ -739,7 +963,12 @@ static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
                (e->launch_err.empty() ? "" : " [" + e->launch_err + "]");
       return BPF_ERR_HIP;
     }
-    if (*e->h_count == 0) break;
+    bool any = false;
+    for (int h = 0; h < K; h++) {
+      active[h] = active[h] && view[h].hc[0] > 0;
+      any = any || active[h];
+    }
+    if (!any) break;
   }
   return BPF_OK;
 }
@@ -749,9 +978,10 @@ static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
 // is a masked solve with the same factors, an update and the |V|-normalised
 // mismatch with the reference's convergence checks
 static bpf_status run_fd(bpf_engine *e) {
-  bpf_status s = ensure_lu(e, 1);
+  B4<double> *LU, *TLd;
+  double *xg;
+  bpf_status s = fp64_factors(e, &LU, &TLd, &xg);
   if (s) return s;
-  B4<double> *LU = (B4<double> *)e->d_LU;
   const unsigned T = (unsigned)e->T;
   tbegin(e, "bmat");
   hipLaunchKernelGGL(k_bmat, dim3(div_up(e->g.nnzY, 256), T), dim3(256), 0,
@@ -759,8 +989,7 @@ static bpf_status run_fd(bpf_engine *e) {
   tend(e);
   tbegin(e, "lu_factor");
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(T), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, 0, e->fplan, LU,
-                     (B4<double> *)e->d_TL, 0, e->d.jsing);
+                     e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing);
   tend(e);
   size_t xs = sizeof(double) * 2 * (size_t)e->S.nb;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
@@ -769,7 +998,8 @@ static bpf_status run_fd(bpf_engine *e) {
       tbegin(e, "fd_solve");
       launch_solve<double>(e, T, LU, half, 0, use_lds,
                            sweep_mode(e->S, sizeof(double), use_lds,
-                                      e->sweep_form, e->solve_threads));
+                                      e->sweep_form, e->solve_threads),
+                           TLd, xg);
       tend(e);
       tbegin(e, "update");
       hipLaunchKernelGGL(k_fd_update, xgrid(T, e->n, 256), dim3(256),
@@ -828,6 +1058,11 @@ extern "C" bpf_status bpf_solve(bpf_engine *e) {
     return BPF_ERR_UNSUPPORTED;
   }
   e->graph_used = 0;
+  e->graph_fallback = 0;
+  // an error code left in this thread by the caller's own HIP calls (or a
+  // library's) is not this solve's: clear it so that the checks below read
+  // only what the solve itself raises
+  (void)hipGetLastError();
   bool graph_failed = false;
   bpf_status s = solve_once(e, &graph_failed);
   if (!s && graph_failed) {
@@ -838,6 +1073,18 @@ extern "C" bpf_status bpf_solve(bpf_engine *e) {
   tcollect(e);
   if (s) return s;
   e->solved = true;
+  e->ybus_ready = true;
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_make_ybus(bpf_engine *e) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  if (e->T <= 0) { e->err = "bpf_set_instances not called"; return BPF_ERR_STATE; }
+  hipLaunchKernelGGL(k_ybus, xgrid(e->T, e->g.nnzY, 256), dim3(256), 0,
+                     e->stream, e->g, e->d);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->ybus_ready = true;
   return BPF_OK;
 }
 
@@ -1076,6 +1323,29 @@ extern "C" bpf_status bpf_analyse_slots(const bpf_case *c, bpf_info *info,
   return BPF_OK;
 }
 
+extern "C" bpf_status bpf_analyse_slots_transposed(const bpf_case *c,
+                                                   int32_t *counts,
+                                                   int32_t *sl_chunk,
+                                                   int32_t *sl_slot) {
+  if (!c || !counts || c->nbus <= 0 || c->nbranch < 0 || !c->bus_type ||
+      (c->nbranch && (!c->br_from || !c->br_to)))
+    return BPF_ERR_INVALID_ARG;
+  Symbolic S;
+  std::string err;
+  if (!analyse(c->nbus, c->nbranch, c->br_from, c->br_to, c->bus_type, S, &err))
+    return BPF_ERR_INVALID_ARG;
+  counts[0] = S.sl_ok_t ? S.sl_nfw_t : 0;
+  counts[1] = S.sl_ok_t ? S.sl_nbw_t : 0;
+  counts[2] = S.sl_ok_t ? (int)S.sl_slot_t.size() / 4 : 0;
+  if (S.sl_ok_t) {
+    if (sl_chunk && !S.sl_chunk_t.empty())
+      memcpy(sl_chunk, S.sl_chunk_t.data(), sizeof(int32_t) * S.sl_chunk_t.size());
+    if (sl_slot && !S.sl_slot_t.empty())
+      memcpy(sl_slot, S.sl_slot_t.data(), sizeof(int32_t) * S.sl_slot_t.size());
+  }
+  return BPF_OK;
+}
+
 extern "C" bpf_status bpf_analyse_fill(const bpf_case *c, bpf_info *info,
                                        int32_t *jac_lu, int32_t *jac_l0) {
   if (!c || !info || c->nbus <= 0 || c->nbranch < 0 || !c->bus_type ||
@@ -1153,6 +1423,8 @@ extern "C" bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info) {
   fill_info(S, info);
   info->instances = e->T;
   info->graph_used = e->graph_used;
+  info->graph_fallback = e->graph_fallback;
+  info->graph_captures = e->graph_captures;
   int fp64 = e->opt.precision == BPF_FP64;
   info->lds_solve = ((fp64 ? 8 : 4) * 2 * (size_t)S.nb) <= (size_t)SOLVE_LDS_BYTES;
   return BPF_OK;
@@ -1177,7 +1449,10 @@ extern "C" bpf_status bpf_get_kernel_time(const bpf_engine *e, const char *name,
 extern "C" bpf_status bpf_get_ybus(bpf_engine *e, int32_t t, int32_t *rowptr,
                                    int32_t *colind, double *val) {
   if (!e || t < 0 || t >= e->T) return BPF_ERR_INVALID_ARG;
-  if (!e->solved) { e->err = "no solution yet"; return BPF_ERR_STATE; }
+  if (!e->ybus_ready) {
+    e->err = "no Ybus yet: call bpf_make_ybus or bpf_solve";
+    return BPF_ERR_STATE;
+  }
   if (rowptr) memcpy(rowptr, e->S.yrow.data(), sizeof(int) * (e->n + 1));
   if (colind) memcpy(colind, e->S.ycol.data(), sizeof(int) * e->S.ycol.size());
   if (val)
@@ -1239,14 +1514,15 @@ extern "C" bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V,
   if (!e || !V || t < 0 || t >= e->T || !nnz) return BPF_ERR_INVALID_ARG;
   bpf_status s = stage_eval(e, V);
   if (s) return s;
-  if ((s = ensure_lu(e, 1))) return s;
+  B4<double> *LU, *TLd;
+  double *xg;
+  if ((s = fp64_factors(e, &LU, &TLd, &xg))) return s;
   // mark instances active for the fill
   std::vector<InstState> st(e->T);
   HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * e->T, hipMemcpyDeviceToHost));
   std::vector<InstState> st2 = st;
   for (auto &x : st2) x.done = 0;
   HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
-  B4<double> *LU = (B4<double> *)e->d_LU;
   HIPCHK(hipMemset(LU, 0, sizeof(B4<double>) * (size_t)e->T * e->g.nnzLU));
   hipLaunchKernelGGL(k_jacobian<double>, xgrid(e->T, ((size_t)e->g.nJb + 1) / 2, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU, 0);
@@ -1328,71 +1604,229 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
     e->err = "the adjoint uses the Newton Jacobian: solve with BPF_NR";
     return BPF_ERR_UNSUPPORTED;
   }
-  bpf_status s = ensure_lu(e, 1);
+  B4<double> *LU, *TLd;
+  double *xg;
+  bpf_status s = fp64_factors(e, &LU, &TLd, &xg);
   if (s) return s;
   const size_t T = e->T, n = e->n, m = e->m, nb2 = 2 * (size_t)e->S.nb;
-  std::vector<DBuf> tmp;
-  void *p;
-  double *d_rate = nullptr, *d_G, *d_lam, *d_gP, *d_gQ, *d_gx, *d_obj;
-  auto cleanup = [&]() { free_pool(tmp); };
-#define TAL(dst, count)                                                    \
-  if ((s = dalloc(e, tmp, sizeof(double) * (count), &p))) { cleanup(); return s; } \
-  dst = (double *)p;
-  if (rate) {
-    TAL(d_rate, m);
-    hipMemcpy(d_rate, rate, sizeof(double) * m, hipMemcpyHostToDevice);
+  auto &a = e->adj;
+  if (!a.G) {
+    void *p;
+    double **dst[] = {&a.rate, &a.G, &a.lam, &a.gP, &a.gQ, &a.gx, &a.obj};
+    const size_t cnt[] = {std::max<size_t>(m, 1), T * nb2, T * nb2, T * n, T * n,
+                          T * std::max<size_t>(m, 1), T};
+    for (int k = 0; k < 7; k++) {
+      if ((s = dalloc(e, e->inst_owned, sizeof(double) * cnt[k], &p))) {
+        a = bpf_engine::AdjBufs();
+        return s;
+      }
+      *dst[k] = (double *)p;
+    }
   }
-  TAL(d_G, T * nb2);
-  TAL(d_lam, T * nb2);
-  TAL(d_gP, T * n);
-  TAL(d_gQ, T * n);
-  TAL(d_gx, T * m);
-  TAL(d_obj, T);
-#undef TAL
+  if (rate && m)
+    HIPCHK(hipMemcpyAsync(a.rate, rate, sizeof(double) * m, hipMemcpyHostToDevice,
+                          e->stream));
+  const double *d_rate = rate ? a.rate : nullptr;
   // only converged instances take part
   std::vector<InstState> st(T), st2;
   HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * T, hipMemcpyDeviceToHost));
   st2 = st;
   for (auto &x : st2) x.done = x.verdict == BPF_V_CONVERGED ? 0 : 1;
   HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * T, hipMemcpyHostToDevice));
-  B4<double> *LU = (B4<double> *)e->d_LU;
   const unsigned TT = (unsigned)T;
   tbegin(e, "adjoint");
   hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, ((size_t)e->g.nJb + 1) / 2, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU, 0);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, 0, e->fplan, LU,
-                     (B4<double> *)e->d_TL, 0, e->d.jsing);
+                     e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing);
   hipLaunchKernelGGL(k_adj_rhs, dim3(div_up(e->S.nb, 256), TT), dim3(256), 0,
-                     e->stream, e->g, e->d, (const double *)d_rate, w, d_G);
+                     e->stream, e->g, e->d, d_rate, w, a.G);
   size_t xs = sizeof(double) * nb2;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
-  hipLaunchKernelGGL(k_adj_solve, dim3(TT), dim3(e->solve_threads),
-                     0, e->stream, e->g, e->d, (const B4<double> *)LU,
-                     (const B4<double> *)e->d_TL, (const double *)d_G, d_lam,
-                     e->splan, use_lds, (double *)e->d_xglob);
+  // the slot form of the sweeps, transposed, where the Newton solve would
+  // use it (512 lanes: in FP64 with the pivots in the forward chunks, 1024
+  // lanes would spill registers); else one lane group per row by levels
+  const int th = e->solve_threads;
+  if (e->S.sl_ok_t &&
+      sweep_mode(e->S, sizeof(double), use_lds, e->sweep_form, 512) == 2) {
+    hipLaunchKernelGGL(k_adj_solve_slot<SLOT_CHUNK / 512>, dim3(TT), dim3(512), 0,
+                       e->stream, e->g, e->d, (const B4<double> *)LU,
+                       (const B4<double> *)TLd, (const double *)a.G, a.lam,
+                       e->splan);
+  } else {
+    hipLaunchKernelGGL(k_adj_solve, dim3(TT), dim3(th),
+                       0, e->stream, e->g, e->d, (const B4<double> *)LU,
+                       (const B4<double> *)TLd, (const double *)a.G, a.lam,
+                       e->splan, use_lds, xg);
+  }
   hipLaunchKernelGGL(k_adj_grad, dim3(div_up(std::max(n, m), 256), TT),
                      dim3(256), 0, e->stream, e->g, e->d,
-                     (const double *)d_lam, (const double *)d_rate, w, d_gP,
-                     d_gQ, d_gx);
+                     (const double *)a.lam, d_rate, w, a.gP, a.gQ, a.gx);
   hipLaunchKernelGGL(k_objective, dim3(TT), dim3(256), 0, e->stream, e->g,
-                     e->d, (const double *)d_rate, w, d_obj);
+                     e->d, d_rate, w, a.obj);
   tend(e);
   hipError_t r1 = hipStreamSynchronize(e->stream);
   hipError_t r2 = hipGetLastError();
   tcollect(e);
   // factorization singularities are not expected at a converged point; keep
   // the verdicts of the solve either way
-  hipMemcpy(e->d.st, st.data(), sizeof(InstState) * T, hipMemcpyHostToDevice);
+  HIPCHK(hipMemcpy(e->d.st, st.data(), sizeof(InstState) * T, hipMemcpyHostToDevice));
   if (r1 != hipSuccess || r2 != hipSuccess) {
     e->err = std::string("adjoint: ") + hipGetErrorString(r1 != hipSuccess ? r1 : r2);
-    cleanup();
     return BPF_ERR_HIP;
   }
-  if (gP) hipMemcpy(gP, d_gP, sizeof(double) * T * n, hipMemcpyDeviceToHost);
-  if (gQ) hipMemcpy(gQ, d_gQ, sizeof(double) * T * n, hipMemcpyDeviceToHost);
-  if (gx) hipMemcpy(gx, d_gx, sizeof(double) * T * m, hipMemcpyDeviceToHost);
-  if (obj) hipMemcpy(obj, d_obj, sizeof(double) * T, hipMemcpyDeviceToHost);
-  cleanup();
+  if (gP) HIPCHK(hipMemcpy(gP, a.gP, sizeof(double) * T * n, hipMemcpyDeviceToHost));
+  if (gQ) HIPCHK(hipMemcpy(gQ, a.gQ, sizeof(double) * T * n, hipMemcpyDeviceToHost));
+  if (gx) HIPCHK(hipMemcpy(gx, a.gx, sizeof(double) * T * m, hipMemcpyDeviceToHost));
+  if (obj) HIPCHK(hipMemcpy(obj, a.obj, sizeof(double) * T, hipMemcpyDeviceToHost));
+  return BPF_OK;
+}
+
+// bus order of the reference's fast-decoupled matrices (makeB.h:311-501):
+// B' over pv then pq, B'' over pq, buses ascending within each group
+static void fd_index(const bpf_engine *e, const int8_t *ty, int which,
+                     std::vector<int> &idx) {
+  idx.assign(e->n, -1);
+  int k = 0;
+  if (which == 1)
+    for (int i = 0; i < e->n; i++) if (ty[i] == BPF_PV) idx[i] = k++;
+  for (int i = 0; i < e->n; i++)
+    if (ty[i] == BPF_PQ && e->S.blk_of_bus[i] >= 0) idx[i] = k++;
+}
+
+// instances' bus types and state as a solve starts them (k_init), then
+// the caller's voltages
+static bpf_status stage_start(bpf_engine *e, const double *V) {
+  hipLaunchKernelGGL(k_init, xgrid(e->T, e->n, 256), dim3(256), 0, e->stream,
+                     e->g, e->d, (const int8_t *)e->d_type0,
+                     (const double *)e->d_Qinj0);
+  HIPCHK(hipGetLastError());
+  if (V)
+    HIPCHK(hipMemcpyAsync(e->d.V, V, sizeof(cd) * (size_t)e->T * e->n,
+                          hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  e->solved = false;   // the state of the last solve is overwritten
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_stage_bmat(bpf_engine *e, int32_t t, int32_t which,
+                                     int32_t cap, int32_t *row, int32_t *col,
+                                     double *val, int32_t *nnz) {
+  if (!e || t < 0 || t >= e->T || (which != 1 && which != 2) || !nnz)
+    return BPF_ERR_INVALID_ARG;
+  if (e->opt.alg == BPF_NR) {
+    e->err = "B' and B'' belong to the fast-decoupled variants (BPF_FDXB / BPF_FDBX)";
+    return BPF_ERR_UNSUPPORTED;
+  }
+  B4<double> *LU, *TLd;
+  double *xg;
+  bpf_status s = fp64_factors(e, &LU, &TLd, &xg);
+  if (s) return s;
+  if ((s = stage_start(e, nullptr))) return s;
+  HIPCHK(hipMemsetAsync(LU, 0, sizeof(B4<double>) * (size_t)e->T * e->g.nnzLU,
+                        e->stream));
+  hipLaunchKernelGGL(k_bmat, dim3(div_up(e->g.nnzY, 256), e->T), dim3(256), 0,
+                     e->stream, e->g, e->d, e->opt.alg, LU);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  std::vector<B4<double>> blk(e->g.nnzLU);
+  HIPCHK(hipMemcpy(blk.data(), LU + (size_t)t * e->g.nnzLU,
+                   sizeof(B4<double>) * e->g.nnzLU, hipMemcpyDeviceToHost));
+  std::vector<int8_t> ty(e->n);
+  HIPCHK(hipMemcpy(ty.data(), e->d.type + (size_t)t * e->n, e->n, hipMemcpyDeviceToHost));
+  std::vector<int> idx;
+  fd_index(e, ty.data(), which, idx);
+  const Symbolic &S = e->S;
+  int cnt = 0;
+  for (int p = 0; p < e->g.nnzY; p++) {
+    const int q = S.y2lu[p];
+    if (q < 0) continue;
+    const int i = idx[S.yrowof[p]], j = idx[S.ycol[p]];
+    if (i < 0 || j < 0) continue;
+    if (cnt < cap && row && col && val) {
+      row[cnt] = i;
+      col[cnt] = j;
+      val[cnt] = which == 1 ? blk[q].a : blk[q].d;
+    }
+    cnt++;
+  }
+  *nnz = cnt;
+  return cnt <= cap ? BPF_OK : BPF_ERR_INVALID_ARG;
+}
+
+extern "C" bpf_status bpf_stage_fd_halfstep(bpf_engine *e, const double *V,
+                                            int32_t which, double *Vout,
+                                            double *norm) {
+  if (!e || !V || !Vout || (which != 1 && which != 2)) return BPF_ERR_INVALID_ARG;
+  if (e->opt.alg == BPF_NR) {
+    e->err = "the half-steps belong to the fast-decoupled variants (BPF_FDXB / BPF_FDBX)";
+    return BPF_ERR_UNSUPPORTED;
+  }
+  if (!e->ybus_ready) {
+    e->err = "no Ybus yet: call bpf_make_ybus or bpf_solve";
+    return BPF_ERR_STATE;
+  }
+  B4<double> *LU, *TLd;
+  double *xg;
+  bpf_status s = fp64_factors(e, &LU, &TLd, &xg);
+  if (s) return s;
+  if ((s = stage_start(e, V))) return s;
+  const unsigned T = (unsigned)e->T;
+  // B' and B'' factored, the |V|-normalised mismatch, one masked solve and
+  // the angle (P) or magnitude (Q) update: the body of fdpf.h:480-560
+  hipLaunchKernelGGL(k_bmat, dim3(div_up(e->g.nnzY, 256), T), dim3(256), 0,
+                     e->stream, e->g, e->d, e->opt.alg, LU);
+  hipLaunchKernelGGL(k_lu_factor<double>, dim3(T), dim3(e->lu_threads), 0,
+                     e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing);
+  mismatch_pair(e, 2, e->opt.maxit_fd, 0, 1);
+  const size_t xs = sizeof(double) * 2 * (size_t)e->S.nb;
+  const int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
+  launch_solve<double>(e, T, LU, which, 0, use_lds,
+                       sweep_mode(e->S, sizeof(double), use_lds, e->sweep_form,
+                                  e->solve_threads),
+                       TLd, xg);
+  hipLaunchKernelGGL(k_fd_update, xgrid(T, e->n, 256), dim3(256), 0, e->stream,
+                     e->g, e->d, which);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  HIPCHK(hipMemcpy(Vout, e->d.V, sizeof(cd) * (size_t)T * e->n, hipMemcpyDeviceToHost));
+  if (norm) {
+    std::vector<InstState> st(T);
+    HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * T, hipMemcpyDeviceToHost));
+    for (unsigned t = 0; t < T; t++) norm[t] = st[t].norm;
+  }
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_stage_tail_inverse(bpf_engine *e, int32_t t,
+                                             int32_t *tail_bus, double *Sinv) {
+  if (!e || t < 0 || t >= e->T) return BPF_ERR_INVALID_ARG;
+  const int s = e->S.tail_n;
+  if (tail_bus)
+    for (int a = 0; a < s; a++) tail_bus[a] = e->S.bus_of_blk[e->S.tail_piv[a]];
+  if (!Sinv || s == 0) return BPF_OK;
+  if (!e->d_TL || e->lu_fp64 < 0) {
+    e->err = "no factorization yet: call bpf_stage_newton_step or bpf_solve";
+    return BPF_ERR_STATE;
+  }
+  const size_t ss = (size_t)s * s, w = 2 * (size_t)s;
+  auto put = [&](size_t k, double a, double b, double c, double d) {
+    const size_t r = 2 * (k / s), col = 2 * (k % s);
+    Sinv[r * w + col] = a;
+    Sinv[r * w + col + 1] = b;
+    Sinv[(r + 1) * w + col] = c;
+    Sinv[(r + 1) * w + col + 1] = d;
+  };
+  if (e->lu_fp64) {
+    std::vector<B4<double>> v(ss);
+    HIPCHK(hipMemcpy(v.data(), (const B4<double> *)e->d_TL + (size_t)t * ss,
+                     sizeof(B4<double>) * ss, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < ss; k++) put(k, v[k].a, v[k].b, v[k].c, v[k].d);
+  } else {
+    std::vector<B4<float>> v(ss);
+    HIPCHK(hipMemcpy(v.data(), (const B4<float> *)e->d_TL + (size_t)t * ss,
+                     sizeof(B4<float>) * ss, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < ss; k++) put(k, v[k].a, v[k].b, v[k].c, v[k].d);
+  }
   return BPF_OK;
 }

@@ -138,6 +138,77 @@ __global__ void k_init(DevTopo g, DevInst d, const int8_t *type0,
 }
 
 // ---------------------------------------------------------------------------
+// Per-instance inputs from the outage masks (the reference uploads one
+// particle matrix, runpf.h:611-616; here the masks and controls are uploaded
+// and every per-instance array is built on the device).  Thread (t, i) builds
+// bus i and branch i of instance t: the bus adds the in-service generators
+// of its list in ascending generator order (the last one sets the voltage
+// set-point, loadMatpowerFormat.h:126-128), a PV bus without one becomes PQ.
+// ---------------------------------------------------------------------------
+struct CaseArrays {
+  const double *Pd, *Qd, *Bs, *Vm, *tap;
+  const uint8_t *br_status;
+  const int *gb_ptr, *gb_gen;          // generators of each bus, ascending
+  const double *Pg, *Qg, *Qmax, *Qmin, *VG;
+  const uint8_t *gen_status;
+  int ng;
+};
+
+__global__ void k_expand(DevTopo g, DevInst d, CaseArrays c, const uint8_t *br_on,
+                         const uint8_t *gen_on, int8_t *type0, double *Qinj0,
+                         double *Pinj, uint8_t *on, double *tapT, double *BsT,
+                         double *vset, double *Qmx, double *Qmn) {
+  const int t = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.T) return;
+  if (i < g.n) {
+    const size_t o = (size_t)t * g.n + i;
+    const int bt = g.base_type[i];
+    double P = -c.Pd[i], Q = -c.Qd[i], qmx = 0.0, qmn = 0.0, vg = 0.0;
+    int ngon = 0;
+    for (int k = c.gb_ptr[i]; k < c.gb_ptr[i + 1]; k++) {
+      const int gk = c.gb_gen[k];
+      if (!c.gen_status[gk] || (gen_on && !gen_on[(size_t)t * c.ng + gk])) continue;
+      ngon++;
+      P += c.Pg[gk];
+      Q += c.Qg[gk];
+      qmx += c.Qmax[gk];
+      qmn += c.Qmin[gk];
+      vg = c.VG[gk];
+    }
+    type0[o] = (int8_t)((bt == 2 && ngon == 0) ? 1 : bt);
+    Pinj[o] = P;
+    Qinj0[o] = Q;
+    Qmx[o] = qmx;
+    Qmn[o] = qmn;
+    BsT[o] = c.Bs[i];
+    vset[o] = (bt == 2 && ngon > 0 && !isnan(vg)) ? vg : c.Vm[i];
+  }
+  if (i < g.m) {
+    const size_t o = (size_t)t * g.m + i;
+    on[o] = (c.br_status[i] && (!br_on || br_on[o])) ? 1 : 0;
+    tapT[o] = c.tap[i];
+  }
+}
+
+// controls of the particle vector (Particula.h:26-60), applied in their
+// order per instance (a later control of the same element wins); NaN keeps
+// the default; an OLTC control applies only to a branch with a tap
+__global__ void k_expand_ctl(DevTopo g, DevInst d, const double *tap_case,
+                             int nctl, const int2 *ctl, const double *ctl_val,
+                             double *tapT, double *BsT, double *vset) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.T) return;
+  for (int c = 0; c < nctl; c++) {
+    const double v = ctl_val[(size_t)t * nctl + c];
+    const int kind = ctl[c].x, id = ctl[c].y;
+    if (isnan(v)) continue;
+    if (kind == 1) vset[(size_t)t * g.n + id] = v;
+    else if (kind == 3) BsT[(size_t)t * g.n + id] = v;
+    else if (tap_case[id] != 0.0) tapT[(size_t)t * g.m + id] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // workgroup reductions (wave64 shuffles + LDS)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_max(double v) {
@@ -413,10 +484,13 @@ __device__ __forceinline__ B4<T> blk_mul(const B4<T> &x, const B4<T> &y) {
 // ---------------------------------------------------------------------------
 // schedules of the factorization and of the sweeps (device arrays from the
 // symbolic analysis)
+constexpr int TK_LEN_SHIFT = 19;       // products per schedule < 2^19
 struct FactorPlan {
   int nchunk;                          // chunks of the levels below the tail
   const int *ch_task, *ch_pair, *ch_max;
-  const int4 *tk_a, *tk_b;             // per output: {dst, p0, p1}, pivot range
+  // per output {dst | flags, first product, end product, pivot range}; the
+  // pivot range of an L output packed as first | length << TK_LEN_SHIFT
+  const int4 *tk;
   const int2 *pairs;                   // {L, U} positions (U < 0: init entry)
   int tn, tnent, tmax;                 // dense tail: pivots, entries, longest list
   const int *tdense, *tpos, *tpp, *tpairL, *tpairU;
@@ -440,6 +514,9 @@ struct SolvePlan {
   int sl_nfw, sl_nbw;
   const int4 *sl_chunk;                // {slot0, slot1, log2 largest group, 0}
   const int4 *sl_slot;                 // {position, column + 1 | log2 group, row or -1, pivot}
+  // the same for the transposed sweeps (adjoint)
+  int sl_nfw_t, sl_nbw_t;
+  const int4 *sl_chunk_t, *sl_slot_t;
 };
 
 // Lanes that share one item (a task's pair list, a row's entries): a power of
@@ -955,15 +1032,13 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       }
     };
     load_operands(0);
-    // the thread's first batch of outputs: descriptors
-    int4 ta[NB], tb[NB];
+    // the thread's first batch of outputs: descriptors (requested with the
+    // operands; one int4 per output)
+    int4 ta[NB];
 #pragma unroll
     for (int u = 0; u < NB; u++) {
       const int k = kf + u * stride;
-      if (wave_any(k, k1)) {
-        ta[u] = P.tk_a[min(k, klast)];
-        tb[u] = P.tk_b[min(k, klast)];
-      }
+      if (wave_any(k, k1)) ta[u] = P.tk[min(k, klast)];
     }
     __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     form_products(0);
@@ -999,7 +1074,10 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
         B4<T> e = lane_range_sum(buf, ta[u].y + lane - q0, ta[u].z - q0, gsz);
         group_sum4_first(e, gsz);
         if (ta[u].x & TASK_LCOL) {
-          B4<T> pv = lane_range_sum(buf, tb[u].y + lane - q0, tb[u].z - q0, gsz);
+          const unsigned pw = (unsigned)ta[u].w;
+          const int v0 = (int)(pw & ((1u << TK_LEN_SHIFT) - 1));
+          const int v1 = v0 + (int)(pw >> TK_LEN_SHIFT);
+          B4<T> pv = lane_range_sum(buf, v0 + lane - q0, v1 - q0, gsz);
           group_sum4_first(pv, gsz);
           if (lane == 0) {
             B4<T> inv;
@@ -1022,10 +1100,7 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
 #pragma unroll
       for (int u = 0; u < NB; u++) {
         const int k = kb + u * stride;
-        if (wave_any(k, k1)) {
-          ta[u] = P.tk_a[min(k, klast)];
-          tb[u] = P.tk_b[min(k, klast)];
-        }
+        if (wave_any(k, k1)) ta[u] = P.tk[min(k, klast)];
       }
       reduce_batch(kb);
     }
@@ -1774,17 +1849,21 @@ __device__ __forceinline__ SlotIdx<SP> slot_index(const int4 *slot, const int4 &
   return ix;
 }
 
-template <typename T, int SP>
+// TR: the transposed sweeps (U^T forward with the pivots, L^T backward),
+// whose slots hold mirror positions; the pivots then belong to the forward
+// chunks
+template <typename T, int SP, bool TR = false>
 __device__ __forceinline__ SlotVal<T, SP> slot_values(const B4<T> *A,
                                                       const int4 &m, bool bw,
                                                       const SlotIdx<SP> &ix) {
   SlotVal<T, SP> v;
+  const bool piv = TR ? !bw : bw;
 #pragma unroll
   for (int k = 0; k < SP; k++) {
     const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
     if (wave_any(q, m.y)) {
       v.a[k] = A[pinned(ix.s[k].x)];
-      if (bw) v.d[k] = A[pinned(ix.s[k].w)];
+      if (piv) v.d[k] = A[pinned(ix.s[k].w)];
     }
   }
   return v;
@@ -1793,7 +1872,7 @@ __device__ __forceinline__ SlotVal<T, SP> slot_values(const B4<T> *A,
 // chunk c (slots cur, blocks val, both received); before its reduction the
 // blocks of chunk c + 1 are requested from its slots (nxt) and the slots of
 // chunk c + 2 (c_after, clamped to the last chunk) into cur
-template <typename T, int SP>
+template <typename T, int SP, bool TR = false>
 __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
                                            const B4<T> *A, T *X, int c,
                                            bool bw, SlotIdx<SP> &cur,
@@ -1823,7 +1902,7 @@ __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
       lg[k] = y >> SLOT_LG_SHIFT;
     }
   }
-  nval = slot_values<T, SP>(A, mn, next_bw, nxt);
+  nval = slot_values<T, SP, TR>(A, mn, next_bw, nxt);
   cur = slot_index<SP>(slot, ma);
   T p0[SP], p1[SP];
 #pragma unroll
@@ -1834,8 +1913,13 @@ __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
       const T y0 = X[2 * cc], y1 = X[2 * cc + 1];
       const B4<T> &a = val.a[k];
       const bool ent = col[k] >= 0;
-      p0[k] = ent ? a.a * y0 + a.b * y1 : (T)0;
-      p1[k] = ent ? a.c * y0 + a.d * y1 : (T)0;
+      if (TR) {   // the block (col, row) used transposed
+        p0[k] = ent ? a.a * y0 + a.c * y1 : (T)0;
+        p1[k] = ent ? a.b * y0 + a.d * y1 : (T)0;
+      } else {
+        p0[k] = ent ? a.a * y0 + a.b * y1 : (T)0;
+        p1[k] = ent ? a.c * y0 + a.d * y1 : (T)0;
+      }
     }
   }
 #pragma unroll
@@ -1848,10 +1932,15 @@ __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
         // one) also applies the pivot inverse in the backward sweep
         const int i = lead[k] & (SLOT_NOT_LAST - 1);
         const T x0 = X[2 * i] - p0[k], x1 = X[2 * i + 1] - p1[k];
-        if (bw && !(lead[k] & SLOT_NOT_LAST)) {
+        if ((TR ? !bw : bw) && !(lead[k] & SLOT_NOT_LAST)) {
           const B4<T> &D = val.d[k];
-          X[2 * i] = D.a * x0 + D.b * x1;
-          X[2 * i + 1] = D.c * x0 + D.d * x1;
+          if (TR) {   // the pivot inverse transposed
+            X[2 * i] = D.a * x0 + D.c * x1;
+            X[2 * i + 1] = D.b * x0 + D.d * x1;
+          } else {
+            X[2 * i] = D.a * x0 + D.b * x1;
+            X[2 * i + 1] = D.c * x0 + D.d * x1;
+          }
         } else {
           X[2 * i] = x0;
           X[2 * i + 1] = x1;
@@ -1862,35 +1951,37 @@ __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
   __syncthreads();
 }
 
-template <typename T, int SP>
+template <typename T, int SP, bool TR = false>
 __device__ __forceinline__ void lu_sweeps_slot(const B4<T> *A, const B4<T> *Ti,
                                                T *X, T *rs, const int4 *swc,
                                                const SolvePlan &P) {
-  const int nc = P.sl_nfw + P.sl_nbw;
+  const int nfw = TR ? P.sl_nfw_t : P.sl_nfw;
+  const int nc = nfw + (TR ? P.sl_nbw_t : P.sl_nbw);
   const bool tail = P.tn > 0;
+  const int4 *slots = TR ? P.sl_slot_t : P.sl_slot;
   if (nc == 0) {
-    if (tail) tail_rows<T>(A, Ti, X, rs, P, false);
+    if (tail) tail_rows<T>(A, Ti, X, rs, P, TR);
     return;
   }
   const int cl = nc - 1;
   SlotIdx<SP> ia, ib;
   SlotVal<T, SP> va, vb;
-  ia = slot_index<SP>(P.sl_slot, swc[0]);
+  ia = slot_index<SP>(slots, swc[0]);
   __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
-  va = slot_values<T, SP>(A, swc[0], 0 >= P.sl_nfw, ia);
-  ib = slot_index<SP>(P.sl_slot, swc[min(1, cl)]);
+  va = slot_values<T, SP, TR>(A, swc[0], 0 >= nfw, ia);
+  ib = slot_index<SP>(slots, swc[min(1, cl)]);
   for (int c = 0; c < nc; c += 2) {
-    if (c == P.sl_nfw && tail) tail_rows<T>(A, Ti, X, rs, P, false);
-    slot_chunk<T, SP>(P.sl_slot, swc, A, X, c, c >= P.sl_nfw, ia, va,
-                      min(c + 1, cl), min(c + 1, cl) >= P.sl_nfw, ib, vb,
-                      min(c + 2, cl));
+    if (c == nfw && tail) tail_rows<T>(A, Ti, X, rs, P, TR);
+    slot_chunk<T, SP, TR>(slots, swc, A, X, c, c >= nfw, ia, va,
+                          min(c + 1, cl), min(c + 1, cl) >= nfw, ib, vb,
+                          min(c + 2, cl));
     if (c + 1 >= nc) break;
-    if (c + 1 == P.sl_nfw && tail) tail_rows<T>(A, Ti, X, rs, P, false);
-    slot_chunk<T, SP>(P.sl_slot, swc, A, X, c + 1, c + 1 >= P.sl_nfw, ib, vb,
-                      min(c + 2, cl), min(c + 2, cl) >= P.sl_nfw, ia, va,
-                      min(c + 3, cl));
+    if (c + 1 == nfw && tail) tail_rows<T>(A, Ti, X, rs, P, TR);
+    slot_chunk<T, SP, TR>(slots, swc, A, X, c + 1, c + 1 >= nfw, ib, vb,
+                          min(c + 2, cl), min(c + 2, cl) >= nfw, ia, va,
+                          min(c + 3, cl));
   }
-  if (P.sl_nbw == 0 && tail) tail_rows<T>(A, Ti, X, rs, P, false);
+  if (nc == nfw && tail) tail_rows<T>(A, Ti, X, rs, P, TR);
 }
 
 // ---------------------------------------------------------------------------
@@ -2293,6 +2384,31 @@ __global__ void k_adj_solve(DevTopo g, DevInst d, const B4<double> *LU,
 
 // gradients: df/dP_i = lambda_P(i), df/dQ_i = lambda_Q(i) (PQ buses), and
 // df/dx_b = explicit - lambda^T dF/dx_b through dYs/dx = -j Ys^2 (in service)
+// the adjoint solve J^T lambda = g in the slot form (X in LDS): the
+// transposed slots of the Newton sweeps, U^T forward with the pivots, the
+// tail by its transposed inverse, L^T backward; one workgroup per instance
+template <int SP>
+__global__ void __launch_bounds__(SLOT_CHUNK / SP)
+k_adj_solve_slot(DevTopo g, DevInst d, const B4<double> *LU, const B4<double> *TL,
+                 const double *G, double *lam, SolvePlan P) {
+  __shared__ double X[SOLVE_LDS_BYTES / sizeof(double)];
+  __shared__ double rs[2 * TAIL_MAX];
+  __shared__ int4 s_swc[MAX_CHUNKS];
+  const int t = blockIdx.x;
+  if (d.st[t].done) return;
+  const B4<double> *A = LU + (size_t)t * g.nnzLU;
+  const B4<double> *Ti = TL + (size_t)t * P.tn * P.tn;
+  const double *Gt = G + (size_t)t * 2 * g.nb;
+  for (int q = threadIdx.x; q < P.sl_nfw_t + P.sl_nbw_t; q += blockDim.x)
+    s_swc[q] = P.sl_chunk_t[q];
+  for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) X[k] = Gt[k];
+  __syncthreads();
+  lu_sweeps_slot<double, SP, true>(A, Ti, X, rs, s_swc, P);
+  __syncthreads();
+  double *lt = lam + (size_t)t * 2 * g.nb;
+  for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) lt[k] = X[k];
+}
+
 __global__ void k_adj_grad(DevTopo g, DevInst d, const double *lam,
                            const double *rate, double w, double *gP,
                            double *gQ, double *gx) {
@@ -2431,5 +2547,6 @@ template __global__ void k_lu_solve<double, 0>(DevTopo, DevInst, const B4<double
 template __global__ void k_lu_solve<double, 2>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
 template __global__ void k_lu_solve<double, 4>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
 template __global__ void k_lu_solve<double, 8>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
+template __global__ void k_adj_solve_slot<4>(DevTopo, DevInst, const B4<double> *, const B4<double> *, const double *, double *, SolvePlan);
 
 }  // namespace bpf

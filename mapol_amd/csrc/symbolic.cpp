@@ -2,6 +2,7 @@
 // level schedules of the batched factorization and triangular solves.
 #include <algorithm>
 #include <cstdlib>
+#include <functional>
 #include <map>
 #include <numeric>
 #include <string>
@@ -57,6 +58,221 @@ std::vector<int> min_degree_order(std::vector<std::vector<int>> adj) {
   return order;
 }
 
+// Nested dissection (George 1973): split the graph by a vertex separator
+// taken from a breadth-first level structure rooted at a pseudo-peripheral
+// vertex, order both sides recursively and the separator last; parts of at
+// most `leaf` vertices are ordered by minimum degree.  The separator is the
+// smallest level whose sides both hold at least a third of the vertices,
+// thinned by moving every separator vertex without a neighbour on one side
+// to the other side.
+std::vector<int> nested_dissection_order(const std::vector<std::vector<int>> &adj0,
+                                         int leaf) {
+  const int n = (int)adj0.size();
+  std::vector<std::vector<int>> adj(adj0);
+  for (auto &a : adj) {
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+  }
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<int> part(n, -1), dist(n, -1), loc(n, -1);
+  int next_part = 0;
+  // BFS over the vertices of part p from s; returns the visit order
+  auto bfs = [&](int s, int p, std::vector<int> &seen) {
+    seen.clear();
+    seen.push_back(s);
+    dist[s] = 0;
+    for (size_t h = 0; h < seen.size(); h++) {
+      int v = seen[h];
+      for (int u : adj[v])
+        if (part[u] == p && dist[u] < 0) { dist[u] = dist[v] + 1; seen.push_back(u); }
+    }
+  };
+  auto clear_dist = [&](const std::vector<int> &vs) { for (int v : vs) dist[v] = -1; };
+  std::vector<int> seen, seen2;
+  // work stack of vertex sets; the separators are emitted in reverse
+  std::function<void(std::vector<int>)> rec = [&](std::vector<int> vs) {
+    const int p = next_part++;
+    for (int v : vs) part[v] = p;
+    if ((int)vs.size() <= leaf) {
+      std::vector<std::vector<int>> sub(vs.size());
+      for (size_t a = 0; a < vs.size(); a++) loc[vs[a]] = (int)a;
+      for (size_t a = 0; a < vs.size(); a++)
+        for (int u : adj[vs[a]])
+          if (part[u] == p) sub[a].push_back(loc[u]);
+      for (int k : min_degree_order(sub)) order.push_back(vs[k]);
+      return;
+    }
+    // components
+    bfs(vs[0], p, seen);
+    if (seen.size() < vs.size()) {
+      std::vector<int> first = seen;
+      clear_dist(seen);
+      std::vector<int> rest;
+      for (int v : first) part[v] = -2;
+      for (int v : vs) if (part[v] == p) rest.push_back(v);
+      for (int v : first) part[v] = p;
+      rec(first);
+      rec(rest);
+      return;
+    }
+    // pseudo-peripheral root: repeat BFS from the farthest vertex of
+    // smallest degree while the eccentricity grows
+    int root = vs[0], ecc = dist[seen.back()];
+    for (int rounds = 0; rounds < 8; rounds++) {
+      int far = seen.back(), fd = dist[far];
+      for (int v : seen)
+        if (dist[v] == fd && adj[v].size() < adj[far].size()) far = v;
+      clear_dist(seen);
+      bfs(far, p, seen2);
+      int e2 = dist[seen2.back()];
+      if (e2 <= ecc) { clear_dist(seen2); bfs(root, p, seen); break; }
+      root = far;
+      ecc = e2;
+      std::swap(seen, seen2);
+    }
+    const int nl = ecc + 1;
+    std::vector<int> cnt(nl, 0);
+    for (int v : seen) cnt[dist[v]]++;
+    const int nv = (int)vs.size();
+    int best = -1;
+    long below = 0;
+    for (int l = 0; l < nl; l++) {
+      const long above = nv - below - cnt[l];
+      if (l > 0 && l < nl - 1 && 3 * below >= nv && 3 * above >= nv &&
+          (best < 0 || cnt[l] < cnt[best]))
+        best = l;
+      below += cnt[l];
+    }
+    if (best < 0) {   // no balanced level: the middle one
+      below = 0;
+      for (int l = 0; l < nl; l++) {
+        if (2 * (below + cnt[l]) >= nv && l > 0 && l < nl - 1) { best = l; break; }
+        below += cnt[l];
+      }
+    }
+    if (best < 0) {   // too small to split
+      clear_dist(seen);
+      std::vector<std::vector<int>> sub(vs.size());
+      for (size_t a = 0; a < vs.size(); a++) loc[vs[a]] = (int)a;
+      for (size_t a = 0; a < vs.size(); a++)
+        for (int u : adj[vs[a]])
+          if (part[u] == p) sub[a].push_back(loc[u]);
+      for (int k : min_degree_order(sub)) order.push_back(vs[k]);
+      return;
+    }
+    // side: 0 = A (levels < best), 1 = separator, 2 = B
+    std::vector<int> A, Sep, B;
+    for (int v : seen) (dist[v] < best ? A : dist[v] == best ? Sep : B).push_back(v);
+    clear_dist(seen);
+    std::map<int, int> sd;
+    for (int v : A) sd[v] = 0;
+    for (int v : Sep) sd[v] = 1;
+    for (int v : B) sd[v] = 2;
+    std::vector<int> keep;
+    for (int v : Sep) {
+      bool nA = false, nB = false;
+      for (int u : adj[v]) {
+        if (part[u] != p) continue;
+        int su = sd[u];
+        nA |= su == 0;
+        nB |= su == 2;
+      }
+      if (!nB) { sd[v] = 0; A.push_back(v); }
+      else if (!nA) { sd[v] = 2; B.push_back(v); }
+      else keep.push_back(v);
+    }
+    rec(A);
+    rec(B);
+    // the separator last, in minimum-degree order of its own graph
+    const int ps = next_part++;
+    for (int v : keep) part[v] = ps;
+    std::vector<std::vector<int>> sub(keep.size());
+    for (size_t a = 0; a < keep.size(); a++) loc[keep[a]] = (int)a;
+    for (size_t a = 0; a < keep.size(); a++)
+      for (int u : adj[keep[a]])
+        if (part[u] == ps) sub[a].push_back(loc[u]);
+    for (int k : min_degree_order(sub)) order.push_back(keep[k]);
+  };
+  std::vector<int> all(n);
+  for (int v = 0; v < n; v++) all[v] = v;
+  if (n) rec(all);
+  return order;
+}
+
+// Column structures of the filled graph of `adj` under `order`: st[k] =
+// positions > k of the nonzeros of column k of L (symmetric pattern), built
+// from the original edges and the children's structures (George & Liu).
+static std::vector<std::vector<int>> filled_structure(
+    const std::vector<std::vector<int>> &adj, const std::vector<int> &order) {
+  const int nb = (int)adj.size();
+  std::vector<int> newof(nb);
+  for (int k = 0; k < nb; k++) newof[order[k]] = k;
+  std::vector<std::vector<int>> st(nb), children(nb);
+  for (int a = 0; a < nb; a++)
+    for (int c : adj[a])
+      if (newof[a] < newof[c]) st[newof[a]].push_back(newof[c]);
+  std::vector<int> tmp;
+  for (int k = 0; k < nb; k++) {
+    auto &s = st[k];
+    std::sort(s.begin(), s.end());
+    s.erase(std::unique(s.begin(), s.end()), s.end());
+    for (int ch : children[k]) {
+      tmp.clear();
+      std::set_union(s.begin(), s.end(), st[ch].begin(), st[ch].end(),
+                     std::back_inserter(tmp));
+      s.clear();
+      for (int x : tmp)
+        if (x != k) s.push_back(x);
+    }
+    if (!s.empty()) children[s.front()].push_back(k);
+  }
+  return st;
+}
+
+// Minimum-height reordering of a perfect elimination order (Jess & Kees
+// 1982; Liu & Mirzaian 1989): the filled graph of `order` is chordal, and
+// eliminating, round by round, a maximal independent set of its simplicial
+// vertices (lowest position first) gives an equivalent order -- the same
+// filled graph, so the same fill -- whose elimination tree height is the
+// number of rounds, the least over all equivalent orders.  Returns the new
+// order in the vertex numbering of `adj`.
+std::vector<int> min_height_reorder(const std::vector<std::vector<int>> &adj,
+                                    const std::vector<int> &order) {
+  const int nb = (int)adj.size();
+  const std::vector<std::vector<int>> st = filled_structure(adj, order);
+  std::vector<std::vector<int>> nbr(nb);   // filled-graph neighbours, sorted
+  for (int k = 0; k < nb; k++)
+    for (int j : st[k]) { nbr[k].push_back(j); nbr[j].push_back(k); }
+  for (auto &a : nbr) std::sort(a.begin(), a.end());
+  std::vector<char> alive(nb, 1), taken(nb, 0), blocked(nb, 0);
+  std::vector<int> out, live, round;
+  out.reserve(nb);
+  int remaining = nb;
+  while (remaining > 0) {
+    round.clear();
+    for (int v = 0; v < nb; v++) {
+      if (!alive[v] || blocked[v]) continue;
+      live.clear();
+      for (int u : nbr[v]) if (alive[u]) live.push_back(u);
+      bool simplicial = true;
+      for (size_t a = 0; simplicial && a < live.size(); a++)
+        for (size_t b = a + 1; b < live.size(); b++)
+          if (!std::binary_search(nbr[live[a]].begin(), nbr[live[a]].end(), live[b])) {
+            simplicial = false;
+            break;
+          }
+      if (!simplicial) continue;
+      round.push_back(v);
+      for (int u : live) blocked[u] = 1;   // independent set
+    }
+    for (int v : round) { alive[v] = 0; out.push_back(order[v]); }
+    remaining -= (int)round.size();
+    std::fill(blocked.begin(), blocked.end(), 0);
+  }
+  return out;
+}
+
 static int find_pos(const std::vector<int> &rp, const std::vector<int> &ci,
                     int r, int c) {
   auto b = ci.begin() + rp[r], e = ci.begin() + rp[r + 1];
@@ -66,8 +282,22 @@ static int find_pos(const std::vector<int> &rp, const std::vector<int> &ci,
 }
 
 bool analyse(int n, int m, const int *from, const int *to, const int *type,
-             Symbolic &S, std::string *err) {
+             Symbolic &S, std::string *err, const AnalysisParams &prm) {
   S = Symbolic();
+  if (prm.tail_limit < 0 || prm.tail_limit > TAIL_MAX || prm.chunk_pairs < 0 ||
+      prm.chunk_pairs > CHUNK_PAIRS || prm.slot_chunk < 0 ||
+      prm.slot_chunk > SLOT_CHUNK ||
+      (prm.slot_chunk && prm.slot_chunk < 64) || prm.ordering < 0 ||
+      prm.ordering > 2) {
+    if (err) *err = "tile sizes out of range (tail <= " + std::to_string(TAIL_MAX) +
+                    " pivots, chunk <= " + std::to_string(CHUNK_PAIRS) +
+                    " products, 64 <= slots <= " + std::to_string(SLOT_CHUNK) + ")";
+    return false;
+  }
+  S.prm.tail_limit = prm.tail_limit ? prm.tail_limit : TAIL_MAX;
+  S.prm.chunk_pairs = prm.chunk_pairs ? prm.chunk_pairs : CHUNK_PAIRS;
+  S.prm.slot_chunk = prm.slot_chunk ? prm.slot_chunk : SLOT_CHUNK;
+  S.prm.ordering = prm.ordering;
   S.n = n;
   S.m = m;
   for (int b = 0; b < m; b++) {
@@ -150,7 +380,15 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     int a = loc[from[b]], c = loc[to[b]];
     if (a >= 0 && c >= 0 && a != c) { adj[a].push_back(c); adj[c].push_back(a); }
   }
-  std::vector<int> order = min_degree_order(adj);
+  // ordering: minimum degree (0), then optionally its minimum-height
+  // equivalent (1, same fill), or nested dissection (2)
+  std::vector<int> order;
+  if (S.prm.ordering == 2)
+    order = nested_dissection_order(adj, 64);
+  else if (S.prm.ordering == 1)
+    order = min_height_reorder(adj, min_degree_order(adj));
+  else
+    order = min_degree_order(adj);
   S.bus_of_blk.resize(nb);
   S.blk_of_bus.assign(n, -1);
   std::vector<int> newof(nb);
@@ -243,12 +481,8 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     std::vector<int> cnt(S.nlev + 1, 0);  // pivots at levels >= l
     for (int k = 0; k < nb; k++) cnt[lev[k]]++;
     for (int l = S.nlev - 1; l >= 0; l--) cnt[l] += cnt[l + 1];
-    // BPF_TAIL_LIMIT (1..TAIL_MAX) caps the tail below its LDS capacity
-    int limit = TAIL_MAX;
-    if (const char *ev = getenv("BPF_TAIL_LIMIT")) {
-      const int v = atoi(ev);
-      if (v >= 1 && v <= TAIL_MAX) limit = v;
-    }
+    // the tail limit (at most TAIL_MAX, the LDS capacity) is a tuning knob
+    const int limit = S.prm.tail_limit;
     int Ls = S.nlev;
     while (Ls > 0 && cnt[Ls - 1] <= limit) Ls--;
     // worth it only when it removes a few levels
@@ -322,7 +556,10 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
                      [&](const std::vector<Task> &x, const std::vector<Task> &y) {
                        return gpairs(x) > gpairs(y);
                      });
-    // pack groups into chunks (first fit in order of decreasing size)
+    // pack groups into chunks (first fit in order of decreasing size) of at
+    // most chunk_pairs products; a group above that (but within the LDS
+    // buffer of CHUNK_PAIRS) gets a chunk of its own
+    const size_t cap = (size_t)S.prm.chunk_pairs;
     std::vector<std::vector<int>> chunks;   // group indices
     std::vector<size_t> fill;
     for (int gi = 0; gi < (int)groups.size(); gi++) {
@@ -334,7 +571,7 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
       }
       int c = -1;
       for (int q = 0; q < (int)chunks.size(); q++)
-        if (fill[q] + np <= (size_t)CHUNK_PAIRS) { c = q; break; }
+        if (fill[q] + np <= cap) { c = q; break; }
       if (c < 0) { chunks.push_back({}); fill.push_back(0); c = (int)chunks.size() - 1; }
       chunks[c].push_back(gi);
       fill[c] += np;
@@ -609,16 +846,23 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   // segment's first lane subtracts its partial sum from x_i, and only the
   // last one applies the pivot inverse (backward).
   struct Piece { int row, p0, p1, last; };
+  // tr: the transposed sweeps of the adjoint (J^T = U^T L^T): U^T forward
+  // over the rows of L (every row: each applies its pivot inverse at its
+  // last segment), L^T backward over the rows of U (rows without entries
+  // have nothing to do); every position is the mirror of the entry's, the
+  // block (k, i) for (i, k)
   auto slots = [&](const std::vector<int> &ptr, const std::vector<int> &rows,
-                   int nl, bool fw) {
+                   int nl, bool fw, bool tr, std::vector<int> &chunk,
+                   std::vector<int> &slot) {
     int nch = 0;
+    const bool pivot = tr ? fw : !fw;
     for (int l = 0; l < nl; l++) {
       std::vector<std::vector<Piece>> pass;
       for (int r = ptr[l]; r < ptr[l + 1]; r++) {
         const int i = rows[r];
         const int p0 = fw ? S.lrow[i] : S.ldiag[i] + 1;
         const int p1 = fw ? S.ldiag[i] : S.lrow[i + 1];
-        if (fw && p1 == p0) continue;   // x = b: nothing to do
+        if (!pivot && p1 == p0) continue;   // x = b: nothing to do
         const int nseg = std::max(1, (p1 - p0 + 63) / 64);
         if ((int)pass.size() < nseg) pass.resize(nseg);
         for (int j = 0; j < nseg; j++)
@@ -631,41 +875,44 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
         });
         size_t r = 0;
         while (r < pieces.size()) {
-          const int s0 = (int)S.sl_slot.size() / 4;
+          const int s0 = (int)slot.size() / 4;
           int glg = -1;
           while (r < pieces.size()) {
             const Piece &pc = pieces[r];
             const int i = pc.row, len = pc.p1 - pc.p0;
             int lg = 0;
             while ((1 << lg) < len) lg++;
-            const int ns = (int)S.sl_slot.size() / 4;
-            if (ns > s0 && ns - s0 + (1 << lg) > SLOT_CHUNK) break;
+            const int ns = (int)slot.size() / 4;
+            if (ns > s0 && ns - s0 + (1 << lg) > S.prm.slot_chunk) break;
             const int first = len > 0 ? pc.p0 : std::max(S.ldiag[i], 0);
             const int lead = i | (pc.last ? 0 : SLOT_NOT_LAST);
             for (int k = 0; k < (1 << lg); k++) {
               const int p = pc.p0 + k;
               const bool ent = p < pc.p1;
-              const int pos = ent ? p : first;
-              S.sl_slot.insert(S.sl_slot.end(),
-                               {pos, (ent ? S.lcol[p] + 1 : 0) | (lg << SLOT_LG_SHIFT),
-                                k == 0 ? lead : -1,
-                                (k == 0 && !fw && pc.last) ? S.ldiag[i] : pos});
+              const int pos = tr ? S.mirror[ent ? p : first] : (ent ? p : first);
+              slot.insert(slot.end(),
+                          {pos, (ent ? S.lcol[p] + 1 : 0) | (lg << SLOT_LG_SHIFT),
+                           k == 0 ? lead : -1,
+                           (k == 0 && pivot && pc.last) ? S.ldiag[i] : pos});
             }
             glg = std::max(glg, lg);
             r++;
           }
-          const int s1 = (int)S.sl_slot.size() / 4;
+          const int s1 = (int)slot.size() / 4;
           if (s1 == s0) continue;
-          S.sl_chunk.insert(S.sl_chunk.end(), {s0, s1, glg, 0});
+          chunk.insert(chunk.end(), {s0, s1, glg, 0});
           nch++;
         }
       }
     }
     return nch;
   };
-  S.sl_nfw = slots(S.fw_ptr, S.fw_rows, S.nfw, true);
-  S.sl_nbw = slots(S.bw_ptr, S.bw_rows, S.nbw, false);
+  S.sl_nfw = slots(S.fw_ptr, S.fw_rows, S.nfw, true, false, S.sl_chunk, S.sl_slot);
+  S.sl_nbw = slots(S.bw_ptr, S.bw_rows, S.nbw, false, false, S.sl_chunk, S.sl_slot);
   S.sl_ok = S.sl_nfw + S.sl_nbw <= MAX_CHUNKS;
+  S.sl_nfw_t = slots(S.fw_ptr, S.fw_rows, S.nfw, true, true, S.sl_chunk_t, S.sl_slot_t);
+  S.sl_nbw_t = slots(S.bw_ptr, S.bw_rows, S.nbw, false, true, S.sl_chunk_t, S.sl_slot_t);
+  S.sl_ok_t = S.sl_nfw_t + S.sl_nbw_t <= MAX_CHUNKS;
   return true;
 }
 

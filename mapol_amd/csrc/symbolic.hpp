@@ -27,7 +27,18 @@ constexpr int TAIL_MAX = 64;         // dense tail: at most 64 blocks square
 // (with the chunk tables, which the tail no longer needs, in the same space)
 constexpr int TAIL_STAGE_MAX = 5760;
 
+// tile sizes of the schedules (0 = the largest the kernels hold): pivots
+// of the dense tail, products per factorization chunk, lane slots per
+// slot-form sweep chunk
+struct AnalysisParams {
+  int tail_limit = 0, chunk_pairs = 0, slot_chunk = 0;
+  // bus ordering: 0 minimum degree, 1 minimum degree reordered to the least
+  // elimination-tree height with the same fill, 2 nested dissection
+  int ordering = 0;
+};
+
 struct Symbolic {
+  AnalysisParams prm;   // the values used (0s replaced by the maxima)
   int n = 0, m = 0, nb = 0;
   // Ybus pattern over all buses (every branch, in service or not: outages
   // only zero values, as the reference keeps the structure, makeYbus.h:69-71)
@@ -115,14 +126,28 @@ struct Symbolic {
   // slot; | SLOT_NOT_LAST for an earlier segment) or -1, pivot position
   // (first slot of a row's last segment, backward) or the position}
   std::vector<int> sl_slot;
+  // the slot form of the transposed sweeps (adjoint, U^T forward with the
+  // pivots, L^T backward): mirror positions, chunks of their own
+  int sl_nfw_t = 0, sl_nbw_t = 0, sl_ok_t = 0;
+  std::vector<int> sl_chunk_t, sl_slot_t;
 };
 
 // bus_type: MATPOWER codes; buses of type PV or PQ are unknowns, every other
 // bus keeps its voltage (slack).  Returns false with a message on bad input.
 bool analyse(int n, int m, const int *from, const int *to,
-             const int *bus_type, Symbolic &S, std::string *err);
+             const int *bus_type, Symbolic &S, std::string *err,
+             const AnalysisParams &prm = AnalysisParams());
 
 // minimum-degree order of a symmetric graph given as adjacency lists
 std::vector<int> min_degree_order(std::vector<std::vector<int>> adj);
+
+// nested-dissection order (separators last; parts of at most `leaf`
+// vertices by minimum degree)
+// the same filled graph as `order`, with the least elimination-tree height
+std::vector<int> min_height_reorder(const std::vector<std::vector<int>> &adj,
+                                    const std::vector<int> &order);
+
+std::vector<int> nested_dissection_order(const std::vector<std::vector<int>> &adj,
+                                         int leaf);
 
 }  // namespace bpf

@@ -27,11 +27,14 @@ EXPORTS = [
     "bpf_default_options", "bpf_create", "bpf_set_instances", "bpf_solve",
     "bpf_get_results", "bpf_get_branch_flows", "bpf_get_branch_loading",
     "bpf_get_branch_loading_noalloc", "bpf_destroy",
-    "bpf_last_error", "bpf_get_info", "bpf_get_ybus", "bpf_stage_mismatch",
-    "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_adjoint",
+    "bpf_last_error", "bpf_get_info", "bpf_make_ybus", "bpf_get_ybus",
+    "bpf_stage_bmat", "bpf_stage_fd_halfstep", "bpf_stage_mismatch",
+    "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_stage_tail_inverse",
+    "bpf_adjoint",
     "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
+    "bpf_set_schedule",
     "bpf_analyse", "bpf_analyse_sweeps", "bpf_analyse_fill", "bpf_analyse_tail",
-    "bpf_analyse_slots",
+    "bpf_analyse_slots", "bpf_analyse_slots_transposed",
     "bpf_load_matpower",
     "bpf_case_error", "bpf_free_case",
 ]
@@ -53,7 +56,9 @@ class BpfOptions(C.Structure):
                 ("refine_steps", C.c_int32), ("maxit_nr", C.c_int32),
                 ("maxit_fd", C.c_int32), ("enforce_qlim", C.c_int32),
                 ("max_qlim_rounds", C.c_int32), ("use_graph", C.c_int32),
-                ("tol", C.c_double), ("qlim_tol", C.c_double)]
+                ("tol", C.c_double), ("qlim_tol", C.c_double),
+                ("tail_limit", C.c_int32), ("chunk_products", C.c_int32),
+                ("slot_chunk", C.c_int32), ("ordering", C.c_int32)]
 
 
 class BpfCtl(C.Structure):
@@ -75,7 +80,8 @@ class BpfInfo(C.Structure):
                 ("sweep_rows", C.c_int32), ("sweep_entries", C.c_int32),
                 ("fill_level0_chunks", C.c_int32), ("tail_stage", C.c_int32),
                 ("slot_fw_chunks", C.c_int32), ("slot_bw_chunks", C.c_int32),
-                ("slots", C.c_int32)]
+                ("slots", C.c_int32), ("graph_fallback", C.c_int32),
+                ("graph_captures", C.c_int32)]
 
 
 _LIB = None
@@ -111,20 +117,27 @@ def load_library(path: str = LIB_PATH):
     L.bpf_last_error.argtypes = [V]
     L.bpf_last_error.restype = C.c_char_p
     L.bpf_get_info.argtypes = [V, C.POINTER(BpfInfo)]
+    L.bpf_make_ybus.argtypes = [V]
     L.bpf_get_ybus.argtypes = [V, C.c_int32, _pi, _pi, _pd]
+    L.bpf_stage_bmat.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, _pi, _pi,
+                                 _pd, _pi]
+    L.bpf_stage_fd_halfstep.argtypes = [V, _pd, C.c_int32, _pd, _pd]
     L.bpf_stage_mismatch.argtypes = [V, _pd, _pd, _pd]
     L.bpf_stage_jacobian.argtypes = [V, _pd, C.c_int32, C.c_int32, _pi, _pi,
                                      _pd, _pi]
     L.bpf_stage_newton_step.argtypes = [V, _pd, _pd]
+    L.bpf_stage_tail_inverse.argtypes = [V, C.c_int32, _pi, _pd]
     L.bpf_adjoint.argtypes = [V, _pd, C.c_double, _pd, _pd, _pd, _pd]
     L.bpf_set_timing.argtypes = [V, C.c_int32]
     L.bpf_get_kernel_time.argtypes = [V, C.c_char_p, _pi, _pd]
     L.bpf_set_launch.argtypes = [V, C.c_int32, C.c_int32, C.c_int32]
+    L.bpf_set_schedule.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 21
     L.bpf_analyse_sweeps.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 3
     L.bpf_analyse_fill.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
     L.bpf_analyse_tail.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
     L.bpf_analyse_slots.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
+    L.bpf_analyse_slots_transposed.argtypes = [C.POINTER(BpfCase)] + [_pi] * 3
     L.bpf_load_matpower.argtypes = [C.c_char_p, C.c_int32, C.POINTER(BpfCase),
                                     C.POINTER(_pd), C.POINTER(C.c_void_p)]
     L.bpf_case_error.argtypes = [C.c_void_p]
@@ -283,6 +296,19 @@ def analyse(case):
     if st != 0:
         raise EngineError(f"bpf_analyse_slots failed ({st})")
     out.update(sl)
+    cnt = np.zeros(3, np.int32)
+    st = L.bpf_analyse_slots_transposed(C.byref(s), _ptr(cnt, _pi), None, None)
+    if st != 0:
+        raise EngineError(f"bpf_analyse_slots_transposed failed ({st})")
+    slt = dict(sl_chunk_t=np.zeros((cnt[0] + cnt[1], 4), np.int32),
+               sl_slot_t=np.zeros((cnt[2], 4), np.int32))
+    st = L.bpf_analyse_slots_transposed(C.byref(s), _ptr(cnt, _pi),
+                                        _ptr(slt["sl_chunk_t"], _pi),
+                                        _ptr(slt["sl_slot_t"], _pi))
+    if st != 0:
+        raise EngineError(f"bpf_analyse_slots_transposed failed ({st})")
+    out.update(slt)
+    out["slot_t_fw_chunks"] = int(cnt[0])
     return out
 
 
@@ -293,7 +319,9 @@ class Engine:
                  refine_steps: int = 1, tol: float = 1e-8, maxit_nr: int = 10,
                  maxit_fd: int = 30, enforce_qlim: bool = False,
                  max_qlim_rounds: int = 10, qlim_tol: float = 1e-6,
-                 use_graph: bool = True):
+                 use_graph: bool = True, tail_limit: int = 0,
+                 chunk_products: int = 0, slot_chunk: int = 0,
+                 ordering: int = 0):
         self.L = load_library()
         self.case = case
         self._keep = []
@@ -322,6 +350,8 @@ class Engine:
         o.maxit_nr, o.maxit_fd, o.tol = maxit_nr, maxit_fd, tol
         o.enforce_qlim, o.max_qlim_rounds = int(enforce_qlim), max_qlim_rounds
         o.qlim_tol, o.use_graph = qlim_tol, int(use_graph)
+        o.tail_limit, o.chunk_products = int(tail_limit), int(chunk_products)
+        o.slot_chunk, o.ordering = int(slot_chunk), int(ordering)
         self.options = o
         h = C.c_void_p()
         st = self.L.bpf_create(C.byref(s), C.byref(o), C.byref(h))
@@ -456,6 +486,36 @@ class Engine:
                     "bpf_get_ybus")
         return sp.csr_matrix((v[0::2] + 1j * v[1::2], ci, rp), shape=(n, n))
 
+    def make_ybus(self):
+        self._check(self.L.bpf_make_ybus(self.h), "bpf_make_ybus")
+
+    def stage_bmat(self, t, which):
+        """B' (which 1, pv then pq) or B'' (which 2, pq) of instance t"""
+        import scipy.sparse as sp
+        cap = self.info()["nnz_ybus"]
+        r = np.zeros(cap, np.int32)
+        c = np.zeros(cap, np.int32)
+        v = np.zeros(cap)
+        nnz = C.c_int32()
+        self._check(self.L.bpf_stage_bmat(self.h, int(t), int(which), cap,
+                                          _ptr(r, _pi), _ptr(c, _pi), _ptr(v, _pd),
+                                          C.byref(nnz)), "bpf_stage_bmat")
+        k = nnz.value
+        nr = int(max(r[:k].max(), c[:k].max())) + 1 if k else 0
+        return sp.coo_matrix((v[:k], (r[:k], c[:k])), shape=(nr, nr)).tocsr()
+
+    def stage_fd_halfstep(self, V, which):
+        """one P (which 1) or Q (which 2) half-step of every instance from V;
+        returns (new V, inf-norm of the |V|-normalised mismatch at V)"""
+        T, n = self.T, self.case.nbus
+        Vd = self._vin(V)
+        out = np.zeros((T, n, 2))
+        nrm = np.zeros(T)
+        self._check(self.L.bpf_stage_fd_halfstep(self.h, _ptr(Vd, _pd), int(which),
+                                                 _ptr(out, _pd), _ptr(nrm, _pd)),
+                    "bpf_stage_fd_halfstep")
+        return out[..., 0] + 1j * out[..., 1], nrm
+
     @staticmethod
     def _vin(V):
         V = np.asarray(V)
@@ -496,6 +556,18 @@ class Engine:
                     "bpf_stage_newton_step")
         return dx
 
+    def stage_tail_inverse(self, t):
+        """(buses of the dense tail's pivots, inverse of its Schur complement
+        as a dense (2 tail_n)^2 matrix) of instance t from the last
+        factorization"""
+        s = self.info()["tail_n"]
+        bus = np.zeros(s, np.int32)
+        Si = np.zeros((2 * s, 2 * s))
+        self._check(self.L.bpf_stage_tail_inverse(self.h, int(t), _ptr(bus, _pi),
+                                                  _ptr(Si, _pd)),
+                    "bpf_stage_tail_inverse")
+        return bus, Si
+
     def adjoint(self, rate=None, w=0.0):
         T, n, m = self.T, self.case.nbus, self.case.nbranch
         gP = np.zeros((T, n))
@@ -519,6 +591,14 @@ class Engine:
                                                C.byref(n), C.byref(ms)),
                     "bpf_get_kernel_time")
         return n.value, ms.value
+
+    def set_schedule(self, newton_per_graph=0, tail_inverse=0, sweep_form=0,
+                     parts=0):
+        """run-time schedule choices (``bpf_set_schedule``; 0 keeps one)"""
+        self._check(self.L.bpf_set_schedule(self.h, int(newton_per_graph),
+                                            int(tail_inverse), int(sweep_form),
+                                            int(parts)),
+                    "bpf_set_schedule")
 
     def set_launch(self, lu=0, solve=0, mismatch=0):
         self._check(self.L.bpf_set_launch(self.h, lu, solve, mismatch),

@@ -86,6 +86,25 @@ def parse(text: str, name: str = "", honour_gen_status: bool = False) -> Case:
         rate=br[:, 5] / base)
 
 
+def write(path: str, baseMVA: float, bus, gen, branch) -> None:
+    """MATPOWER version-2 text of the three tables (the inverse of the
+    reader: every value written in full precision, Inf as ``Inf``)."""
+    def fmt(v):
+        v = float(v)
+        if np.isinf(v):
+            return "Inf" if v > 0 else "-Inf"
+        return repr(v) if v != int(v) or abs(v) >= 1e15 else str(int(v))
+
+    lines = ["function mpc = case", "mpc.version = '2';",
+             f"mpc.baseMVA = {float(baseMVA)!r};"]
+    for name, tab in (("bus", bus), ("gen", gen), ("branch", branch)):
+        lines.append(f"mpc.{name} = [")
+        lines += ["\t" + "\t".join(fmt(v) for v in row) + ";" for row in tab]
+        lines.append("];")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
 def load(path: str, honour_gen_status: bool = False) -> Case:
     with open(path) as fh:
         text = fh.read()

@@ -111,3 +111,17 @@ def test_cpp_mirror_reads_matpower_case(tmp_path):
     r = oracle.Oracle(matpower.load(path)).solve()
     assert r["converged"]
     assert losses == pytest.approx([r["loss"]] * 2, rel=1e-9)
+
+
+@pytest.mark.skipif(not os.path.exists(engine.LIB_PATH),
+                    reason="libbatchpf.so not built")
+def test_tile_options_are_validated():
+    """bpf_create refuses tile sizes the kernels cannot hold before it
+    touches a device"""
+    from mapol_amd import synth
+    from mapol_amd.engine import Engine
+    case, _ = synth.workload("pf118x200", T=2)
+    for bad in (dict(tail_limit=65), dict(chunk_products=8192), dict(slot_chunk=32),
+                dict(ordering=3)):
+        with pytest.raises(Exception, match="out of range|tile"):
+            Engine(case, **bad)

@@ -169,3 +169,22 @@ def test_autograd_wrapper_matches_adjoint():
     an = float((Pd.grad.numpy() * dP).sum() + (Qd.grad.numpy() * dQ).sum()
                + (X.grad.numpy() * dX).sum())
     assert abs(an - fd) <= 1e-6 * max(abs(fd), 1e-3)
+
+
+@pytest.mark.parametrize("name,T", [("pf1354x200", 8), ("pf9241x200", 4)])
+def test_adjoint_slot_form_matches_level_rows(name, T):
+    """the transposed slot-form sweeps of the adjoint against its level-row
+    sweeps (another summation order of the same J^T lambda = g)"""
+    case, cont = synth.workload(name, T=T)
+    rate = _rates(case)
+    e = Engine(case, tol=1e-12, maxit_nr=20)
+    e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
+    e.solve()
+    a_slot = e.adjoint(rate=rate, w=W)
+    e.set_schedule(sweep_form=3)
+    a_rows = e.adjoint(rate=rate, w=W)
+    e.close()
+    for k in ("grad_P", "grad_Q", "grad_x"):
+        ref = a_rows[k]
+        assert np.abs(a_slot[k] - ref).max() <= 1e-9 * np.abs(ref).max(), k
+    np.testing.assert_array_equal(a_slot["objective"], a_rows["objective"])

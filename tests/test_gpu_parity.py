@@ -293,3 +293,36 @@ def test_graph_replay_matches_direct_launches():
     np.testing.assert_array_equal(a["iterations"], b["iterations"])
     np.testing.assert_array_equal(a["verdict"], b["verdict"])
     np.testing.assert_array_equal(a["V"], b["V"])
+
+
+@pytest.mark.parametrize("tiles", [
+    dict(ordering=1), dict(tail_limit=40), dict(tail_limit=47),
+    dict(chunk_products=2048), dict(slot_chunk=1024),
+    dict(ordering=1, tail_limit=48, chunk_products=3072, slot_chunk=1024)])
+def test_tile_options_match_oracle(tiles):
+    """every schedule the autotuner may choose (ordering, dense-tail size,
+    chunk sizes, graph length, tail inverse, concurrent parts of the batch)
+    gives the oracle's iterations, verdicts and voltages"""
+    name, T = "pf9241x200", 6
+    print(f"[tiles] {tiles}", flush=True)
+    case, cont = synth.workload(name)
+    e = Engine(case, precision=BPF_MIXED, enforce_qlim=True, **tiles)
+    e.set_instances(br_on=cont.br_on[:T], gen_on=cont.gen_on[:T])
+    for sched in (dict(), dict(newton_per_graph=2, tail_inverse=2, parts=2),
+                  dict(newton_per_graph=4, tail_inverse=1, parts=4)):
+        e.set_schedule(**sched)
+        e.solve()
+        r = e.results()
+        orc = oracle.Oracle(case)
+        for t in range(T):
+            ro = orc.solve(oracle.Oracle.opts(enforce_qlim=True),
+                           br_on=cont.br_on[t], gen_on=cont.gen_on[t])
+            assert r["verdict"][t] == ro["verdict"], (t, sched)
+            assert r["iterations"][t] == ro["iterations"], (t, sched)
+            if ro["converged"]:
+                assert np.abs(r["V"][t] - ro["V"]).max() <= DV_TOL
+    info = e.info()
+    if "tail_limit" in tiles:
+        assert info["tail_n"] <= tiles["tail_limit"]
+    e.close()
+

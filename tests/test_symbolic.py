@@ -240,6 +240,51 @@ def _sweeps_slot(a, A, Sinv, b):
     return x.ravel()
 
 
+def _sweeps_slot_t(a, A, Sinv, g):
+    """the adjoint's transposed slot-form sweeps: the transposed slots hold
+    mirror positions (blocks used transposed); the forward chunks (U^T)
+    apply the transposed pivot inverse at a row's last segment; the tail
+    through S^-T (its rows' non-tail entries through the mirror map)"""
+    x = g.reshape(-1, 2).copy()
+    nfw = a["slot_t_fw_chunks"]
+    ch, sl = a["sl_chunk_t"], a["sl_slot_t"]
+    mir = _mirror(a)
+    LG = 27
+
+    def tail():
+        s = a["info"]["tail_n"]
+        piv = a["tail_piv"]
+        rs = np.zeros((s, 2))
+        for k in range(s):
+            rs[k] = x[piv[k]]
+            for q in range(a["ts_ptr"][k], a["ts_ptr"][k + 1]):
+                rs[k] -= A[mir[a["ts_pos"][q]]].T @ x[a["ts_col"][q]]
+        for k in range(s):
+            x[piv[k]] = sum(Sinv[j, k].T @ rs[j] for j in range(s))
+
+    for c in range(len(ch)):
+        if c == nfw:
+            tail()
+        s0, s1 = ch[c, 0], ch[c, 1]
+        colp = sl[s0:s1, 1] & ((1 << LG) - 1)
+        prod = np.array([A[p].T @ x[max(cp - 1, 0)] if cp > 0 else np.zeros(2)
+                         for p, cp in zip(sl[s0:s1, 0], colp)])
+        q = 0
+        while q < s1 - s0:
+            pos, w, lead, piv = sl[s0 + q]
+            gl = w >> LG
+            row, last = lead & (NOT_LAST - 1), not lead & NOT_LAST
+            v = prod[q:q + (1 << gl)]
+            while len(v) > 1:
+                v = v[0::2] + v[1::2]
+            xi = x[row] - v[0]
+            x[row] = A[piv].T @ xi if c < nfw and last else xi
+            q += 1 << gl
+    if nfw == len(ch):
+        tail()
+    return x.ravel()
+
+
 def _system(name, seed=0, case=None):
     if case is None:
         case, cont = synth.workload(name)
@@ -355,6 +400,11 @@ def test_transposed_replay_solves_the_adjoint_system(name):
     lam = _sweeps(a, LU, Sinv, g, trans=True)
     ref = np.linalg.solve(M.T, g)
     assert np.abs(lam - ref).max() <= 1e-10 * max(1, np.abs(ref).max())
+    # and its slot form (k_adj_solve_slot), whose forward chunks also hold
+    # the rows without L entries (they still apply their pivots)
+    assert a["sl_chunk_t"].shape[0] > 0
+    lam_s = _sweeps_slot_t(a, LU, Sinv, g)
+    assert np.abs(lam_s - ref).max() <= 1e-10 * max(1, np.abs(ref).max())
 
 
 def test_tail_inverse_is_the_schur_complement_inverse():
