@@ -361,6 +361,20 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(g.br_from, e->br_from); UP(g.br_to, e->br_to);
   UP(g.bbr_ptr, S.bbr_ptr); UP(g.bbr, S.bbr);
   UP(g.R, e->R); UP(g.X, e->X); UP(g.B, e->B); UP(g.shift, e->shift);
+  {
+    // instance-independent branch quotients, computed on the device
+    void *p1, *p2;
+    if ((s = dalloc(e, e->owned, sizeof(cd) * std::max(e->m, 1), &p1))) return s;
+    if ((s = dalloc(e, e->owned, sizeof(cd) * std::max(e->m, 1), &p2))) return s;
+    if (e->m) {
+      hipLaunchKernelGGL(k_branch_pre, dim3(div_up(e->m, 256)), dim3(256), 0,
+                         e->stream, g, (cd *)p1, (cd *)p2);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(e->stream));
+    }
+    g.ys = (const cd *)p1;
+    g.ejs = (const cd *)p2;
+  }
   UP(g.Gs, e->Gs); UP(g.Qd, e->Qd); UP(g.Va0, e->Va);
   UP(g.base_type, e->bus_type);
   {
@@ -416,10 +430,7 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(F.tdense, S.tl_dense); UP(F.tpos, S.tl_pos); UP(F.tpp, S.tl_pp);
   UP(F.tpairL, S.tl_pairL); UP(F.tpairU, S.tl_pairU);
   F.tstage_n = (int)S.tl_stage.size();
-  {
-    const char *tv = getenv("BPF_TAIL_VALU");
-    F.tail_mfma = !(tv && tv[0] == '1');
-  }
+  F.tail_mfma = 1;   // bpf_set_schedule chooses
   UP(F.tstage, nonempty(S.tl_stage, 1)); UP(F.tpairS, nonempty(S.tl_pairS, 1));
   SolvePlan &SP = e->splan;
   SP.nfw = S.nfw;
@@ -455,9 +466,8 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
     SP.sl_nfw_t = S.sl_nfw_t;
     SP.sl_nbw_t = S.sl_nbw_t;
   }
-  const char *swr = getenv("BPF_SWEEP_ROWS"), *swe = getenv("BPF_SWEEP_ENTRY");
-  e->sweep_form = (swr && swr[0] == '1') ? 2 : (swe && swe[0] == '1') ? 1 : 0;
-  e->tail_form = F.tail_mfma ? BPF_TAIL_MFMA : BPF_TAIL_VALU;
+  e->sweep_form = 0;
+  e->tail_form = BPF_TAIL_MFMA;
 #undef UP
   return BPF_OK;
 }

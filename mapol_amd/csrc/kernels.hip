@@ -41,14 +41,26 @@ __device__ __forceinline__ cd cexpj(double th) {
   return mk(c, s);
 }
 
+// the instance-independent parts of every branch's pi-model: its series
+// admittance 1 / (R + jX) and its phase-shift factor e^(j shift), computed
+// once per case with the operations branch_entry would use per instance
+__global__ void k_branch_pre(DevTopo g, cd *ys, cd *ejs) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= g.m) return;
+  ys[b] = cdiv(mk(1.0, 0.0), mk(g.R[b], g.X[b]));
+  ejs[b] = cexpj(M_PI / 180.0 * g.shift[b]);
+}
+
 // pi-model entry of branch b for instance arrays (makeYbus.h:63-80)
 __device__ cd branch_entry(const DevTopo &g, const DevInst &d, int t, int b,
                            int kind) {
-  double stat = d.br_on[(size_t)t * g.m + b] ? 1.0 : 0.0;
-  cd Ys = cdiv(mk(stat, 0.0), mk(g.R[b], g.X[b]));
+  const bool on = d.br_on[(size_t)t * g.m + b] != 0;
+  const double stat = on ? 1.0 : 0.0;
+  // stat / (R + jX): the precomputed quotient for an in-service branch
+  const cd Ys = on ? g.ys[b] : cdiv(mk(0.0, 0.0), mk(g.R[b], g.X[b]));
   double tp = d.tap[(size_t)t * g.m + b];
   cd tap = (tp != 0.0) ? mk(tp, 0.0) : mk(1.0, 0.0);
-  tap = cmul(tap, cexpj(M_PI / 180.0 * g.shift[b]));
+  tap = cmul(tap, g.ejs[b]);
   switch (kind) {
     case KFF: {
       cd Bc = cmul(mk(stat, 0.0), mk(0.0, g.B[b]));
@@ -327,9 +339,9 @@ __device__ void mis_bus(const DevTopo &g, const DevInst &d, int t, int i,
   cd mis = csub(Sb, mk(d.Pinj[o], d.Qinj[o]));
   if (fd_norm) mis = cdiv(mis, mk(cabs_(V[i]), 0.0));
   bool pq = d.type[o] == 1;
-  double *F = d.F + (size_t)t * 2 * g.nb;
-  F[2 * k] = mis.x;
-  F[2 * k + 1] = pq ? mis.y : 0.0;
+  // the block's (P, Q) pair in one 16-byte store
+  double2 *F2 = (double2 *)(d.F + (size_t)t * 2 * g.nb);
+  F2[k] = make_double2(mis.x, pq ? mis.y : 0.0);
   lmax = fmax(lmax, fabs(mis.x));
   lnan |= isnan(mis.x);
   if (pq) { lmax = fmax(lmax, fabs(mis.y)); lnan |= isnan(mis.y); }
@@ -2102,8 +2114,10 @@ __global__ void k_residual(DevTopo g, DevInst d) {
   if (d.st[t].done) return;
   const int k = g.blk_of_bus[i];
   if (k < 0) return;
-  const double *F = d.F + (size_t)t * 2 * g.nb;
-  const double *dx = d.dx + (size_t)t * 2 * g.nb;
+  // F, dx and the residual as (P, Q) / (angle, magnitude) pairs: one
+  // 16-byte access per block
+  const double2 *F = (const double2 *)(d.F + (size_t)t * 2 * g.nb);
+  const double2 *dx = (const double2 *)(d.dx + (size_t)t * 2 * g.nb);
   const cd *Y = d.Y + (size_t)t * g.nnzY;
   const cd *V = d.V + (size_t)t * g.n;
   cd acc = mk(0.0, 0.0);
@@ -2115,7 +2129,7 @@ __global__ void k_residual(DevTopo g, DevInst d) {
   for (int p0 = pa; p0 < pe; p0 += RU) {
     int col[RU], bj[RU];
     cd y[RU], vj[RU];
-    double d0[RU], d1[RU];
+    double2 dj[RU];
 #pragma unroll
     for (int u = 0; u < RU; u++) {
       const int p = min(p0 + u, pe - 1);
@@ -2127,13 +2141,12 @@ __global__ void k_residual(DevTopo g, DevInst d) {
     for (int u = 0; u < RU; u++) {
       const int b = max(bj[u], 0);
       vj[u] = V[col[u]];
-      d0[u] = dx[2 * b];
-      d1[u] = dx[2 * b + 1];
+      dj[u] = dx[b];
     }
 #pragma unroll
     for (int u = 0; u < RU; u++) {
-      const double m = d1[u] / cabs_(vj[u]);
-      const cd s = cadd(acc, cmul(y[u], cmul(vj[u], mk(m, d0[u]))));
+      const double m = dj[u].y / cabs_(vj[u]);
+      const cd s = cadd(acc, cmul(y[u], cmul(vj[u], mk(m, dj[u].x))));
       const bool use = p0 + u < pe && bj[u] >= 0;
       acc.x = use ? s.x : acc.x;
       acc.y = use ? s.y : acc.y;
@@ -2141,11 +2154,11 @@ __global__ void k_residual(DevTopo g, DevInst d) {
   }
   const size_t o = (size_t)t * g.n + i;
   const cd vi = V[i];
-  const cd dvi = cmul(vi, mk(dx[2 * k + 1] / cabs_(vi), dx[2 * k]));
+  const double2 dk = dx[k], fk = F[k];
+  const cd dvi = cmul(vi, mk(dk.y / cabs_(vi), dk.x));
   const cd dS = cadd(cmul(dvi, cconj(d.Ib[o])), cmul(vi, cconj(acc)));
-  double *R = d.rhs + (size_t)t * 2 * g.nb;
-  R[2 * k] = F[2 * k] - dS.x;
-  R[2 * k + 1] = d.type[o] == 1 ? F[2 * k + 1] - dS.y : -dx[2 * k + 1];
+  double2 *R = (double2 *)(d.rhs + (size_t)t * 2 * g.nb);
+  R[k] = make_double2(fk.x - dS.x, d.type[o] == 1 ? fk.y - dS.y : -dk.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -2160,11 +2173,11 @@ __global__ void k_update(DevTopo g, DevInst d) {
   const int k = g.blk_of_bus[i];
   if (k < 0) return;
   size_t o = (size_t)t * g.n + i;
-  const double *dx = d.dx + (size_t)t * 2 * g.nb;
+  const double2 dk = ((const double2 *)(d.dx + (size_t)t * 2 * g.nb))[k];
   cd v = d.V[o];
   double mag = cabs_(v);
-  if (d.type[o] == 1) mag -= dx[2 * k + 1];
-  cd e = cexpj(atan(v.y / v.x) - dx[2 * k]);
+  if (d.type[o] == 1) mag -= dk.y;
+  cd e = cexpj(atan(v.y / v.x) - dk.x);
   d.V[o] = cmul(mk(mag, 0.0), e);
 }
 

@@ -38,6 +38,7 @@ struct DevTopo {
   const int *lrow, *lcol, *ldiag, *mirror;
   const int *br_from, *br_to, *bbr_ptr, *bbr;
   const double *R, *X, *B, *shift, *Gs, *Qd, *Va0;
+  const cd *ys, *ejs;      // per branch: 1 / (R + jX), e^(j shift) (k_branch_pre)
   const int *base_type;
   // Jacobian blocks in L+U position order: position, Ybus position
   int nJb;

@@ -218,13 +218,13 @@ def test_instances_can_be_set_again_with_another_count():
 
 @pytest.mark.parametrize("form,threads", [("slot", 1024), ("slot", 512), ("slot", 256),
                                           ("entry", 512), ("rows", 512)])
-def test_sweep_forms_match_oracle(monkeypatch, form, threads):
+def test_sweep_forms_match_oracle(form, threads):
     """Every form of the triangular sweeps (slot form with 1024, 512 and 256
-    lanes, entry-parallel, level rows) solves pf9241x200's first contingencies
-    with the oracle's iterations and verdicts, FP64 and mixed precision."""
+    lanes, entry-parallel, level rows; bpf_set_schedule) solves pf9241x200's
+    first contingencies with the oracle's iterations and verdicts, FP64 and
+    mixed precision."""
     print(f"[sweeps] form={form} threads={threads}", flush=True)
-    monkeypatch.setenv("BPF_SWEEP_ENTRY", "1" if form == "entry" else "0")
-    monkeypatch.setenv("BPF_SWEEP_ROWS", "1" if form == "rows" else "0")
+    sweep = {"slot": 1, "entry": 2, "rows": 3}[form]
     case, cont = synth.workload("pf9241x200")
     T = 4
     orc = oracle.Oracle(case)
@@ -233,6 +233,7 @@ def test_sweep_forms_match_oracle(monkeypatch, form, threads):
     for precision in (BPF_FP64, BPF_MIXED):
         e = Engine(case, precision=precision, enforce_qlim=True)
         e.set_launch(solve=threads)
+        e.set_schedule(sweep_form=sweep)
         e.set_instances(br_on=cont.br_on[:T], gen_on=cont.gen_on[:T])
         e.solve()
         r = e.results()
@@ -246,15 +247,13 @@ def test_sweep_forms_match_oracle(monkeypatch, form, threads):
 
 @pytest.mark.parametrize("lu,tail", [(1024, "mfma"), (1024, "valu"), (512, "valu"),
                                      (256, "valu")])
-def test_factorization_workgroup_sizes_match_oracle(lu, tail, monkeypatch):
+def test_factorization_workgroup_sizes_match_oracle(lu, tail):
     """Every workgroup size and dense-tail inverse of the factorization: with
     1024 lanes the 61-pivot tail is inverted on matrix cores (FP32, one
-    32x32 tile per wave, pivot quads) or, with BPF_TAIL_VALU=1, by pivot
+    32x32 tile per wave, pivot quads) or, with tail_inverse 2, by pivot
     pairs in 2x2 tiles of blocks; FP64 takes one pivot per step; 512 and 256
     lanes keep several entries per thread.  Same iterations, verdicts and
     voltages as the oracle."""
-    if tail == "valu":
-        monkeypatch.setenv("BPF_TAIL_VALU", "1")
     print(f"[factor] lu threads={lu} tail={tail}", flush=True)
     case, cont = synth.workload("pf9241x200")
     T = 4
@@ -264,6 +263,7 @@ def test_factorization_workgroup_sizes_match_oracle(lu, tail, monkeypatch):
     for precision in (BPF_FP64, BPF_MIXED):
         e = Engine(case, precision=precision, enforce_qlim=True)
         e.set_launch(lu=lu)
+        e.set_schedule(tail_inverse=2 if tail == "valu" else 1)
         e.set_instances(br_on=cont.br_on[:T], gen_on=cont.gen_on[:T])
         e.solve()
         r = e.results()
