@@ -190,14 +190,25 @@ def cpu_baseline(case, cont, qlim, threads, runs=5, warmup=2, single_s=2.5):
         if k >= warmup:
             times.append(dt)
     multi = cont.T / float(np.median(times))
-    # one core, contingencies in sequence: every k-th contingency, as many
-    # as one core solves in about single_s seconds (timed once first)
+    # one core, contingencies in sequence: a sample stratified by the Newton
+    # iteration counts of the all-thread run (the middle contingency of each
+    # of ns equal strata in order of iterations), as many as one core solves
+    # in about single_s seconds; its rate is scaled to the whole set by
+    # iterations, since a few hard contingencies (non-convergent ones run to
+    # the iteration limit) carry a share of the time no small sample matches
+    its = np.array([out[t]["iterations"] for t in range(cont.T)], float)
+    rank = np.argsort(its, kind="stable")
+
+    def stratified(k):
+        return rank[((np.arange(k) + 0.5) * cont.T / k).astype(int)]
+
     t0 = time.perf_counter()
-    for t in range(4):
-        one(t)
-    per = (time.perf_counter() - t0) / 4
+    probe = stratified(4)
+    for t in probe:
+        one(int(t))
+    per = (time.perf_counter() - t0) / len(probe)
     ns = int(max(4, min(cont.T, round(single_s / max(per, 1e-6)))))
-    sample = np.linspace(0, cont.T - 1, ns).round().astype(int)
+    sample = stratified(ns)
     st = []
     for k in range(warmup + runs):
         t0 = time.perf_counter()
@@ -205,9 +216,12 @@ def cpu_baseline(case, cont, qlim, threads, runs=5, warmup=2, single_s=2.5):
             one(int(t))
         if k >= warmup:
             st.append(time.perf_counter() - t0)
-    single = ns / float(np.median(st))
+    sample_its = float(its[sample].sum())
+    single_raw = ns / float(np.median(st))
+    single = single_raw * (sample_its / ns) / max(float(its.mean()), 1e-9)
     return out, dict(multi=multi, multi_runs=times, single=single, single_n=ns,
-                     single_runs=st)
+                     single_runs=st, single_raw=single_raw,
+                     single_its=sample_its / ns, all_its=float(its.mean()))
 
 
 REAL_GRIDS = {
@@ -513,10 +527,15 @@ def main():
                       f"({', '.join(f'{x:.2f}' for x in cb['multi_runs'])} s)",
             "single_core": {
                 "value": round(cb["single"], 3), "cores": 1,
-                "sample": f"{cb['single_n']} contingencies spread over the "
-                          f"{nsolve} in sequence, "
-                          f"median of {len(cb['single_runs'])} runs after 2 "
-                          f"warm-up runs ({', '.join(f'{x:.2f}' for x in cb['single_runs'])} s)"},
+                "sample": f"{cb['single_n']} contingencies in sequence, one "
+                          f"from each of {cb['single_n']} equal strata of the "
+                          f"{nsolve} ranked by Newton iterations "
+                          f"({cb['single_its']:.2f} iterations per contingency "
+                          f"against {cb['all_its']:.2f} over all), median of "
+                          f"{len(cb['single_runs'])} runs after 2 warm-up runs "
+                          f"({', '.join(f'{x:.2f}' for x in cb['single_runs'])} s); "
+                          f"value scaled to the whole set by iterations",
+                "sample_rate": round(cb["single_raw"], 3)},
             "nproc": nproc, "cpu_model": model,
         }
         out["speedup_vs_cpu"] = round(value / cb["multi"], 2)
