@@ -1051,7 +1051,7 @@ static bpf_status solve_once(bpf_engine *e, bool *graph_failed) {
     if (s || *graph_failed) return s;
   }
   tbegin(e, "loss");
-  hipLaunchKernelGGL(k_loss, dim3(T), dim3(256), 0, e->stream, e->g, e->d,
+  hipLaunchKernelGGL(k_loss, dim3(T), dim3(1024), 0, e->stream, e->g, e->d,
                      e->d_Sf, e->d_St, e->d_loss);
   tend(e);
   HIPCHK(hipStreamSynchronize(e->stream));

@@ -1881,39 +1881,48 @@ __device__ __forceinline__ SlotVal<T, SP> slot_values(const B4<T> *A,
   return v;
 }
 
-// chunk c (slots cur, blocks val, both received); before its reduction the
-// blocks of chunk c + 1 are requested from its slots (nxt) and the slots of
-// chunk c + 2 (c_after, clamped to the last chunk) into cur
+// chunk c (range m; slots cur, blocks val, both received): the ranges of
+// chunks c + 1 (mn) and c + 2 (ma) are read first, then every x the chunk
+// needs; while those reads are in flight the blocks of chunk c + 1 are
+// requested from its slots (nxt) and the slots of chunk c + 2 into cur.
+// No slot of a chunk reads an x that another slot of it writes (its columns
+// are rows of earlier levels, and a row has one segment per chunk), so the
+// entries' x_j and the rows' own x_i are all read before any is written: one
+// LDS round trip per chunk.  Lanes without an entry or a row read x_0 (a
+// broadcast, discarded), which keeps the reads unconditional and the wait
+// for the two ranges exact.  Returns mn.
 template <typename T, int SP, bool TR = false>
-__device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
-                                           const B4<T> *A, T *X, int c,
+__device__ __forceinline__ int4 slot_chunk(const int4 *slot, const int4 *swc,
+                                           const B4<T> *A, T *X, const int4 m,
                                            bool bw, SlotIdx<SP> &cur,
                                            const SlotVal<T, SP> &val,
                                            int c_next, bool next_bw,
                                            const SlotIdx<SP> &nxt,
                                            SlotVal<T, SP> &nval, int c_after) {
   __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
-  int4 m = swc[c];
-  m.x = __builtin_amdgcn_readfirstlane(m.x);
-  m.y = __builtin_amdgcn_readfirstlane(m.y);
-  m.z = __builtin_amdgcn_readfirstlane(m.z);
-  const int4 mn = swc[c_next], ma = swc[c_after];
-  // the fields of this chunk's slots, so that the slots of chunk c + 2 can
-  // be requested into the same registers at once (a load into registers
-  // still in use would be waited for), then the blocks of chunk c + 1: both
-  // have the whole chunk to arrive
+  int4 mn = swc[c_next], ma = swc[c_after];
   int col[SP], lead[SP], lg[SP];
+  T y0[SP], y1[SP], x0[SP], x1[SP];
 #pragma unroll
   for (int k = 0; k < SP; k++) {
     const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
-    if (wave_any(q, m.y)) {
-      const int y = pinned(cur.s[k].y), z = pinned(cur.s[k].z);
-      const int colp = y & ((1 << SLOT_LG_SHIFT) - 1);
-      col[k] = q < m.y ? colp - 1 : -1;
-      lead[k] = q < m.y ? z : -1;
-      lg[k] = y >> SLOT_LG_SHIFT;
-    }
+    const int y = pinned(cur.s[k].y), z = pinned(cur.s[k].z);
+    const int colp = y & ((1 << SLOT_LG_SHIFT) - 1);
+    col[k] = q < m.y ? colp - 1 : -1;
+    lead[k] = q < m.y ? z : -1;
+    lg[k] = y >> SLOT_LG_SHIFT;
+    const int cc = max(col[k], 0);
+    const int ii = max(lead[k], 0) & (SLOT_NOT_LAST - 1);
+    y0[k] = X[2 * cc];
+    y1[k] = X[2 * cc + 1];
+    x0[k] = X[2 * ii];
+    x1[k] = X[2 * ii + 1];
   }
+  mn.x = __builtin_amdgcn_readfirstlane(mn.x);
+  mn.y = __builtin_amdgcn_readfirstlane(mn.y);
+  mn.z = __builtin_amdgcn_readfirstlane(mn.z);
+  ma.x = __builtin_amdgcn_readfirstlane(ma.x);
+  ma.y = __builtin_amdgcn_readfirstlane(ma.y);
   nval = slot_values<T, SP, TR>(A, mn, next_bw, nxt);
   cur = slot_index<SP>(slot, ma);
   T p0[SP], p1[SP];
@@ -1921,16 +1930,14 @@ __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
   for (int k = 0; k < SP; k++) {
     const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
     if (wave_any(q, m.y)) {
-      const int cc = max(col[k], 0);
-      const T y0 = X[2 * cc], y1 = X[2 * cc + 1];
       const B4<T> &a = val.a[k];
       const bool ent = col[k] >= 0;
       if (TR) {   // the block (col, row) used transposed
-        p0[k] = ent ? a.a * y0 + a.c * y1 : (T)0;
-        p1[k] = ent ? a.b * y0 + a.d * y1 : (T)0;
+        p0[k] = ent ? a.a * y0[k] + a.c * y1[k] : (T)0;
+        p1[k] = ent ? a.b * y0[k] + a.d * y1[k] : (T)0;
       } else {
-        p0[k] = ent ? a.a * y0 + a.b * y1 : (T)0;
-        p1[k] = ent ? a.c * y0 + a.d * y1 : (T)0;
+        p0[k] = ent ? a.a * y0[k] + a.b * y1[k] : (T)0;
+        p1[k] = ent ? a.c * y0[k] + a.d * y1[k] : (T)0;
       }
     }
   }
@@ -1943,24 +1950,25 @@ __device__ __forceinline__ void slot_chunk(const int4 *slot, const int4 *swc,
         // a row's earlier segments only subtract; its last one (or its only
         // one) also applies the pivot inverse in the backward sweep
         const int i = lead[k] & (SLOT_NOT_LAST - 1);
-        const T x0 = X[2 * i] - p0[k], x1 = X[2 * i + 1] - p1[k];
+        const T u0 = x0[k] - p0[k], u1 = x1[k] - p1[k];
         if ((TR ? !bw : bw) && !(lead[k] & SLOT_NOT_LAST)) {
           const B4<T> &D = val.d[k];
           if (TR) {   // the pivot inverse transposed
-            X[2 * i] = D.a * x0 + D.c * x1;
-            X[2 * i + 1] = D.b * x0 + D.d * x1;
+            X[2 * i] = D.a * u0 + D.c * u1;
+            X[2 * i + 1] = D.b * u0 + D.d * u1;
           } else {
-            X[2 * i] = D.a * x0 + D.b * x1;
-            X[2 * i + 1] = D.c * x0 + D.d * x1;
+            X[2 * i] = D.a * u0 + D.b * u1;
+            X[2 * i + 1] = D.c * u0 + D.d * u1;
           }
         } else {
-          X[2 * i] = x0;
-          X[2 * i + 1] = x1;
+          X[2 * i] = u0;
+          X[2 * i + 1] = u1;
         }
       }
     }
   }
   __syncthreads();
+  return mn;
 }
 
 template <typename T, int SP, bool TR = false>
@@ -1978,20 +1986,24 @@ __device__ __forceinline__ void lu_sweeps_slot(const B4<T> *A, const B4<T> *Ti,
   const int cl = nc - 1;
   SlotIdx<SP> ia, ib;
   SlotVal<T, SP> va, vb;
-  ia = slot_index<SP>(slots, swc[0]);
+  int4 m = swc[0];
+  m.x = __builtin_amdgcn_readfirstlane(m.x);
+  m.y = __builtin_amdgcn_readfirstlane(m.y);
+  m.z = __builtin_amdgcn_readfirstlane(m.z);
+  ia = slot_index<SP>(slots, m);
   __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
-  va = slot_values<T, SP, TR>(A, swc[0], 0 >= nfw, ia);
+  va = slot_values<T, SP, TR>(A, m, 0 >= nfw, ia);
   ib = slot_index<SP>(slots, swc[min(1, cl)]);
   for (int c = 0; c < nc; c += 2) {
     if (c == nfw && tail) tail_rows<T>(A, Ti, X, rs, P, TR);
-    slot_chunk<T, SP, TR>(slots, swc, A, X, c, c >= nfw, ia, va,
-                          min(c + 1, cl), min(c + 1, cl) >= nfw, ib, vb,
-                          min(c + 2, cl));
+    m = slot_chunk<T, SP, TR>(slots, swc, A, X, m, c >= nfw, ia, va,
+                              min(c + 1, cl), min(c + 1, cl) >= nfw, ib, vb,
+                              min(c + 2, cl));
     if (c + 1 >= nc) break;
     if (c + 1 == nfw && tail) tail_rows<T>(A, Ti, X, rs, P, TR);
-    slot_chunk<T, SP, TR>(slots, swc, A, X, c + 1, c + 1 >= nfw, ib, vb,
-                          min(c + 2, cl), min(c + 2, cl) >= nfw, ia, va,
-                          min(c + 3, cl));
+    m = slot_chunk<T, SP, TR>(slots, swc, A, X, m, c + 1 >= nfw, ib, vb,
+                              min(c + 2, cl), min(c + 2, cl) >= nfw, ia, va,
+                              min(c + 3, cl));
   }
   if (nc == nfw && tail) tail_rows<T>(A, Ti, X, rs, P, TR);
 }
