@@ -91,7 +91,7 @@ typedef struct {
    * mapol_amd/tune.py picks them per case and instance count) */
   int32_t tail_limit;      /* pivots of the dense tail, 1..64 */
   int32_t chunk_products;  /* block products per factorization chunk, <= 4096 */
-  int32_t slot_chunk;      /* lane slots per slot-form sweep chunk, 64..2048 */
+  int32_t slot_chunk;      /* lane slots per slot-form sweep chunk, a multiple of 64 up to 2048 */
   int32_t ordering;        /* 0 minimum degree; 1 the same, reordered to the
                               least elimination-tree height (same fill);
                               2 nested dissection */

@@ -42,7 +42,9 @@ static unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b
 static int sweep_mode(const Symbolic &S, size_t scalar, int use_lds, int form,
                       int threads) {
   if (!use_lds || form == 2) return 0;
-  if (form == 0 && S.sl_ok && (threads == 1024 || threads == 512 || threads == 256))
+  // the slot form: X after a zero pair in the solve LDS
+  if (form == 0 && S.sl_ok && (threads == 1024 || threads == 512 || threads == 256) &&
+      (2 * (size_t)S.nb + SLOT_X_PAD) * scalar <= (size_t)SOLVE_LDS_BYTES)
     return 2;
   if (!S.sw_ok) return 0;
   return ((size_t)ep_offset(S.nb) + 2 * (size_t)SWEEP_CHUNK) * scalar <=

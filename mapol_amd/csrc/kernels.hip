@@ -654,6 +654,44 @@ __device__ __forceinline__ B4<T> lane_range_sum(const B4<T> *buf, int q, int q1,
   return e0;
 }
 
+// the shares of two product ranges at once (an L output's own range and its
+// pivot's): two partial sums per range, the reads of both ranges in flight
+// together.  A lane past the end of a range reads the zero block z instead
+// (adding zero leaves a sum as it is).
+template <typename T>
+__device__ __forceinline__ void lane_range_sum2(const B4<T> *buf, const B4<T> *z,
+                                                int q, int q1, int r, int r1,
+                                                int gsz, B4<T> &e, B4<T> &f) {
+  B4<T> e0 = {0, 0, 0, 0}, e1 = e0, f0 = e0, f1 = e0;
+  while (q < q1 || r < r1) {
+    const B4<T> a0 = *(q < q1 ? buf + q : z), a1 = *(q + gsz < q1 ? buf + q + gsz : z);
+    const B4<T> b0 = *(r < r1 ? buf + r : z), b1 = *(r + gsz < r1 ? buf + r + gsz : z);
+    blk_acc(e0, a0); blk_acc(e1, a1);
+    blk_acc(f0, b0); blk_acc(f1, b1);
+    q += 2 * gsz;
+    r += 2 * gsz;
+  }
+  blk_acc(e0, e1);
+  blk_acc(f0, f1);
+  e = e0;
+  f = f0;
+}
+
+// group_sum4_first on two blocks at once (independent chains interleaved)
+template <typename T>
+__device__ __forceinline__ void group_sum4x2_first(B4<T> &v, B4<T> &w, int gsz) {
+#define BPF_GS2(STEP)                                                     \
+  v.a += STEP(v.a); v.b += STEP(v.b); v.c += STEP(v.c); v.d += STEP(v.d); \
+  w.a += STEP(w.a); w.b += STEP(w.b); w.c += STEP(w.c); w.d += STEP(w.d);
+  if (gsz > 1) { BPF_GS2(dpp_row<0x101>) }
+  if (gsz > 2) { BPF_GS2(dpp_row<0x102>) }
+  if (gsz > 4) { BPF_GS2(dpp_row<0x104>) }
+  if (gsz > 8) { BPF_GS2(dpp_row<0x108>) }
+  if (gsz > 16) { BPF_GS2(lane_up16) }
+  if (gsz > 32) { BPF_GS2(lane_up32) }
+#undef BPF_GS2
+}
+
 template <typename T>
 __device__ __forceinline__ B4<T> blk_zero() {
   B4<T> z;
@@ -964,8 +1002,11 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   // chunks below c_start (level 0) were finished by the Jacobian fill, which
   // flags a singular pivot in jsing[t] (taken over and cleared here)
   __shared__ int sing;
+  // read by the reductions for a range that has ended
+  __shared__ B4<T> zblk;
   if (threadIdx.x == 0) {
     sing = 0;
+    zblk = blk_zero<T>();
     if (c_start > 0) {
       sing = jsing[t];
       jsing[t] = 0;
@@ -1083,14 +1124,26 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       for (int u = 0; u < NB; u++) {
         if (kb + u * stride >= k1) break;
         const int p = ta[u].x & TASK_POS_MASK;
-        B4<T> e = lane_range_sum(buf, ta[u].y + lane - q0, ta[u].z - q0, gsz);
-        group_sum4_first(e, gsz);
-        if (ta[u].x & TASK_LCOL) {
-          const unsigned pw = (unsigned)ta[u].w;
-          const int v0 = (int)(pw & ((1u << TK_LEN_SHIFT) - 1));
-          const int v1 = v0 + (int)(pw >> TK_LEN_SHIFT);
-          B4<T> pv = lane_range_sum(buf, v0 + lane - q0, v1 - q0, gsz);
-          group_sum4_first(pv, gsz);
+        const bool lcol = ta[u].x & TASK_LCOL;
+        // an L output also sums its pivot's range (empty for the others);
+        // in FP32 both ranges are read together (FP64 has not the registers)
+        const unsigned pw = (unsigned)ta[u].w;
+        const int v0 = (int)(pw & ((1u << TK_LEN_SHIFT) - 1));
+        const int v1 = lcol ? v0 + (int)(pw >> TK_LEN_SHIFT) : v0;
+        B4<T> e, pv;
+        if constexpr (sizeof(T) == 4) {
+          lane_range_sum2(buf, &zblk, ta[u].y + lane - q0, ta[u].z - q0,
+                          v0 + lane - q0, v1 - q0, gsz, e, pv);
+          group_sum4x2_first(e, pv, gsz);
+        } else {
+          e = lane_range_sum(buf, ta[u].y + lane - q0, ta[u].z - q0, gsz);
+          group_sum4_first(e, gsz);
+          if (lcol) {
+            pv = lane_range_sum(buf, v0 + lane - q0, v1 - q0, gsz);
+            group_sum4_first(pv, gsz);
+          }
+        }
+        if (lcol) {
           if (lane == 0) {
             B4<T> inv;
             blk_inv(pv, inv);   // a singular pivot is flagged by its own output
@@ -1792,8 +1845,8 @@ __device__ __forceinline__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti,
 // first lane.  An unconditional tree (step s: lane i adds lane i + 2^s) leaves
 // in the first lane of every aligned window of 2^(s+1) lanes that window's
 // sum, whatever the groups; a group's first lane keeps the value after step
-// lg - 1, its own window.  glg = the largest lg of the chunk.  Every lane of
-// the wave must take part.
+// lg - 1, its own window.  glg = the largest lg among the wave's groups.
+// Every lane of the wave must take part.
 template <typename T>
 __device__ __forceinline__ void slot_group_sum(T &p0, T &p1, int lg, int glg) {
   T r0 = p0, r1 = p1;
@@ -1888,9 +1941,12 @@ __device__ __forceinline__ SlotVal<T, SP> slot_values(const B4<T> *A,
 // No slot of a chunk reads an x that another slot of it writes (its columns
 // are rows of earlier levels, and a row has one segment per chunk), so the
 // entries' x_j and the rows' own x_i are all read before any is written: one
-// LDS round trip per chunk.  Lanes without an entry or a row read x_0 (a
-// broadcast, discarded), which keeps the reads unconditional and the wait
-// for the two ranges exact.  Returns mn.
+// LDS round trip per chunk.  A chunk spans whole 64-slot windows, so a wave
+// whose window starts inside it has every lane on a slot of it: a slot
+// without an entry (column 0) reads the zero pair X[-2], X[-1] and forms a
+// zero product without a test; a lane without a row reads x_0 (a broadcast,
+// discarded).  The reads are unconditional, which keeps the wait for the
+// two ranges exact.  Returns mn.
 template <typename T, int SP, bool TR = false>
 __device__ __forceinline__ int4 slot_chunk(const int4 *slot, const int4 *swc,
                                            const B4<T> *A, T *X, const int4 m,
@@ -1901,18 +1957,15 @@ __device__ __forceinline__ int4 slot_chunk(const int4 *slot, const int4 *swc,
                                            SlotVal<T, SP> &nval, int c_after) {
   __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
   int4 mn = swc[c_next], ma = swc[c_after];
-  int col[SP], lead[SP], lg[SP];
+  int lead[SP], lg[SP];
   T y0[SP], y1[SP], x0[SP], x1[SP];
 #pragma unroll
   for (int k = 0; k < SP; k++) {
-    const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
     const int y = pinned(cur.s[k].y), z = pinned(cur.s[k].z);
-    const int colp = y & ((1 << SLOT_LG_SHIFT) - 1);
-    col[k] = q < m.y ? colp - 1 : -1;
-    lead[k] = q < m.y ? z : -1;
+    const int cc = (y & ((1 << SLOT_LG_SHIFT) - 1)) - 1;
+    lead[k] = z;
     lg[k] = y >> SLOT_LG_SHIFT;
-    const int cc = max(col[k], 0);
-    const int ii = max(lead[k], 0) & (SLOT_NOT_LAST - 1);
+    const int ii = max(z, 0) & (SLOT_NOT_LAST - 1);
     y0[k] = X[2 * cc];
     y1[k] = X[2 * cc + 1];
     x0[k] = X[2 * ii];
@@ -1931,13 +1984,12 @@ __device__ __forceinline__ int4 slot_chunk(const int4 *slot, const int4 *swc,
     const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
     if (wave_any(q, m.y)) {
       const B4<T> &a = val.a[k];
-      const bool ent = col[k] >= 0;
       if (TR) {   // the block (col, row) used transposed
-        p0[k] = ent ? a.a * y0[k] + a.c * y1[k] : (T)0;
-        p1[k] = ent ? a.b * y0[k] + a.d * y1[k] : (T)0;
+        p0[k] = a.a * y0[k] + a.c * y1[k];
+        p1[k] = a.b * y0[k] + a.d * y1[k];
       } else {
-        p0[k] = ent ? a.a * y0[k] + a.b * y1[k] : (T)0;
-        p1[k] = ent ? a.c * y0[k] + a.d * y1[k] : (T)0;
+        p0[k] = a.a * y0[k] + a.b * y1[k];
+        p1[k] = a.c * y0[k] + a.d * y1[k];
       }
     }
   }
@@ -1945,7 +1997,9 @@ __device__ __forceinline__ int4 slot_chunk(const int4 *slot, const int4 *swc,
   for (int k = 0; k < SP; k++) {
     const int q = m.x + (int)threadIdx.x + k * (int)blockDim.x;
     if (wave_any(q, m.y)) {
-      slot_group_sum(p0[k], p1[k], lg[k], m.z);
+      // groups come largest first, each aligned to its size, so the wave's
+      // first lane starts its largest group: the steps that group needs
+      slot_group_sum(p0[k], p1[k], lg[k], __builtin_amdgcn_readfirstlane(lg[k]));
       if (lead[k] >= 0) {
         // a row's earlier segments only subtract; its last one (or its only
         // one) also applies the pivot inverse in the backward sweep
@@ -2018,6 +2072,8 @@ __device__ __forceinline__ void lu_sweeps_slot(const B4<T> *A, const B4<T> *Ti,
 // i.e. 18944 blocks in FP32 or 9472 in FP64; larger systems keep the vector
 // in global memory (use_lds = 0)
 constexpr int SOLVE_LDS_BYTES = 148 * 1024;
+// the slot form keeps a zero pair before X (the x of slots without an entry)
+constexpr int SLOT_X_PAD = 2;
 // the entry-parallel sweeps keep their product buffer (2 * SWEEP_CHUNK
 // scalars) after X in the same LDS block
 __host__ __device__ constexpr int ep_offset(int nb) { return (2 * nb + 3) & ~3; }
@@ -2101,9 +2157,11 @@ k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, const B4<T> *TL,
     }
   double *dx = d.dx + (size_t)t * 2 * g.nb;
   T *prod = smem + ep_offset(g.nb);
+  // the slot form keeps X after a zero pair (read by slots without an entry)
+  if (SP > 0 && threadIdx.x < SLOT_X_PAD) smem[threadIdx.x] = (T)0;
   if (SP > 0)
-    solve_pass<T, true, SP>(g, A, Ti, smem, prod, rs, s_swc, s_swm, P, 0, src,
-                            rhs_mode, accumulate, dx);
+    solve_pass<T, true, SP>(g, A, Ti, smem + SLOT_X_PAD, prod, rs, s_swc, s_swm,
+                            P, 0, src, rhs_mode, accumulate, dx);
   else if (use_lds)
     solve_pass<T, true, 0>(g, A, Ti, smem, prod, rs, s_swc, s_swm, P, ep,
                            src, rhs_mode, accumulate, dx);
@@ -2416,11 +2474,14 @@ template <int SP>
 __global__ void __launch_bounds__(SLOT_CHUNK / SP)
 k_adj_solve_slot(DevTopo g, DevInst d, const B4<double> *LU, const B4<double> *TL,
                  const double *G, double *lam, SolvePlan P) {
-  __shared__ double X[SOLVE_LDS_BYTES / sizeof(double)];
+  __shared__ double Xs[SOLVE_LDS_BYTES / sizeof(double)];
   __shared__ double rs[2 * TAIL_MAX];
   __shared__ int4 s_swc[MAX_CHUNKS];
   const int t = blockIdx.x;
   if (d.st[t].done) return;
+  // X after a zero pair (read by slots without an entry)
+  double *X = Xs + SLOT_X_PAD;
+  if (threadIdx.x < SLOT_X_PAD) Xs[threadIdx.x] = 0.0;
   const B4<double> *A = LU + (size_t)t * g.nnzLU;
   const B4<double> *Ti = TL + (size_t)t * P.tn * P.tn;
   const double *Gt = G + (size_t)t * 2 * g.nb;

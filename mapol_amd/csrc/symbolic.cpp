@@ -287,11 +287,11 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   if (prm.tail_limit < 0 || prm.tail_limit > TAIL_MAX || prm.chunk_pairs < 0 ||
       prm.chunk_pairs > CHUNK_PAIRS || prm.slot_chunk < 0 ||
       prm.slot_chunk > SLOT_CHUNK ||
-      (prm.slot_chunk && prm.slot_chunk < 64) || prm.ordering < 0 ||
+      prm.slot_chunk % 64 || prm.ordering < 0 ||
       prm.ordering > 2) {
     if (err) *err = "tile sizes out of range (tail <= " + std::to_string(TAIL_MAX) +
                     " pivots, chunk <= " + std::to_string(CHUNK_PAIRS) +
-                    " products, 64 <= slots <= " + std::to_string(SLOT_CHUNK) + ")";
+                    " products, slots a multiple of 64 up to " + std::to_string(SLOT_CHUNK) + ")";
     return false;
   }
   S.prm.tail_limit = prm.tail_limit ? prm.tail_limit : TAIL_MAX;
@@ -898,8 +898,14 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
             glg = std::max(glg, lg);
             r++;
           }
-          const int s1 = (int)slot.size() / 4;
+          int s1 = (int)slot.size() / 4;
           if (s1 == s0) continue;
+          // the chunk ends on a multiple of 64 slots, so that a wave's 64
+          // slots lie wholly inside a chunk or wholly outside it: padding
+          // slots carry no entry (column 0) and no row (-1); their block
+          // position repeats the last real slot's
+          const int padpos = slot[slot.size() - 4];
+          for (; (s1 - s0) % 64; s1++) slot.insert(slot.end(), {padpos, 0, -1, padpos});
           chunk.insert(chunk.end(), {s0, s1, glg, 0});
           nch++;
         }

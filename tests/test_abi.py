@@ -122,6 +122,6 @@ def test_tile_options_are_validated():
     from mapol_amd.engine import Engine
     case, _ = synth.workload("pf118x200", T=2)
     for bad in (dict(tail_limit=65), dict(chunk_products=8192), dict(slot_chunk=32),
-                dict(ordering=3)):
+                dict(slot_chunk=1000), dict(ordering=3)):
         with pytest.raises(Exception, match="out of range|tile"):
             Engine(case, **bad)

@@ -227,6 +227,8 @@ def _sweeps_slot(a, A, Sinv, b):
         q = 0
         while q < s1 - s0:
             pos, w, lead, piv = sl[s0 + q]
+            if lead < 0:        # padding to the chunk's 64-slot end
+                break
             g = w >> LG
             row, last = lead & (NOT_LAST - 1), not lead & NOT_LAST
             v = prod[q:q + (1 << g)]
@@ -272,6 +274,8 @@ def _sweeps_slot_t(a, A, Sinv, g):
         q = 0
         while q < s1 - s0:
             pos, w, lead, piv = sl[s0 + q]
+            if lead < 0:        # padding to the chunk's 64-slot end
+                break
             gl = w >> LG
             row, last = lead & (NOT_LAST - 1), not lead & NOT_LAST
             v = prod[q:q + (1 << gl)]
@@ -536,8 +540,16 @@ def _check_slots(a, tail_n):
         for c in range(lo, hi):
             s0, s1 = ch[c, 0], ch[c, 1]
             q, gmax, rows_here = s0, 0, set()
+            assert s0 % 64 == 0 and s1 % 64 == 0
             while q < s1:
                 pos, w, lead, piv = sl[q]
+                if lead < 0:
+                    # padding to the chunk's 64-slot end: no entry, no row,
+                    # fewer than 64 slots
+                    pad = sl[q:s1]
+                    assert s1 - q < 64 and np.all(pad[:, 1] == 0) and np.all(pad[:, 2] == -1)
+                    assert np.all(pad[:, 0] == pad[:, 3]) and np.all(pad[:, 0] == sl[q - 1, 0])
+                    break
                 g = w >> LG
                 gmax = max(gmax, g)
                 row, last = lead & (NOT_LAST - 1), not lead & NOT_LAST
