@@ -95,6 +95,12 @@ typedef struct {
   int32_t ordering;        /* 0 minimum degree; 1 the same, reordered to the
                               least elimination-tree height (same fill);
                               2 nested dissection */
+  int32_t bus_numbering;   /* the engine's own bus numbers: 1 (default) reverse
+                              Cuthill-McKee, so a bus's neighbours sit near it
+                              in every per-bus array; 0 the caller's.  The
+                              elimination order, and every bus index, value
+                              and matrix the interface takes or returns, stay
+                              in the caller's numbering either way. */
 } bpf_options;
 
 /* one control of the reference's particle vector (Particula::Estrutura) */

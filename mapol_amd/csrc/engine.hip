@@ -6,10 +6,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <map>
 #include <mutex>
+#include <numeric>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/batchpf.h"
@@ -63,9 +67,14 @@ struct bpf_engine {
   std::string err;
   int dev = 0;
   hipStream_t stream = nullptr;
-  // case (host copies)
+  // case (host copies, buses in the engine's numbering)
   int n = 0, m = 0, ng = 0;
   std::vector<int> bus_type, br_from, br_to, gen_bus;
+  // the engine's bus numbers: bus_ext[k] is the caller's bus k is, bus_int
+  // the inverse; the caller's Ybus pattern (rows by the caller's bus, columns
+  // ascending) and, for each of its entries, the engine's Ybus position
+  std::vector<int> bus_ext, bus_int;
+  std::vector<int> yx_row, yx_col, yx_pos;
   std::vector<double> Pd, Qd, Gs, Bs, Vm, Va, R, X, B, tap, shift, Pg, Qg,
       Qmax, Qmin, VG;
   std::vector<uint8_t> br_status, gen_status;
@@ -226,6 +235,7 @@ extern "C" void bpf_default_options(bpf_options *o) {
   o->chunk_products = 0;
   o->slot_chunk = 0;
   o->ordering = 0;
+  o->bus_numbering = 1;
 }
 
 extern "C" const char *bpf_last_error(const bpf_engine *e) {
@@ -251,6 +261,30 @@ extern "C" void bpf_destroy(bpf_engine *e) {
 template <typename T>
 static std::vector<T> vcopy(const T *p, int n) {
   return p ? std::vector<T>(p, p + n) : std::vector<T>();
+}
+
+// rows of T per-bus values between the caller's bus numbering and the
+// engine's: in, dst[t][k] = src[t][bus_ext[k]]; out, the inverse
+template <class V>
+static void bus_rows_in(const bpf_engine *e, const V *src, V *dst, size_t T) {
+  const size_t n = e->n;
+  for (size_t t = 0; t < T; t++)
+    for (size_t k = 0; k < n; k++) dst[t * n + k] = src[t * n + e->bus_ext[k]];
+}
+template <class V>
+static void bus_rows_out(const bpf_engine *e, const V *src, V *dst, size_t T) {
+  const size_t n = e->n;
+  for (size_t t = 0; t < T; t++)
+    for (size_t k = 0; k < n; k++) dst[t * n + e->bus_ext[k]] = src[t * n + k];
+}
+
+// T rows of per-bus values from the device, in the caller's numbering
+template <class V>
+static bpf_status bus_rows_download(bpf_engine *e, const void *dsrc, V *dst, size_t T) {
+  std::vector<V> h(T * e->n);
+  HIPCHK(hipMemcpy(h.data(), dsrc, sizeof(V) * h.size(), hipMemcpyDeviceToHost));
+  bus_rows_out(e, h.data(), dst, T);
+  return BPF_OK;
 }
 
 extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
@@ -279,37 +313,90 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   if (o) e->opt = *o; else bpf_default_options(&e->opt);
   if (e->opt.alg < 0 || e->opt.alg > 2 || e->opt.precision < 0 ||
       e->opt.precision > 1 || e->opt.refine_steps < 0 || e->opt.maxit_nr < 0 ||
-      e->opt.maxit_fd < 0 || !(e->opt.tol > 0)) {
+      e->opt.maxit_fd < 0 || !(e->opt.tol > 0) || e->opt.bus_numbering < 0 ||
+      e->opt.bus_numbering > 1) {
     e->err = "invalid options";
     return BPF_ERR_INVALID_ARG;
   }
-  e->n = c->nbus; e->m = c->nbranch; e->ng = c->ngen;
-  e->bus_type = vcopy(c->bus_type, e->n);
-  e->Pd = vcopy(c->Pd, e->n); e->Qd = vcopy(c->Qd, e->n);
-  e->Gs = vcopy(c->Gs, e->n); e->Bs = vcopy(c->Bs, e->n);
-  e->Vm = vcopy(c->Vm, e->n); e->Va = vcopy(c->Va, e->n);
-  e->br_from = vcopy(c->br_from, e->m); e->br_to = vcopy(c->br_to, e->m);
-  e->R = vcopy(c->R, e->m); e->X = vcopy(c->X, e->m); e->B = vcopy(c->B, e->m);
-  e->tap = vcopy(c->tap, e->m); e->shift = vcopy(c->shift, e->m);
-  e->br_status = vcopy(c->br_status, e->m);
-  e->gen_bus = vcopy(c->gen_bus, e->ng);
-  e->Pg = vcopy(c->Pg, e->ng); e->Qg = vcopy(c->Qg, e->ng);
-  e->Qmax = vcopy(c->Qmax, e->ng); e->Qmin = vcopy(c->Qmin, e->ng);
-  e->VG = vcopy(c->VG, e->ng); e->gen_status = vcopy(c->gen_status, e->ng);
+  const int n = c->nbus;
+  e->n = n; e->m = c->nbranch; e->ng = c->ngen;
+  for (int b = 0; b < e->m; b++)
+    if (c->br_from[b] < 0 || c->br_from[b] >= n || c->br_to[b] < 0 ||
+        c->br_to[b] >= n) {
+      e->err = "branch " + std::to_string(b) + " has a bus out of range";
+      return BPF_ERR_INVALID_ARG;
+    }
   for (int k = 0; k < e->ng; k++)
-    if (e->gen_bus[k] < 0 || e->gen_bus[k] >= e->n) {
+    if (c->gen_bus[k] < 0 || c->gen_bus[k] >= n) {
       e->err = "generator bus out of range";
       return BPF_ERR_INVALID_ARG;
     }
+  // The engine numbers buses by reverse Cuthill-McKee: the per-bus kernels
+  // gather over each bus's branches, and near numbers put those reads in
+  // few cache lines.  The elimination order is taken from the caller's
+  // numbering (minimum degree breaks ties by bus number), so the factors and
+  // schedules are those of the case as given.
+  std::vector<int> order;
+  e->bus_ext.resize(n);
+  if (e->opt.bus_numbering) {
+    order = elimination_order(n, e->m, c->br_from, c->br_to, c->bus_type,
+                              e->opt.ordering);
+    e->bus_ext = rcm_numbering(n, e->m, c->br_from, c->br_to);
+  } else {
+    std::iota(e->bus_ext.begin(), e->bus_ext.end(), 0);
+  }
+  e->bus_int.assign(n, 0);
+  for (int k = 0; k < n; k++) e->bus_int[e->bus_ext[k]] = k;
+  for (int &b : order) b = e->bus_int[b];
+  auto bus_vec = [&](auto *p) {
+    std::vector<std::remove_const_t<std::remove_pointer_t<decltype(p)>>> v(n);
+    for (int k = 0; k < n; k++) v[k] = p[e->bus_ext[k]];
+    return v;
+  };
+  auto bus_ids = [&](const int *p, int cnt) {
+    std::vector<int> v(cnt);
+    for (int k = 0; k < cnt; k++) v[k] = e->bus_int[p[k]];
+    return v;
+  };
+  e->bus_type = bus_vec(c->bus_type);
+  e->Pd = bus_vec(c->Pd); e->Qd = bus_vec(c->Qd);
+  e->Gs = bus_vec(c->Gs); e->Bs = bus_vec(c->Bs);
+  e->Vm = bus_vec(c->Vm); e->Va = bus_vec(c->Va);
+  e->br_from = bus_ids(c->br_from, e->m); e->br_to = bus_ids(c->br_to, e->m);
+  e->R = vcopy(c->R, e->m); e->X = vcopy(c->X, e->m); e->B = vcopy(c->B, e->m);
+  e->tap = vcopy(c->tap, e->m); e->shift = vcopy(c->shift, e->m);
+  e->br_status = vcopy(c->br_status, e->m);
+  e->gen_bus = bus_ids(c->gen_bus, e->ng);
+  e->Pg = vcopy(c->Pg, e->ng); e->Qg = vcopy(c->Qg, e->ng);
+  e->Qmax = vcopy(c->Qmax, e->ng); e->Qmin = vcopy(c->Qmin, e->ng);
+  e->VG = vcopy(c->VG, e->ng); e->gen_status = vcopy(c->gen_status, e->ng);
   {
     AnalysisParams prm;
     prm.tail_limit = e->opt.tail_limit;
     prm.chunk_pairs = e->opt.chunk_products;
     prm.slot_chunk = e->opt.slot_chunk;
     prm.ordering = e->opt.ordering;
+    prm.order_bus = order;
     if (!analyse(e->n, e->m, e->br_from.data(), e->br_to.data(),
                  e->bus_type.data(), e->S, &e->err, prm))
       return BPF_ERR_INVALID_ARG;
+  }
+  {
+    // the caller's Ybus pattern and where each of its entries sits here
+    const Symbolic &S = e->S;
+    std::vector<std::pair<int, int>> rc;
+    e->yx_row.assign(n + 1, 0);
+    e->yx_col.clear();
+    e->yx_pos.clear();
+    for (int r = 0; r < n; r++) {
+      const int R = e->bus_int[r];
+      rc.clear();
+      for (int p = S.yrow[R]; p < S.yrow[R + 1]; p++)
+        rc.push_back({e->bus_ext[S.ycol[p]], p});
+      std::sort(rc.begin(), rc.end());
+      for (auto &x : rc) { e->yx_col.push_back(x.first); e->yx_pos.push_back(x.second); }
+      e->yx_row[r + 1] = (int)e->yx_col.size();
+    }
   }
   // device
   const char *dbg = getenv("BPF_DEBUG_SYNC");
@@ -603,7 +690,12 @@ extern "C" bpf_status bpf_set_instances(bpf_engine *e, int32_t T,
   double *d_ctlv = nullptr;
   if (nctl > 0) {
     std::vector<int2> cv(nctl);
-    for (int c = 0; c < nctl; c++) { cv[c].x = ctl[c].kind; cv[c].y = ctl[c].id; }
+    for (int c = 0; c < nctl; c++) {
+      const int k = ctl[c].kind;
+      cv[c].x = k;
+      cv[c].y = (k == BPF_CTL_AVR || k == BPF_CTL_SHC) ? e->bus_int[ctl[c].id]
+                                                        : ctl[c].id;
+    }
     d_ctl = (int2 *)e->ctl_buf;
     d_ctlv = (double *)(d_ctl + nctl);
     HIPCHK(hipMemcpyAsync(d_ctl, cv.data(), sizeof(int2) * nctl,
@@ -1107,7 +1199,8 @@ extern "C" bpf_status bpf_get_results(bpf_engine *e, double *V, int32_t *its,
   if (!e) return BPF_ERR_INVALID_ARG;
   if (!e->solved) { e->err = "no solution yet"; return BPF_ERR_STATE; }
   const size_t T = e->T, n = e->n;
-  if (V) HIPCHK(hipMemcpy(V, e->d.V, sizeof(cd) * T * n, hipMemcpyDeviceToHost));
+  bpf_status s;
+  if (V && (s = bus_rows_download(e, e->d.V, (cd *)V, T))) return s;
   std::vector<InstState> st(T);
   HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * T, hipMemcpyDeviceToHost));
   for (size_t t = 0; t < T; t++) {
@@ -1119,7 +1212,7 @@ extern "C" bpf_status bpf_get_results(bpf_engine *e, double *V, int32_t *its,
   if (loss) HIPCHK(hipMemcpy(loss, e->d_loss, sizeof(double) * T, hipMemcpyDeviceToHost));
   if (bus_type) {
     std::vector<int8_t> ty(T * n);
-    HIPCHK(hipMemcpy(ty.data(), e->d.type, T * n, hipMemcpyDeviceToHost));
+    if ((s = bus_rows_download(e, e->d.type, ty.data(), T))) return s;
     for (size_t k = 0; k < T * n; k++) bus_type[k] = ty[k];
   }
   return BPF_OK;
@@ -1465,30 +1558,38 @@ extern "C" bpf_status bpf_get_ybus(bpf_engine *e, int32_t t, int32_t *rowptr,
     e->err = "no Ybus yet: call bpf_make_ybus or bpf_solve";
     return BPF_ERR_STATE;
   }
-  if (rowptr) memcpy(rowptr, e->S.yrow.data(), sizeof(int) * (e->n + 1));
-  if (colind) memcpy(colind, e->S.ycol.data(), sizeof(int) * e->S.ycol.size());
-  if (val)
-    HIPCHK(hipMemcpy(val, e->d.Y + (size_t)t * e->g.nnzY, sizeof(cd) * e->g.nnzY,
+  if (rowptr) memcpy(rowptr, e->yx_row.data(), sizeof(int) * (e->n + 1));
+  if (colind) memcpy(colind, e->yx_col.data(), sizeof(int) * e->yx_col.size());
+  if (val) {
+    std::vector<cd> y(e->g.nnzY);
+    HIPCHK(hipMemcpy(y.data(), e->d.Y + (size_t)t * e->g.nnzY, sizeof(cd) * e->g.nnzY,
                      hipMemcpyDeviceToHost));
+    cd *out = (cd *)val;
+    for (size_t p = 0; p < y.size(); p++) out[p] = y[e->yx_pos[p]];
+  }
   return BPF_OK;
 }
 
 // upload voltages and evaluate the mismatch (mode 2: no verdicts)
 static bpf_status stage_eval(bpf_engine *e, const double *V) {
   if (!e->solved) { e->err = "call bpf_solve first (Ybus is assembled there)"; return BPF_ERR_STATE; }
-  HIPCHK(hipMemcpy(e->d.V, V, sizeof(cd) * (size_t)e->T * e->n, hipMemcpyHostToDevice));
+  std::vector<cd> Vi((size_t)e->T * e->n);
+  bus_rows_in(e, (const cd *)V, Vi.data(), e->T);
+  HIPCHK(hipMemcpy(e->d.V, Vi.data(), sizeof(cd) * Vi.size(), hipMemcpyHostToDevice));
   mismatch_pair(e, 2, e->opt.maxit_nr, 0, 0);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
   return BPF_OK;
 }
 
-// block-ordered vector [T][2nb] -> reference layout [P(pv); P(pq); Q(pq)]
+// block-ordered vector [T][2nb] -> reference layout [P(pv); P(pq); Q(pq)],
+// pv and pq buses ascending in the caller's numbering
 static void to_ref_layout(const bpf_engine *e, const int8_t *ty,
                           const double *blk, double *out) {
   const Symbolic &S = e->S;
   std::vector<int> pv, pq;
-  for (int i = 0; i < e->n; i++) {
+  for (int x = 0; x < e->n; x++) {
+    const int i = e->bus_int[x];
     if (ty[i] == BPF_PV) pv.push_back(i);
     else if (ty[i] == BPF_PQ && S.blk_of_bus[i] >= 0) pq.push_back(i);
   }
@@ -1549,12 +1650,18 @@ extern "C" bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V,
   // reference indices: P row/angle col = indicePVPQ, Q row/magnitude col =
   // indicePVPQ + npq (newtonpf.h:256-291)
   const Symbolic &S = e->S;
+  // (indices and entries in the caller's bus order)
   std::vector<int> idx(e->n, -1);
   int npv = 0, npq = 0;
-  for (int i = 0; i < e->n; i++) if (ty[i] == BPF_PV) idx[i] = npv++;
-  for (int i = 0; i < e->n; i++) if (ty[i] == BPF_PQ && S.blk_of_bus[i] >= 0) idx[i] = npv + npq++;
+  for (int x = 0; x < e->n; x++)
+    if (ty[e->bus_int[x]] == BPF_PV) idx[e->bus_int[x]] = npv++;
+  for (int x = 0; x < e->n; x++) {
+    const int i = e->bus_int[x];
+    if (ty[i] == BPF_PQ && S.blk_of_bus[i] >= 0) idx[i] = npv + npq++;
+  }
   int cnt = 0;
-  for (int p = 0; p < e->g.nnzY; p++) {
+  for (int px = 0; px < e->g.nnzY; px++) {
+    const int p = e->yx_pos[px];
     int q = S.y2lu[p];
     if (q < 0) continue;
     int i = S.yrowof[p], j = S.ycol[p];
@@ -1687,8 +1794,8 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
     e->err = std::string("adjoint: ") + hipGetErrorString(r1 != hipSuccess ? r1 : r2);
     return BPF_ERR_HIP;
   }
-  if (gP) HIPCHK(hipMemcpy(gP, a.gP, sizeof(double) * T * n, hipMemcpyDeviceToHost));
-  if (gQ) HIPCHK(hipMemcpy(gQ, a.gQ, sizeof(double) * T * n, hipMemcpyDeviceToHost));
+  if (gP && (s = bus_rows_download(e, a.gP, gP, T))) return s;
+  if (gQ && (s = bus_rows_download(e, a.gQ, gQ, T))) return s;
   if (gx) HIPCHK(hipMemcpy(gx, a.gx, sizeof(double) * T * m, hipMemcpyDeviceToHost));
   if (obj) HIPCHK(hipMemcpy(obj, a.obj, sizeof(double) * T, hipMemcpyDeviceToHost));
   return BPF_OK;
@@ -1701,9 +1808,12 @@ static void fd_index(const bpf_engine *e, const int8_t *ty, int which,
   idx.assign(e->n, -1);
   int k = 0;
   if (which == 1)
-    for (int i = 0; i < e->n; i++) if (ty[i] == BPF_PV) idx[i] = k++;
-  for (int i = 0; i < e->n; i++)
+    for (int x = 0; x < e->n; x++)
+      if (ty[e->bus_int[x]] == BPF_PV) idx[e->bus_int[x]] = k++;
+  for (int x = 0; x < e->n; x++) {
+    const int i = e->bus_int[x];
     if (ty[i] == BPF_PQ && e->S.blk_of_bus[i] >= 0) idx[i] = k++;
+  }
 }
 
 // instances' bus types and state as a solve starts them (k_init), then
@@ -1713,9 +1823,13 @@ static bpf_status stage_start(bpf_engine *e, const double *V) {
                      e->g, e->d, (const int8_t *)e->d_type0,
                      (const double *)e->d_Qinj0);
   HIPCHK(hipGetLastError());
-  if (V)
-    HIPCHK(hipMemcpyAsync(e->d.V, V, sizeof(cd) * (size_t)e->T * e->n,
+  std::vector<cd> Vi;
+  if (V) {
+    Vi.resize((size_t)e->T * e->n);
+    bus_rows_in(e, (const cd *)V, Vi.data(), e->T);
+    HIPCHK(hipMemcpyAsync(e->d.V, Vi.data(), sizeof(cd) * Vi.size(),
                           hipMemcpyHostToDevice, e->stream));
+  }
   HIPCHK(hipStreamSynchronize(e->stream));
   e->solved = false;   // the state of the last solve is overwritten
   return BPF_OK;
@@ -1750,7 +1864,8 @@ extern "C" bpf_status bpf_stage_bmat(bpf_engine *e, int32_t t, int32_t which,
   fd_index(e, ty.data(), which, idx);
   const Symbolic &S = e->S;
   int cnt = 0;
-  for (int p = 0; p < e->g.nnzY; p++) {
+  for (int px = 0; px < e->g.nnzY; px++) {
+    const int p = e->yx_pos[px];
     const int q = S.y2lu[p];
     if (q < 0) continue;
     const int i = idx[S.yrowof[p]], j = idx[S.ycol[p]];
@@ -1801,7 +1916,7 @@ extern "C" bpf_status bpf_stage_fd_halfstep(bpf_engine *e, const double *V,
                      e->g, e->d, which);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(e->stream));
-  HIPCHK(hipMemcpy(Vout, e->d.V, sizeof(cd) * (size_t)T * e->n, hipMemcpyDeviceToHost));
+  if ((s = bus_rows_download(e, e->d.V, (cd *)Vout, T))) return s;
   if (norm) {
     std::vector<InstState> st(T);
     HIPCHK(hipMemcpy(st.data(), e->d.st, sizeof(InstState) * T, hipMemcpyDeviceToHost));
@@ -1815,7 +1930,8 @@ extern "C" bpf_status bpf_stage_tail_inverse(bpf_engine *e, int32_t t,
   if (!e || t < 0 || t >= e->T) return BPF_ERR_INVALID_ARG;
   const int s = e->S.tail_n;
   if (tail_bus)
-    for (int a = 0; a < s; a++) tail_bus[a] = e->S.bus_of_blk[e->S.tail_piv[a]];
+    for (int a = 0; a < s; a++)
+      tail_bus[a] = e->bus_ext[e->S.bus_of_blk[e->S.tail_piv[a]]];
   if (!Sinv || s == 0) return BPF_OK;
   if (!e->d_TL || e->lu_fp64 < 0) {
     e->err = "no factorization yet: call bpf_stage_newton_step or bpf_solve";

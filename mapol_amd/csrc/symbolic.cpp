@@ -273,6 +273,117 @@ std::vector<int> min_height_reorder(const std::vector<std::vector<int>> &adj,
   return out;
 }
 
+// the PV and PQ buses (ascending), each bus's place among them (-1 for the
+// others) and their graph: one edge per branch between two of them
+static void unknown_graph(int n, int m, const int *from, const int *to,
+                          const int *type, std::vector<int> &unk,
+                          std::vector<int> &loc,
+                          std::vector<std::vector<int>> &adj) {
+  unk.clear();
+  for (int i = 0; i < n; i++)
+    if (type[i] == 1 || type[i] == 2) unk.push_back(i);
+  loc.assign(n, -1);
+  for (int k = 0; k < (int)unk.size(); k++) loc[unk[k]] = k;
+  adj.assign(unk.size(), {});
+  for (int b = 0; b < m; b++) {
+    int a = loc[from[b]], c = loc[to[b]];
+    if (a >= 0 && c >= 0 && a != c) { adj[a].push_back(c); adj[c].push_back(a); }
+  }
+}
+
+// minimum degree (0), then optionally its minimum-height equivalent (1, same
+// fill), or nested dissection (2)
+static std::vector<int> order_graph(const std::vector<std::vector<int>> &adj,
+                                    int ordering) {
+  if (ordering == 2) return nested_dissection_order(adj, 64);
+  if (ordering == 1) return min_height_reorder(adj, min_degree_order(adj));
+  return min_degree_order(adj);
+}
+
+std::vector<int> elimination_order(int n, int m, const int *from, const int *to,
+                                   const int *type, int ordering) {
+  std::vector<int> unk, loc;
+  std::vector<std::vector<int>> adj;
+  unknown_graph(n, m, from, to, type, unk, loc, adj);
+  std::vector<int> ord = order_graph(adj, ordering);
+  for (int &k : ord) k = unk[k];
+  return ord;
+}
+
+std::vector<int> rcm_numbering(int n, int m, const int *from, const int *to) {
+  std::vector<std::vector<int>> adj(n);
+  for (int b = 0; b < m; b++)
+    if (from[b] != to[b]) {
+      adj[from[b]].push_back(to[b]);
+      adj[to[b]].push_back(from[b]);
+    }
+  for (auto &a : adj) {
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+  }
+  auto deg = [&](int i) { return (int)adj[i].size(); };
+  std::vector<int> mark(n, -1), order, lev;
+  order.reserve(n);
+  int stamp = 0;
+  // breadth-first levels from `r` (visits marked with a new stamp): the
+  // vertices in visit order, the start of the last level, the level count
+  auto levels = [&](int r, int &last0, int &depth) {
+    const int s = ++stamp;
+    lev.assign(1, r);
+    mark[r] = s;
+    last0 = 0;
+    depth = 0;
+    for (size_t h = 0; h < lev.size();) {
+      const size_t end = lev.size();
+      last0 = (int)h;
+      depth++;
+      for (; h < end; h++)
+        for (int j : adj[lev[h]])
+          if (mark[j] != s) { mark[j] = s; lev.push_back(j); }
+    }
+  };
+  std::vector<char> done(n, 0);
+  for (int r0 = 0; r0 < n; r0++) {
+    if (done[r0]) continue;
+    // pseudo-peripheral start (George and Liu): from a vertex of least
+    // degree, move to the least-degree vertex of the last level while the
+    // level count grows
+    int last0, depth;
+    levels(r0, last0, depth);
+    int start = r0;
+    for (int v : lev)
+      if (deg(v) < deg(start) || (deg(v) == deg(start) && v < start)) start = v;
+    levels(start, last0, depth);
+    for (;;) {
+      int cand = lev[last0];
+      for (size_t h = last0; h < lev.size(); h++)
+        if (deg(lev[h]) < deg(cand)) cand = lev[h];
+      int l2, d2;
+      const int keep = depth;
+      levels(cand, l2, d2);
+      if (d2 <= keep) break;
+      start = cand;
+      last0 = l2;
+      depth = d2;
+    }
+    // Cuthill-McKee: breadth first, neighbours by ascending degree
+    const size_t c0 = order.size();
+    order.push_back(start);
+    done[start] = 1;
+    std::vector<int> nb;
+    for (size_t h = c0; h < order.size(); h++) {
+      nb.clear();
+      for (int j : adj[order[h]])
+        if (!done[j]) { done[j] = 1; nb.push_back(j); }
+      std::stable_sort(nb.begin(), nb.end(),
+                       [&](int a, int b) { return deg(a) < deg(b); });
+      order.insert(order.end(), nb.begin(), nb.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
 static int find_pos(const std::vector<int> &rp, const std::vector<int> &ci,
                     int r, int c) {
   auto b = ci.begin() + rp[r], e = ci.begin() + rp[r + 1];
@@ -364,31 +475,31 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     }
   }
   // ---- blocks and ordering -------------------------------------------------
-  std::vector<int> unk;
-  for (int i = 0; i < n; i++)
-    if (type[i] == 1 || type[i] == 2) unk.push_back(i);
+  std::vector<int> unk, loc;
+  std::vector<std::vector<int>> adj;
+  unknown_graph(n, m, from, to, type, unk, loc, adj);
   const int nb = (int)unk.size();
   S.nb = nb;
   if (nb == 0) {
     if (err) *err = "no PV or PQ bus";
     return false;
   }
-  std::vector<int> loc(n, -1);
-  for (int k = 0; k < nb; k++) loc[unk[k]] = k;
-  std::vector<std::vector<int>> adj(nb);
-  for (int b = 0; b < m; b++) {
-    int a = loc[from[b]], c = loc[to[b]];
-    if (a >= 0 && c >= 0 && a != c) { adj[a].push_back(c); adj[c].push_back(a); }
-  }
-  // ordering: minimum degree (0), then optionally its minimum-height
-  // equivalent (1, same fill), or nested dissection (2)
   std::vector<int> order;
-  if (S.prm.ordering == 2)
-    order = nested_dissection_order(adj, 64);
-  else if (S.prm.ordering == 1)
-    order = min_height_reorder(adj, min_degree_order(adj));
-  else
-    order = min_degree_order(adj);
+  if (!prm.order_bus.empty()) {
+    std::vector<char> seen(nb, 0);
+    bool ok = (int)prm.order_bus.size() == nb;
+    for (size_t k = 0; ok && k < prm.order_bus.size(); k++) {
+      const int b = prm.order_bus[k], a = (b >= 0 && b < n) ? loc[b] : -1;
+      ok = a >= 0 && !seen[a];
+      if (ok) { seen[a] = 1; order.push_back(a); }
+    }
+    if (!ok) {
+      if (err) *err = "elimination order is not a permutation of the PV and PQ buses";
+      return false;
+    }
+  } else {
+    order = order_graph(adj, S.prm.ordering);
+  }
   S.bus_of_blk.resize(nb);
   S.blk_of_bus.assign(n, -1);
   std::vector<int> newof(nb);

@@ -35,6 +35,9 @@ struct AnalysisParams {
   // bus ordering: 0 minimum degree, 1 minimum degree reordered to the least
   // elimination-tree height with the same fill, 2 nested dissection
   int ordering = 0;
+  // an elimination order given by the caller (every PV and PQ bus once,
+  // first eliminated first) in place of the one `ordering` selects
+  std::vector<int> order_bus;
 };
 
 struct Symbolic {
@@ -138,15 +141,25 @@ bool analyse(int n, int m, const int *from, const int *to,
              const int *bus_type, Symbolic &S, std::string *err,
              const AnalysisParams &prm = AnalysisParams());
 
+// the elimination order `analyse` derives for `ordering` from the case as
+// numbered here: the PV and PQ buses, first eliminated first
+std::vector<int> elimination_order(int n, int m, const int *from, const int *to,
+                                   const int *bus_type, int ordering);
+
+// reverse Cuthill-McKee numbering of the bus graph (every branch, in service
+// or not): entry k is the bus that takes number k.  Neighbouring buses get
+// near numbers, so gathers over a bus's branches touch few cache lines.
+std::vector<int> rcm_numbering(int n, int m, const int *from, const int *to);
+
 // minimum-degree order of a symmetric graph given as adjacency lists
 std::vector<int> min_degree_order(std::vector<std::vector<int>> adj);
 
-// nested-dissection order (separators last; parts of at most `leaf`
-// vertices by minimum degree)
 // the same filled graph as `order`, with the least elimination-tree height
 std::vector<int> min_height_reorder(const std::vector<std::vector<int>> &adj,
                                     const std::vector<int> &order);
 
+// nested-dissection order (separators last; parts of at most `leaf`
+// vertices by minimum degree)
 std::vector<int> nested_dissection_order(const std::vector<std::vector<int>> &adj,
                                          int leaf);
 

@@ -58,7 +58,8 @@ class BpfOptions(C.Structure):
                 ("max_qlim_rounds", C.c_int32), ("use_graph", C.c_int32),
                 ("tol", C.c_double), ("qlim_tol", C.c_double),
                 ("tail_limit", C.c_int32), ("chunk_products", C.c_int32),
-                ("slot_chunk", C.c_int32), ("ordering", C.c_int32)]
+                ("slot_chunk", C.c_int32), ("ordering", C.c_int32),
+                ("bus_numbering", C.c_int32)]
 
 
 class BpfCtl(C.Structure):
@@ -321,7 +322,7 @@ class Engine:
                  max_qlim_rounds: int = 10, qlim_tol: float = 1e-6,
                  use_graph: bool = True, tail_limit: int = 0,
                  chunk_products: int = 0, slot_chunk: int = 0,
-                 ordering: int = 0):
+                 ordering: int = 0, bus_numbering: int = 1):
         self.L = load_library()
         self.case = case
         self._keep = []
@@ -352,6 +353,7 @@ class Engine:
         o.qlim_tol, o.use_graph = qlim_tol, int(use_graph)
         o.tail_limit, o.chunk_products = int(tail_limit), int(chunk_products)
         o.slot_chunk, o.ordering = int(slot_chunk), int(ordering)
+        o.bus_numbering = int(bus_numbering)
         self.options = o
         h = C.c_void_p()
         st = self.L.bpf_create(C.byref(s), C.byref(o), C.byref(h))

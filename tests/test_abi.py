@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 from mapol_amd import engine
@@ -125,3 +126,25 @@ def test_tile_options_are_validated():
                 dict(slot_chunk=1000), dict(ordering=3)):
         with pytest.raises(Exception, match="out of range|tile"):
             Engine(case, **bad)
+
+
+@pytest.mark.skipif(not os.path.exists(engine.LIB_PATH),
+                    reason="libbatchpf.so not built")
+def test_case_and_numbering_options_are_validated():
+    """bpf_create refuses an unknown bus numbering and a branch or generator
+    naming a bus outside the case, before it renumbers or touches a device"""
+    from mapol_amd import synth
+    from mapol_amd.engine import Engine
+    case, _ = synth.workload("pf118x200", T=2)
+    for b in (-1, 2):
+        with pytest.raises(Exception, match="invalid options"):
+            Engine(case, bus_numbering=b)
+    bad = synth.Case(**{k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
+                        for k, v in case.__dict__.items()})
+    bad.br_to[5] = case.nbus
+    with pytest.raises(Exception, match="branch 5 has a bus out of range"):
+        Engine(bad)
+    bad.br_to[5] = case.br_to[5]
+    bad.gen_bus[0] = -1
+    with pytest.raises(Exception, match="generator bus out of range"):
+        Engine(bad)

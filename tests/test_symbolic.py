@@ -609,3 +609,48 @@ def test_slot_form_splits_long_rows():
     ref = sla.spsolve(J.tocsc(), F)
     got = np.array([x[2 * b + c] for b, c in rowmap])
     assert np.abs(got - ref).max() <= 1e-9 * max(1, np.abs(ref).max())
+
+
+def _numbering_check(tmp_path_factory):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path_factory.mktemp("numbering") / "numbering_check")
+    subprocess.run(["g++", "-std=c++17", "-O2",
+                    os.path.join(root, "tests", "cpp", "numbering_check.cpp"),
+                    os.path.join(root, "mapol_amd", "csrc", "symbolic.cpp"),
+                    "-o", exe], check=True, capture_output=True)
+    return exe
+
+
+def _run_check(exe, n, m, types, f, t, ordering):
+    import subprocess
+    txt = f"{n} {m} {ordering}\n" + " ".join(map(str, types)) + "\n" + \
+        "\n".join(f"{a} {b}" for a, b in zip(f, t)) + "\n"
+    return subprocess.run([exe], input=txt, capture_output=True, text=True)
+
+
+def test_engine_bus_numbering_keeps_the_analysis(tmp_path_factory):
+    """the engine's own bus numbers (reverse Cuthill-McKee) with the
+    elimination order taken from the caller's numbering give the caller's
+    block pattern, schedules and dense tail, array for array
+    (tests/cpp/numbering_check.cpp), and bring branch ends closer"""
+    exe = _numbering_check(tmp_path_factory)
+    cases = [("pf118x200", (0, 1, 2)), ("pf1354x200", (0, 1, 2)),
+             ("pf9241x200", (0,))]
+    for name, orders in cases:
+        case, _ = synth.workload(name, T=2)
+        for o in orders:
+            r = _run_check(exe, case.nbus, case.nbranch, case.bus_type,
+                           case.br_from, case.br_to, o)
+            assert r.returncode == 0 and r.stdout.startswith("identical"), \
+                (name, o, r.stdout, r.stderr)
+            before, after = map(float, r.stdout.split()[-3::2])
+            assert after < before / 4, (name, r.stdout)
+    # several islands, an isolated bus, parallel branches, a PV-only island
+    types = [3, 1, 2, 1, 1, 1, 1, 2, 2]
+    f = [0, 1, 1, 2, 4, 7, 3]
+    t = [1, 2, 2, 3, 5, 8, 0]
+    for o in (0, 1, 2):
+        r = _run_check(exe, len(types), len(f), types, f, t, o)
+        assert r.returncode == 0 and r.stdout.startswith("identical"), (o, r.stdout)
