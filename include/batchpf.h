@@ -435,6 +435,18 @@ bpf_status bpf_set_schedule(bpf_engine *e, int32_t newton_per_graph,
                             int32_t tail_inverse, int32_t sweep_form,
                             int32_t parts);
 
+/* Split factorization (no counterpart in the reference, whose CPU solver
+ * factors one instance per thread): while few instances are active, the
+ * Newton step's factorization runs the subtrees below a cut of the
+ * elimination tree with 2, 4 or 8 workgroups per instance (subtrees share
+ * no operands), then the levels above the cut and the dense tail with one.
+ * workgroups = 0: adaptive, the default (the most workgroups per instance
+ * with active instances x workgroups within the compute units of the
+ * batch's part); 1: never; 2, 4, 8: always that many where the analysis
+ * planned them.  Results do not depend on it: every output is summed in
+ * the same order. */
+bpf_status bpf_set_factor_split(bpf_engine *e, int32_t workgroups);
+
 #ifdef __cplusplus
 }
 #endif

@@ -141,6 +141,14 @@ struct bpf_engine {
   int sweep_form = 0;                // sweep_mode's form (0 slot, 1 entry, 2 rows)
   int tail_form = 1;                 // BPF_TAIL_MFMA / BPF_TAIL_VALU
   int per_graph = 4;                 // Newton iterations per captured graph
+  // split factorization (several workgroups per instance while few
+  // instances are active): the plan has a split variant; split_mode 0
+  // adaptive, 1 off, 2 / 4 / 8 that many workgroups (bpf_set_factor_split);
+  // the compute units and the parts running side by side set the budget
+  bool fplan_split = false;
+  int split_mode = 0;
+  int num_cu = 256;
+  int parts_running = 1;
   bool capturing = false;            // recording the iteration graph
   std::map<std::string, std::pair<int, double>> ktime;
   bool solved = false;
@@ -411,6 +419,8 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
     HIPCHK(hipStreamCreateWithFlags(&e->sub[h], hipStreamNonBlocking));
   HIPCHK(hipDeviceGetAttribute(&e->max_lds,
                                hipDeviceAttributeMaxSharedMemoryPerBlock, e->dev));
+  HIPCHK(hipDeviceGetAttribute(&e->num_cu, hipDeviceAttributeMultiprocessorCount,
+                               e->dev));
   HIPCHK(hipHostMalloc((void **)&e->h_count, sizeof(int) * bpf_engine::MAX_SPLIT,
                        hipHostMallocDefault));
   const Symbolic &S = e->S;
@@ -490,28 +500,73 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   }
   FactorPlan &F = e->fplan;
   F.nchunk = S.nchunk;
-  UP(F.ch_task, S.ch_task); UP(F.ch_pair, S.ch_pair); UP(F.ch_max, S.ch_max);
   {
-    // one descriptor per output: the pivot range of an L output packed into
-    // the fourth word (first product | length << TK_LEN_SHIFT)
-    if (S.npairs >= (int64_t)1 << TK_LEN_SHIFT) {
-      e->err = "factorization schedule has " + std::to_string(S.npairs) +
-               " products, more than the 2^19 its descriptors address";
+    // every chunk as {first task, end task, first product, end product} and
+    // the task count and longest reduction that set its lane groups: the
+    // main chunks, then the split schedules' (their tasks and products
+    // after the main ones)
+    const int nt0 = (int)S.tk_a.size() / 4, np0 = (int)S.pairs.size() / 2;
+    const int nsp = (int)S.sp_lg.size() / 2, nall = S.nchunk + nsp;
+    std::vector<int> ch(4 * nall), lg(2 * nall);
+    std::vector<int> tk(S.tk_a.size() + S.sp_tk_a.size());
+    // one descriptor per output; an L output's pivot range packed as its
+    // offset from the chunk's first product | length << TK_LEN_SHIFT
+    auto pack = [&](const std::vector<int> &ta, const std::vector<int> &tb, int k0,
+                    int k1, int q0, int toff, int poff) {
+      for (int k = k0; k < k1; k++) {
+        int *d = &tk[4 * (size_t)(toff + k)];
+        d[0] = ta[4 * k];
+        d[1] = ta[4 * k + 1] + poff;
+        d[2] = ta[4 * k + 2] + poff;
+        d[3] = 0;
+        if (ta[4 * k] & TASK_LCOL) {
+          const int v0 = tb[4 * k + 1] - q0, len = tb[4 * k + 2] - tb[4 * k + 1];
+          if (v0 < 0 || v0 >= (1 << TK_LEN_SHIFT) || len < 0 || len >= (1 << TK_LEN_SHIFT))
+            return false;
+          d[3] = v0 | (len << TK_LEN_SHIFT);
+        }
+      }
+      return true;
+    };
+    bool ok = true;
+    for (int c = 0; c < S.nchunk; c++) {
+      const int r[4] = {S.ch_task[c], S.ch_task[c + 1], S.ch_pair[c], S.ch_pair[c + 1]};
+      std::copy(r, r + 4, &ch[4 * c]);
+      lg[2 * c] = r[1] - r[0];
+      lg[2 * c + 1] = S.ch_max[c];
+      ok = ok && pack(S.tk_a, S.tk_b, r[0], r[1], r[2], 0, 0);
+    }
+    for (int c = 0; c < nsp; c++) {
+      const int *r = &S.sp_ch[4 * c];
+      int *d = &ch[4 * (S.nchunk + c)];
+      d[0] = r[0] + nt0; d[1] = r[1] + nt0; d[2] = r[2] + np0; d[3] = r[3] + np0;
+      lg[2 * (S.nchunk + c)] = S.sp_lg[2 * c];
+      lg[2 * (S.nchunk + c) + 1] = S.sp_lg[2 * c + 1];
+      ok = ok && pack(S.sp_tk_a, S.sp_tk_b, r[0], r[1], r[2], nt0, np0);
+    }
+    if (!ok) {
+      e->err = "internal: a pivot's product range outside its factorization chunk";
       return BPF_ERR_UNSUPPORTED;
     }
-    std::vector<int> tk(S.tk_a.size());
-    for (size_t k = 0; k + 3 < S.tk_a.size(); k += 4) {
-      tk[k] = S.tk_a[k];
-      tk[k + 1] = S.tk_a[k + 1];
-      tk[k + 2] = S.tk_a[k + 2];
-      const unsigned v0 = (unsigned)S.tk_b[k + 1], len = (unsigned)(S.tk_b[k + 2] - S.tk_b[k + 1]);
-      tk[k + 3] = (int)(v0 | (len << TK_LEN_SHIFT));
-    }
-    const int *tp, *pr;
+    std::vector<int> pairs(S.pairs);
+    pairs.insert(pairs.end(), S.sp_pairs.begin(), S.sp_pairs.end());
+    const int *cp, *lp, *tp, *pr;
+    UP(cp, nonempty(ch, 4));
+    UP(lp, nonempty(lg, 2));
     UP(tp, nonempty(tk, 4));
-    UP(pr, nonempty(S.pairs, 2));
+    UP(pr, nonempty(pairs, 2));
+    F.chunks = (const int4 *)cp;
+    F.chunk_lg = (const int2 *)lp;
     F.tk = (const int4 *)tp;
     F.pairs = (const int2 *)pr;
+    e->fplan_split = false;
+    for (int v = 0; v < SPLIT_VARIANTS; v++) {
+      const bool has = S.sp_level[v] >= 0;
+      F.sp_rest[v] = has ? S.sp_rest[v] : -1;
+      for (int j = 0; j <= SPLIT_MAX; j++)
+        F.sp_grp[v][j] = S.nchunk + S.sp_grp[v * (SPLIT_MAX + 1) + j];
+      e->fplan_split = e->fplan_split || has;
+    }
   }
   F.tn = S.tail_n;
   F.tnent = (int)S.tl_pos.size();
@@ -572,6 +627,18 @@ extern "C" bpf_status bpf_set_launch(bpf_engine *e, int32_t lu, int32_t sv,
   if (lu) e->lu_threads = lu;
   if (sv) e->solve_threads = sv;
   if (mis) e->mis_threads = mis;
+  drop_graphs(e);
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_set_factor_split(bpf_engine *e, int32_t workgroups) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  if (workgroups != 0 && workgroups != 1 && workgroups != 2 && workgroups != 4 &&
+      workgroups != 8) {
+    e->err = "factorization split: 0 (adaptive), 1, 2, 4 or 8 workgroups";
+    return BPF_ERR_INVALID_ARG;
+  }
+  e->split_mode = workgroups;
   drop_graphs(e);
   return BPF_OK;
 }
@@ -872,9 +939,26 @@ static void launch_newton_step(bpf_engine *e) {
                      dim3(256), 0, e->stream, e->g, e->d, LU, c0 > 0 ? 1 : 0);
   tend(e);
   tbegin(e, "lu_factor");
-  hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, 1, e->fplan, LU,
-                     (B4<T> *)e->d_TL, c0, e->d.jsing);
+  if (e->split_mode != 1 && e->fplan_split) {
+    // split launch pair: the subtrees below the cut with several workgroups
+    // per instance when few instances are active, then the top; the
+    // workgroup count follows the batch's active-instance counter (the
+    // compute units shared among the parts that run side by side)
+    const int force = e->split_mode == 0 ? -2 : (e->split_mode == 2 ? 0 : e->split_mode == 4 ? 1 : 2);
+    const int budget = std::max(1, e->num_cu / std::max(1, e->parts_running));
+    hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T * SPLIT_MAX), dim3(e->lu_threads), 0,
+                       e->stream, e->g, e->d.st, 1, e->fplan, LU,
+                       (B4<T> *)e->d_TL, c0, e->d.jsing, 1,
+                       (const int *)e->d.active_count, budget, force);
+    hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,
+                       e->stream, e->g, e->d.st, 1, e->fplan, LU,
+                       (B4<T> *)e->d_TL, c0, e->d.jsing, 2,
+                       (const int *)e->d.active_count, budget, force);
+  } else {
+    hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,
+                       e->stream, e->g, e->d.st, 1, e->fplan, LU,
+                       (B4<T> *)e->d_TL, c0, e->d.jsing, 0, nullptr, 0, -1);
+  }
   tend(e);
   size_t xs = sizeof(T) * 2 * (size_t)e->S.nb;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
@@ -977,6 +1061,7 @@ static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
   bool use_graph = e->opt.use_graph != 0 && !e->graph_broken && !e->timing;
   int K = use_graph ? std::max(1, std::min(e->nsplit, e->T)) : 1;
   if (use_graph && e->graph && e->split_used != K) drop_graphs(e);
+  e->parts_running = K;
   hipGraphExec_t *slot[bpf_engine::MAX_SPLIT] = {&e->graph};
   for (int h = 1; h < bpf_engine::MAX_SPLIT; h++) slot[h] = &e->subgraph[h];
   PartView view[bpf_engine::MAX_SPLIT];
@@ -1006,6 +1091,7 @@ static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
       // direct launches of the whole batch from here on
       drop_graphs(e);
       K = 1;
+      e->parts_running = 1;
       view[0] = part_view(e, 0, 1);
     }
     e->split_used = K;
@@ -1093,7 +1179,7 @@ static bpf_status run_fd(bpf_engine *e) {
   tend(e);
   tbegin(e, "lu_factor");
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(T), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing);
+                     e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing, 0, nullptr, 0, -1);
   tend(e);
   size_t xs = sizeof(double) * 2 * (size_t)e->S.nb;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
@@ -1757,7 +1843,7 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, ((size_t)e->g.nJb + 1) / 2, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU, 0);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing);
+                     e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing, 0, nullptr, 0, -1);
   hipLaunchKernelGGL(k_adj_rhs, dim3(div_up(e->S.nb, 256), TT), dim3(256), 0,
                      e->stream, e->g, e->d, d_rate, w, a.G);
   size_t xs = sizeof(double) * nb2;
@@ -1904,7 +1990,7 @@ extern "C" bpf_status bpf_stage_fd_halfstep(bpf_engine *e, const double *V,
   hipLaunchKernelGGL(k_bmat, dim3(div_up(e->g.nnzY, 256), T), dim3(256), 0,
                      e->stream, e->g, e->d, e->opt.alg, LU);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(T), dim3(e->lu_threads), 0,
-                     e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing);
+                     e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing, 0, nullptr, 0, -1);
   mismatch_pair(e, 2, e->opt.maxit_fd, 0, 1);
   const size_t xs = sizeof(double) * 2 * (size_t)e->S.nb;
   const int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;

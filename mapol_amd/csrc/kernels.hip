@@ -470,20 +470,24 @@ __device__ __forceinline__ B4<T> blk_sel(bool c, const B4<T> &x, const B4<T> &y)
   r.d = c ? x.d : y.d;
   return r;
 }
+// Block products as explicit fused multiply-adds: left to contraction, an
+// expression such as a*b + c*d may fuse either product, and each inlined
+// copy chooses for itself, so the same product formed in two places (two
+// schedules of one factorization) could round differently.
 template <typename T>
 __device__ __forceinline__ void blk_fms(B4<T> &acc, const B4<T> &l, const B4<T> &u) {
-  acc.a -= l.a * u.a + l.b * u.c;
-  acc.b -= l.a * u.b + l.b * u.d;
-  acc.c -= l.c * u.a + l.d * u.c;
-  acc.d -= l.c * u.b + l.d * u.d;
+  acc.a = fma(-l.b, u.c, fma(-l.a, u.a, acc.a));
+  acc.b = fma(-l.b, u.d, fma(-l.a, u.b, acc.b));
+  acc.c = fma(-l.d, u.c, fma(-l.c, u.a, acc.c));
+  acc.d = fma(-l.d, u.d, fma(-l.c, u.b, acc.d));
 }
 template <typename T>
 __device__ __forceinline__ B4<T> blk_mul(const B4<T> &x, const B4<T> &y) {
   B4<T> r;
-  r.a = x.a * y.a + x.b * y.c;
-  r.b = x.a * y.b + x.b * y.d;
-  r.c = x.c * y.a + x.d * y.c;
-  r.d = x.c * y.b + x.d * y.d;
+  r.a = fma(x.b, y.c, x.a * y.a);
+  r.b = fma(x.b, y.d, x.a * y.b);
+  r.c = fma(x.d, y.c, x.c * y.a);
+  r.d = fma(x.d, y.d, x.c * y.b);
   return r;
 }
 
@@ -496,12 +500,18 @@ __device__ __forceinline__ B4<T> blk_mul(const B4<T> &x, const B4<T> &y) {
 // ---------------------------------------------------------------------------
 // schedules of the factorization and of the sweeps (device arrays from the
 // symbolic analysis)
-constexpr int TK_LEN_SHIFT = 19;       // products per schedule < 2^19
+constexpr int TK_LEN_SHIFT = 16;       // pivot range: offset in the chunk | length << 16
 struct FactorPlan {
-  int nchunk;                          // chunks of the levels below the tail
-  const int *ch_task, *ch_pair, *ch_max;
+  int nchunk;                          // main chunks (the levels below the tail)
+  // every chunk, the main ones first, then those of the split schedules:
+  // {first task, end task, first product, end product}, and the task count
+  // and longest reduction that set its lane groups (a split chunk keeps
+  // those of its main chunk, so each output is summed in the same order)
+  const int4 *chunks;
+  const int2 *chunk_lg;
   // per output {dst | flags, first product, end product, pivot range}; the
-  // pivot range of an L output packed as first | length << TK_LEN_SHIFT
+  // pivot range of an L output packed as its first product's offset from
+  // the chunk's first product | length << TK_LEN_SHIFT
   const int4 *tk;
   const int2 *pairs;                   // {L, U} positions (U < 0: init entry)
   int tn, tnent, tmax;                 // dense tail: pivots, entries, longest list
@@ -509,7 +519,25 @@ struct FactorPlan {
   int tstage_n;                        // staged tail blocks (0: no staging)
   const int *tstage, *tpairS;          // their positions; pair -> L | U << 16
   int tail_mfma;                       // FP32 tail inverse on matrix cores
+  // split factorization: variant v (2 << v workgroups per instance) leaves
+  // the main chunks from sp_rest[v] to the top workgroup (-1: no such
+  // variant); its group g runs chunks sp_grp[v][g] .. sp_grp[v][g + 1] - 1
+  int sp_rest[SPLIT_VARIANTS];
+  int sp_grp[SPLIT_VARIANTS][SPLIT_MAX + 1];
 };
+
+// the split variant for the active instances of this batch (part): the
+// most workgroups per instance with active x workgroups within budget
+// (the compute units the batch may fill), -1 for one workgroup; a forced
+// choice (force >= -1) instead when the plan has it
+__device__ __forceinline__ int split_variant(const FactorPlan &P, const int *active,
+                                             int budget, int force) {
+  if (force >= -1) return (force >= 0 && P.sp_rest[force] >= 0) ? force : -1;
+  const int a = max(*active, 1);
+  for (int v = SPLIT_VARIANTS - 1; v >= 0; v--)
+    if (P.sp_rest[v] >= 0 && a * (2 << v) <= budget) return v;
+  return -1;
+}
 
 struct SolvePlan {
   int nfw, nbw;                        // levels over the non-tail rows
@@ -701,7 +729,7 @@ __device__ __forceinline__ B4<T> blk_zero() {
 
 template <typename T>
 __device__ __forceinline__ bool blk_inv(const B4<T> &a, B4<T> &inv) {
-  T det = a.a * a.d - a.b * a.c;
+  T det = fma(a.a, a.d, -(a.b * a.c));
   bool bad = !(det != (T)0) || !isfinite((double)det);
   T r = (T)1 / det;
   inv.a = a.d * r; inv.b = -a.b * r; inv.c = -a.c * r; inv.d = a.a * r;
@@ -975,10 +1003,37 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
 template <typename T>
 __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
                             FactorPlan P, B4<T> *LU, B4<T> *TL, int c_start,
-                            int *jsing) {
+                            int *jsing, int mode, const int *active, int budget,
+                            int force) {
   static_assert(TAIL_MAX * TAIL_MAX == CHUNK_PAIRS,
                 "the chunk products and the dense tail share one LDS buffer");
-  const int t = blockIdx.x;
+  // mode 0: one workgroup per instance runs the main chunks from c_start and
+  // the dense tail.  Split (two launches): mode 1 has SPLIT_MAX workgroups
+  // per instance (workgroup b: instance b mod T, group b / T); with a split
+  // variant for the active count, groups below its K run their subtrees'
+  // chunks and flag a singular pivot in jsing; without one, group 0 does
+  // what mode 0 does.  Mode 2 (one workgroup per instance) then runs the
+  // chunks above the cut and the tail when mode 1 split.
+  int t = blockIdx.x, cb = c_start, ce = P.nchunk;
+  bool top = true;                      // runs the tail, reports singularity
+  if (mode != 0) {
+    const int v = split_variant(P, active, budget, force);
+    if (mode == 1) {
+      const int nt = (int)gridDim.x / SPLIT_MAX, gi = (int)blockIdx.x / nt;
+      t = (int)blockIdx.x - gi * nt;
+      if (v >= 0) {
+        if (gi >= (2 << v)) return;
+        cb = P.sp_grp[v][gi];
+        ce = P.sp_grp[v][gi + 1];
+        top = false;
+      } else if (gi != 0) {
+        return;
+      }
+    } else {
+      if (v < 0) return;
+      cb = P.sp_rest[v];
+    }
+  }
   if (st[t].done) return;
   __shared__ B4<T> buf[CHUNK_PAIRS];
   // chunk tables in LDS (scalar loads inside a phase would delay its LDS
@@ -986,7 +1041,7 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   // holds the staged blocks of the dense tail's products
   constexpr int STAGE = sizeof(T) == 4 ? TAIL_STAGE_MAX : 1;
   union TabStage {
-    struct { int ct[MAX_CHUNKS + 1], cp[MAX_CHUNKS + 1], cm[MAX_CHUNKS]; } tab;
+    struct { int4 ch[MAX_CHUNKS]; int2 lg[MAX_CHUNKS]; } tab;
     B4<T> stage[STAGE];
     // blocked Gauss-Jordan of the dense tail (after the stage is read): per
     // step parity the next pivot pair's columns, rows and pair inverse
@@ -998,24 +1053,26 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     TailQuads mq;
   };
   __shared__ TabStage sh;
-  int *const s_ct = sh.tab.ct, *const s_cp = sh.tab.cp, *const s_cm = sh.tab.cm;
+  int4 *const s_ch = sh.tab.ch;
+  int2 *const s_lg = sh.tab.lg;
+  const int nc = ce - cb;               // this workgroup's chunks
   // chunks below c_start (level 0) were finished by the Jacobian fill, which
-  // flags a singular pivot in jsing[t] (taken over and cleared here)
+  // flags a singular pivot in jsing[t], as do the groups of a split (taken
+  // over and cleared by the workgroup that reports)
   __shared__ int sing;
   // read by the reductions for a range that has ended
   __shared__ B4<T> zblk;
   if (threadIdx.x == 0) {
     sing = 0;
     zblk = blk_zero<T>();
-    if (c_start > 0) {
+    if (top && (c_start > 0 || mode == 2)) {
       sing = jsing[t];
       jsing[t] = 0;
     }
   }
-  for (int q = threadIdx.x; q <= P.nchunk; q += blockDim.x) {
-    s_ct[q] = P.ch_task[q];
-    s_cp[q] = P.ch_pair[q];
-    if (q < P.nchunk) s_cm[q] = P.ch_max[q];
+  for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+    s_ch[q] = P.chunks[cb + q];
+    s_lg[q] = P.chunk_lg[cb + q];
   }
   B4<T> *A = LU + (size_t)t * g.nnzLU;
   __syncthreads();
@@ -1031,8 +1088,8 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   int2 pidx[PF];
   int nq0 = 0, nq1 = 0;
   auto prefetch_pairs = [&](int c) {
-    nq0 = s_cp[c];
-    nq1 = s_cp[c + 1];
+    nq0 = s_ch[c].z;
+    nq1 = s_ch[c].w;
     const int qlast = max(nq1 - 1, 0);
 #pragma unroll
     for (int j = 0; j < PF; j++) {
@@ -1040,7 +1097,7 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       if (wave_any(q, nq1)) pidx[j] = P.pairs[min(q, qlast)];
     }
   };
-  if (P.nchunk > c_start) prefetch_pairs(c_start);
+  if (nc > 0) prefetch_pairs(0);
   // Loads of a slot no lane of the wave needs are skipped by a scalar branch
   // (wave_any), so the number of loads in flight is not known to the
   // compiler and any wait for one of them waits for all (global loads
@@ -1050,11 +1107,11 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
   // the operands and descriptors; the next chunk's pair indices are then
   // requested and stay in flight through the reduction, which reads only
   // LDS and registers.
-  for (int c = c_start; c < P.nchunk; c++) {
+  for (int c = 0; c < nc; c++) {
     __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
     const int q0 = nq0, q1 = nq1;
-    const int k0 = s_ct[c], k1 = s_ct[c + 1];
-    const int gsz = __builtin_amdgcn_readfirstlane(lane_group(k1 - k0, s_cm[c]));
+    const int k0 = s_ch[c].x, k1 = s_ch[c].y;
+    const int gsz = __builtin_amdgcn_readfirstlane(lane_group(s_lg[c].x, s_lg[c].y));
     const int lane = threadIdx.x & (gsz - 1);
     const int stride = (int)blockDim.x / gsz;
     const int kf = k0 + (int)threadIdx.x / gsz;
@@ -1112,7 +1169,7 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       buf[q - q0] = v;
     }
     __syncthreads();
-    if (c + 1 < P.nchunk) prefetch_pairs(c + 1);
+    if (c + 1 < nc) prefetch_pairs(c + 1);
     // outputs in batches of NB per thread; every output's Jacobian value is
     // an init entry of its own product range.  The first batch uses the
     // descriptors received with the operands (its own code path, so that no
@@ -1128,18 +1185,18 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
         // an L output also sums its pivot's range (empty for the others);
         // in FP32 both ranges are read together (FP64 has not the registers)
         const unsigned pw = (unsigned)ta[u].w;
-        const int v0 = (int)(pw & ((1u << TK_LEN_SHIFT) - 1));
+        const int v0 = (int)(pw & ((1u << TK_LEN_SHIFT) - 1));   // in the chunk
         const int v1 = lcol ? v0 + (int)(pw >> TK_LEN_SHIFT) : v0;
         B4<T> e, pv;
         if constexpr (sizeof(T) == 4) {
           lane_range_sum2(buf, &zblk, ta[u].y + lane - q0, ta[u].z - q0,
-                          v0 + lane - q0, v1 - q0, gsz, e, pv);
+                          v0 + lane, v1, gsz, e, pv);
           group_sum4x2_first(e, pv, gsz);
         } else {
           e = lane_range_sum(buf, ta[u].y + lane - q0, ta[u].z - q0, gsz);
           group_sum4_first(e, gsz);
           if (lcol) {
-            pv = lane_range_sum(buf, v0 + lane - q0, v1 - q0, gsz);
+            pv = lane_range_sum(buf, v0 + lane, v1, gsz);
             group_sum4_first(pv, gsz);
           }
         }
@@ -1170,6 +1227,10 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
       reduce_batch(kb);
     }
     __syncthreads();
+  }
+  if (!top) {
+    if (threadIdx.x == 0 && sing) jsing[t] = 1;
+    return;
   }
   if (P.tn > 0) {
     B4<T> *D = buf;
@@ -2623,8 +2684,8 @@ __global__ void k_branch_report(DevTopo g, int T, const cd *Sf, const cd *St,
 // explicit instantiations used by the engine
 template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *, int);
 template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *, int);
-template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *);
-template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *, int, int *);
+template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *, int, const int *, int, int);
+template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *, int, int *, int, const int *, int, int);
 template __global__ void k_lu_solve<float, 0>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
 template __global__ void k_lu_solve<float, 2>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
 template __global__ void k_lu_solve<float, 4>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);

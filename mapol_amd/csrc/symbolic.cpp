@@ -1,6 +1,7 @@
 // Symbolic analysis: Ybus pattern, bus ordering, L+U block pattern and the
 // level schedules of the batched factorization and triangular solves.
 #include <algorithm>
+#include <array>
 #include <cstdlib>
 #include <functional>
 #include <map>
@@ -608,14 +609,68 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   // output also reduces its pivot's products (same chunk) and scales by the
   // pivot inverse, so no separate scaling pass is needed.
   struct Task { int dst; std::vector<int> L, U; };
+  std::vector<std::vector<Task>> pgroup(nb);   // each pivot's outputs
+  auto gpairs = [](const std::vector<Task> &g) {
+    size_t n = 0;
+    for (auto &t : g) n += t.L.size();
+    return n;
+  };
+  // one chunk from the groups of the pivots pks: the tasks heaviest first
+  // (an L output also reduces its pivot's products; ties keep the order of
+  // pks), then their products in task order.  Appends to the three lists;
+  // returns {tasks, longest reduction}.
+  auto emit_chunk = [&](const std::vector<int> &pks, std::vector<int> &tk_a,
+                        std::vector<int> &tk_b, std::vector<int> &prs) {
+    std::vector<std::pair<int, int>> ord;  // (pivot, index in its group)
+    for (int k : pks)
+      for (int q = 0; q < (int)pgroup[k].size(); q++) {
+        const Task &t = pgroup[k][q];
+        // a U block without products keeps its Jacobian value: nothing to do
+        if (t.L.empty() && (t.dst & TASK_HAS_A) && !(t.dst & (TASK_DIAG | TASK_LCOL)))
+          continue;
+        ord.push_back({k, q});
+      }
+    auto work = [&](const std::pair<int, int> &x) {
+      const Task &t = pgroup[x.first][x.second];
+      size_t w = t.L.size();
+      if (t.dst & TASK_LCOL) w = std::max(w, pgroup[x.first][0].L.size());
+      return w;
+    };
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                       return work(x) > work(y);
+                     });
+    // pair offsets of every task (in chunk order)
+    std::map<std::pair<int, int>, int> start;
+    int off = (int)prs.size() / 2;
+    for (auto &x : ord) { start[x] = off; off += (int)pgroup[x.first][x.second].L.size(); }
+    int wmax = 0;
+    for (auto &x : ord) {
+      const Task &t = pgroup[x.first][x.second];
+      const int p0 = start[x], p1 = p0 + (int)t.L.size();
+      tk_a.insert(tk_a.end(), {t.dst, p0, p1, 0});
+      if (t.dst & TASK_LCOL) {
+        const int v0 = start[{x.first, 0}], v1 = v0 + (int)pgroup[x.first][0].L.size();
+        tk_b.insert(tk_b.end(), {0, v0, v1, 0});
+      } else {
+        tk_b.insert(tk_b.end(), {0, 0, 0, 0});
+      }
+      for (size_t q = 0; q < t.L.size(); q++) {
+        prs.push_back(t.L[q]);
+        prs.push_back(t.U[q]);
+      }
+      wmax = std::max(wmax, (int)work(x));
+    }
+    return std::pair<int, int>((int)ord.size(), wmax);
+  };
+  std::vector<std::vector<int>> ch_piv;   // pivots of each main chunk, in order
   S.ch_task.assign(1, 0);
   S.ch_pair.assign(1, 0);
   S.lv_chunk.assign(1, 0);
-  int64_t ntask_total = 0;
   for (int l = 0; l < S.nlev; l++) {
     if (l >= S.tail_level) { S.lv_chunk.push_back((int)S.ch_max.size()); continue; }
     // pivot groups of the level
-    std::vector<std::vector<Task>> groups;
+    std::vector<std::pair<int, std::vector<Task>>> groups;
     for (int k : piv[l]) {
       const auto &sk = st[k];
       std::vector<Task> out(1 + 2 * sk.size());
@@ -656,25 +711,21 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
         t.U.insert(t.U.begin(), -1);
         t.dst &= ~TASK_HAS_A;
       }
-      groups.push_back(std::move(out));
+      groups.push_back({k, std::move(out)});
     }
-    auto gpairs = [](const std::vector<Task> &g) {
-      size_t n = 0;
-      for (auto &t : g) n += t.L.size();
-      return n;
-    };
     std::stable_sort(groups.begin(), groups.end(),
-                     [&](const std::vector<Task> &x, const std::vector<Task> &y) {
-                       return gpairs(x) > gpairs(y);
+                     [&](const std::pair<int, std::vector<Task>> &x,
+                         const std::pair<int, std::vector<Task>> &y) {
+                       return gpairs(x.second) > gpairs(y.second);
                      });
     // pack groups into chunks (first fit in order of decreasing size) of at
     // most chunk_pairs products; a group above that (but within the LDS
     // buffer of CHUNK_PAIRS) gets a chunk of its own
     const size_t cap = (size_t)S.prm.chunk_pairs;
-    std::vector<std::vector<int>> chunks;   // group indices
+    std::vector<std::vector<int>> chunks;   // pivots
     std::vector<size_t> fill;
-    for (int gi = 0; gi < (int)groups.size(); gi++) {
-      size_t np = gpairs(groups[gi]);
+    for (auto &gr : groups) {
+      size_t np = gpairs(gr.second);
       if (np > (size_t)CHUNK_PAIRS) {
         if (err) *err = "a pivot's factorization products exceed one LDS chunk (" +
                         std::to_string(np) + " > " + std::to_string(CHUNK_PAIRS) + ")";
@@ -684,58 +735,16 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
       for (int q = 0; q < (int)chunks.size(); q++)
         if (fill[q] + np <= cap) { c = q; break; }
       if (c < 0) { chunks.push_back({}); fill.push_back(0); c = (int)chunks.size() - 1; }
-      chunks[c].push_back(gi);
+      chunks[c].push_back(gr.first);
       fill[c] += np;
+      pgroup[gr.first] = std::move(gr.second);
     }
     for (auto &cg : chunks) {
-      // tasks of the chunk, heaviest first (an L output also reduces its
-      // pivot's products)
-      std::vector<std::pair<int, int>> ord;  // (group, index in group)
-      for (int gi : cg)
-        for (int q = 0; q < (int)groups[gi].size(); q++) {
-          const Task &t = groups[gi][q];
-          // a U block without products keeps its Jacobian value: nothing to do
-          if (t.L.empty() && (t.dst & TASK_HAS_A) && !(t.dst & (TASK_DIAG | TASK_LCOL)))
-            continue;
-          ord.push_back({gi, q});
-        }
-      auto work = [&](const std::pair<int, int> &x) {
-        const Task &t = groups[x.first][x.second];
-        size_t w = t.L.size();
-        if (t.dst & TASK_LCOL) w = std::max(w, groups[x.first][0].L.size());
-        return w;
-      };
-      std::stable_sort(ord.begin(), ord.end(),
-                       [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
-                         return work(x) > work(y);
-                       });
-      // pair offsets of every task (in chunk order)
-      const int base = (int)S.pairs.size() / 2;
-      std::map<std::pair<int, int>, int> start;
-      int off = base;
-      for (auto &x : ord) { start[x] = off; off += (int)groups[x.first][x.second].L.size(); }
-      int wmax = 0;
-      for (auto &x : ord) {
-        const Task &t = groups[x.first][x.second];
-        const int p0 = start[x], p1 = p0 + (int)t.L.size();
-        S.tk_a.insert(S.tk_a.end(), {t.dst, p0, p1, 0});
-        if (t.dst & TASK_LCOL) {
-          const std::pair<int, int> pv{x.first, 0};
-          const int v0 = start[pv], v1 = v0 + (int)groups[x.first][0].L.size();
-          S.tk_b.insert(S.tk_b.end(), {0, v0, v1, 0});
-        } else {
-          S.tk_b.insert(S.tk_b.end(), {0, 0, 0, 0});
-        }
-        for (size_t q = 0; q < t.L.size(); q++) {
-          S.pairs.push_back(t.L[q]);
-          S.pairs.push_back(t.U[q]);
-        }
-        wmax = std::max(wmax, (int)work(x));
-        ntask_total++;
-      }
-      S.ch_task.push_back((int)ntask_total);
+      const auto r = emit_chunk(cg, S.tk_a, S.tk_b, S.pairs);
+      S.ch_task.push_back(S.ch_task.back() + r.first);
       S.ch_pair.push_back((int)S.pairs.size() / 2);
-      S.ch_max.push_back(wmax);
+      S.ch_max.push_back(r.second);
+      ch_piv.push_back(cg);
     }
     S.lv_chunk.push_back((int)S.ch_max.size());
   }
@@ -774,6 +783,133 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
                     " chunks, more than the " + std::to_string(MAX_CHUNKS) +
                     " the kernel keeps in LDS";
     return false;
+  }
+  // ---- split schedules -------------------------------------------------------
+  // A pivot's outputs read only outputs of its descendants, so the subtrees
+  // below a cut level share no operands and can be factored by workgroups
+  // of their own.  The cut and the grouping come from a model of one
+  // instance's chunk time measured on MI355X with cycle stamps (about
+  // 2.6 us per chunk plus 1 ns per product) and about 4 us for the launch of
+  // the top workgroup: per variant the cut with the least modelled time,
+  // kept when it beats the main schedule.
+  {
+    constexpr double CHUNK_US = 2.6, PRODUCT_US = 1.0e-3, LAUNCH_US = 4.0;
+    auto chunk_us = [&](double prods) { return CHUNK_US + PRODUCT_US * prods; };
+    const int c_first = S.l0_chunks;   // chunks before it are the fill's
+    const int l_first = c_first > 0 ? 1 : 0;
+    std::vector<double> pprod(nb, 0.0), cprod(S.nchunk);
+    for (int k = 0; k < nb; k++) pprod[k] = (double)gpairs(pgroup[k]);
+    for (int c = 0; c < S.nchunk; c++) cprod[c] = S.ch_pair[c + 1] - S.ch_pair[c];
+    double main_us = 0;
+    for (int c = c_first; c < S.nchunk; c++) main_us += chunk_us(cprod[c]);
+    S.sp_model_us[0] = main_us;
+    S.sp_level.assign(SPLIT_VARIANTS, -1);
+    S.sp_rest.assign(SPLIT_VARIANTS, S.nchunk);
+    S.sp_grp.assign(SPLIT_VARIANTS * (SPLIT_MAX + 1), 0);
+    std::vector<int> rt(nb), grp(nb);
+    // the lane-group size a main chunk's reduction uses for each workgroup
+    // size the factorization may be launched with (the kernel's rule):
+    // pieces of main chunks that agree on all three can share a chunk
+    auto lane_groups = [&](int c) {
+      std::array<int, 3> r;
+      const int items = S.ch_task[c + 1] - S.ch_task[c], maxlen = S.ch_max[c];
+      for (int b = 0; b < 3; b++) {
+        int g = 1;
+        while (g < 64 && g < maxlen && items * 2 * g <= (256 << b)) g <<= 1;
+        r[b] = g;
+      }
+      return r;
+    };
+    std::vector<int> ch_level(S.nchunk, 0);
+    for (int l = 0; l + 1 < (int)S.lv_chunk.size(); l++)
+      for (int c = S.lv_chunk[l]; c < S.lv_chunk[l + 1]; c++) ch_level[c] = l;
+    // group j's chunks below the cut: its share of each main chunk, shares
+    // of one level with the same lane groups merged up to chunk_pairs
+    // products; each chunk as {pivots, representative main chunk, products}
+    struct GChunk { std::vector<int> pks; int c; double prods; };
+    auto group_chunks = [&](int Lc, int j) {
+      std::vector<GChunk> out;
+      for (int c = c_first; c < S.lv_chunk[Lc]; c++) {
+        std::vector<int> pks;
+        double pr = 0;
+        for (int k : ch_piv[c])
+          if (grp[k] == j) { pks.push_back(k); pr += pprod[k]; }
+        if (pks.empty()) continue;
+        if (!out.empty()) {
+          GChunk &o = out.back();
+          if (ch_level[o.c] == ch_level[c] && lane_groups(o.c) == lane_groups(c) &&
+              o.prods + pr <= (double)S.prm.chunk_pairs) {
+            o.pks.insert(o.pks.end(), pks.begin(), pks.end());
+            o.prods += pr;
+            continue;
+          }
+        }
+        out.push_back({pks, c, pr});
+      }
+      return out;
+    };
+    // the pivots below level Lc grouped by subtree into K groups (largest
+    // subtrees first, each to the group with the fewest products); returns
+    // the modelled time of the split factorization
+    auto partition = [&](int Lc, int K) {
+      for (int k = nb - 1; k >= 0; k--)   // parents come after their children
+        rt[k] = (lev[k] < Lc && parent[k] >= 0 && lev[parent[k]] < Lc) ? rt[parent[k]] : k;
+      std::vector<double> tp(nb, 0.0);
+      std::vector<int> roots;
+      for (int k = 0; k < nb; k++)
+        if (lev[k] < Lc) {
+          tp[rt[k]] += pprod[k];
+          if (rt[k] == k) roots.push_back(k);
+        }
+      std::stable_sort(roots.begin(), roots.end(),
+                       [&](int x, int y) { return tp[x] > tp[y]; });
+      std::vector<double> load(K, 0.0);
+      std::vector<int> gof(nb, -1);
+      for (int r : roots) {
+        const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[j] += tp[r];
+        gof[r] = j;
+      }
+      for (int k = 0; k < nb; k++) grp[k] = lev[k] < Lc ? gof[rt[k]] : -1;
+      double t1 = 0;
+      for (int j = 0; j < K; j++) {
+        double tj = 0;
+        for (const GChunk &g : group_chunks(Lc, j)) tj += chunk_us(g.prods);
+        t1 = std::max(t1, tj);
+      }
+      double t = t1 + LAUNCH_US;
+      for (int c = S.lv_chunk[Lc]; c < S.nchunk; c++) t += chunk_us(cprod[c]);
+      return t;
+    };
+    for (int v = 0; v < SPLIT_VARIANTS; v++) {
+      const int K = 2 << v;
+      int *gp = &S.sp_grp[v * (SPLIT_MAX + 1)];
+      double best = 0;
+      int bl = -1;
+      for (int Lc = l_first + 1; Lc <= S.tail_level; Lc++) {
+        const double t = partition(Lc, K);
+        if (bl < 0 || t < best) { best = t; bl = Lc; }
+      }
+      S.sp_model_us[v + 1] = best;
+      if (bl >= 0 && best < main_us) {
+        partition(bl, K);
+        S.sp_level[v] = bl;
+        S.sp_rest[v] = S.lv_chunk[bl];
+        for (int j = 0; j < K; j++) {
+          gp[j] = (int)S.sp_lg.size() / 2;
+          for (const GChunk &g : group_chunks(bl, j)) {
+            const int k0 = (int)S.sp_tk_a.size() / 4, q0 = (int)S.sp_pairs.size() / 2;
+            emit_chunk(g.pks, S.sp_tk_a, S.sp_tk_b, S.sp_pairs);
+            S.sp_ch.insert(S.sp_ch.end(), {k0, (int)S.sp_tk_a.size() / 4, q0,
+                                           (int)S.sp_pairs.size() / 2});
+            S.sp_lg.insert(S.sp_lg.end(),
+                           {S.ch_task[g.c + 1] - S.ch_task[g.c], S.ch_max[g.c]});
+          }
+        }
+      }
+      for (int j = S.sp_level[v] >= 0 ? K : 0; j <= SPLIT_MAX; j++)
+        gp[j] = (int)S.sp_lg.size() / 2;
+    }
   }
   // ---- dense tail ------------------------------------------------------------
   {

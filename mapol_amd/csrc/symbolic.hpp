@@ -23,6 +23,8 @@ constexpr int SLOT_LG_SHIFT = 27;    // slot column word: column + 1 | log2 grou
 constexpr int SLOT_NOT_LAST = 1 << 30;  // slot row word: a segment before the row's last
 constexpr int MAX_CHUNKS = 512;      // chunk tables the kernels keep in LDS
 constexpr int TAIL_MAX = 64;         // dense tail: at most 64 blocks square
+constexpr int SPLIT_MAX = 8;         // workgroups per instance, split factorization
+constexpr int SPLIT_VARIANTS = 3;    // splits into 2, 4 and 8 workgroups
 // FP32 factorization: distinct blocks of the tail products staged in LDS
 // (with the chunk tables, which the tail no longer needs, in the same space)
 constexpr int TAIL_STAGE_MAX = 5760;
@@ -82,6 +84,22 @@ struct Symbolic {
   // value at the L position itself (a pivot's Jacobian block)
   std::vector<int> pairs;
   int64_t npairs = 0;
+  // Split factorization, for few instances at a time: variant v splits
+  // the levels below sp_level[v] into K = 2 << v groups of whole subtrees
+  // of the elimination tree (no output of one group is an operand of
+  // another), each run by a workgroup of its own; then one workgroup runs
+  // the main chunks from sp_rest[v] and the dense tail.  A group's chunks
+  // are the main chunks' outputs restricted to the group, in the main order
+  // (sp_tk_a, sp_tk_b, sp_pairs as tk_a, tk_b, pairs); each keeps the task
+  // count and longest reduction of its main chunk (sp_lg), which set the
+  // lane groups, so every output is summed exactly as in the main schedule.
+  // sp_ch = {first task, end task, first product, end product} per chunk;
+  // group g of variant v owns chunks sp_grp[v * (SPLIT_MAX + 1) + g] up to
+  // the next entry.  sp_level[v] = -1: the variant is not worth a launch.
+  std::vector<int> sp_level, sp_rest, sp_grp;
+  std::vector<int> sp_ch, sp_lg;
+  std::vector<int> sp_tk_a, sp_tk_b, sp_pairs;
+  double sp_model_us[SPLIT_VARIANTS + 1] = {0, 0, 0, 0};  // modelled chunk time, one instance
   // dense tail: the pivots of the top levels (>= tail_level, at most
   // TAIL_MAX of them, ascending = topological) are factored as one dense
   // block in LDS.  Its entries start from A minus the contributions of the

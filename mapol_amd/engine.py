@@ -32,7 +32,7 @@ EXPORTS = [
     "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_stage_tail_inverse",
     "bpf_adjoint",
     "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
-    "bpf_set_schedule",
+    "bpf_set_schedule", "bpf_set_factor_split",
     "bpf_analyse", "bpf_analyse_sweeps", "bpf_analyse_fill", "bpf_analyse_tail",
     "bpf_analyse_slots", "bpf_analyse_slots_transposed",
     "bpf_load_matpower",
@@ -133,6 +133,7 @@ def load_library(path: str = LIB_PATH):
     L.bpf_get_kernel_time.argtypes = [V, C.c_char_p, _pi, _pd]
     L.bpf_set_launch.argtypes = [V, C.c_int32, C.c_int32, C.c_int32]
     L.bpf_set_schedule.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.bpf_set_factor_split.argtypes = [V, C.c_int32]
     L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 21
     L.bpf_analyse_sweeps.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 3
     L.bpf_analyse_fill.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
@@ -601,6 +602,13 @@ class Engine:
                                             int(tail_inverse), int(sweep_form),
                                             int(parts)),
                     "bpf_set_schedule")
+
+    def set_factor_split(self, workgroups=0):
+        """workgroups per instance for the factorization's subtrees while few
+        instances are active (``bpf_set_factor_split``): 0 adaptive, 1 never,
+        2 / 4 / 8 always"""
+        self._check(self.L.bpf_set_factor_split(self.h, int(workgroups)),
+                    "bpf_set_factor_split")
 
     def set_launch(self, lu=0, solve=0, mismatch=0):
         self._check(self.L.bpf_set_launch(self.h, lu, solve, mismatch),

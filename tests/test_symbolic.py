@@ -654,3 +654,28 @@ def test_engine_bus_numbering_keeps_the_analysis(tmp_path_factory):
     for o in (0, 1, 2):
         r = _run_check(exe, len(types), len(f), types, f, t, o)
         assert r.returncode == 0 and r.stdout.startswith("identical"), (o, r.stdout)
+
+
+def test_split_factorization_schedules(tmp_path_factory):
+    """the split factorization's group schedules (tests/cpp/split_check.cpp):
+    every output below the cut exactly once with the main schedule's
+    products, order and lane groups; no group reads another group's outputs
+    or its own before they are made"""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path_factory.mktemp("split") / "split_check")
+    subprocess.run(["g++", "-std=c++17", "-O2",
+                    os.path.join(root, "tests", "cpp", "split_check.cpp"),
+                    os.path.join(root, "mapol_amd", "csrc", "symbolic.cpp"),
+                    "-o", exe], check=True, capture_output=True)
+    for name, orders, planned in [("pf118x200", (0,), 0),
+                                  ("pf1354x200", (0, 1), 3),
+                                  ("pf2869x100", (0,), 3),
+                                  ("pf9241x200", (0,), 3)]:
+        case, _ = synth.workload(name, T=2)
+        for o in orders:
+            r = _run_check(exe, case.nbus, case.nbranch, case.bus_type,
+                           case.br_from, case.br_to, o)
+            assert r.returncode == 0, (name, o, r.stdout, r.stderr)
+            assert r.stdout.strip().endswith(f"ok, {planned} variants"), (name, o, r.stdout)
