@@ -950,9 +950,9 @@ static void launch_newton_step(bpf_engine *e) {
                        e->stream, e->g, e->d.st, 1, e->fplan, LU,
                        (B4<T> *)e->d_TL, c0, e->d.jsing, 1,
                        (const int *)e->d.active_count, budget, force);
-    hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,
+    hipLaunchKernelGGL(k_lu_factor_top<T>, dim3(e->T), dim3(e->lu_threads), 0,
                        e->stream, e->g, e->d.st, 1, e->fplan, LU,
-                       (B4<T> *)e->d_TL, c0, e->d.jsing, 2,
+                       (B4<T> *)e->d_TL, c0, e->d.jsing,
                        (const int *)e->d.active_count, budget, force);
   } else {
     hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,

@@ -949,7 +949,6 @@ __device__ __forceinline__ B4<T> jac_block(const DevTopo &g, const DevInst &d,
 
 template <typename T>
 __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *d.active_count = 0;
   // one thread per Jacobian block, in block-position order: the writes into
   // the factor slab are nearly contiguous, the reads of Ybus are gathered
   // JU blocks per thread (rows r and r + blockDim.x of a part of JU *
@@ -1001,10 +1000,11 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
 // pivots (FP64) on the vector ALUs; S^-1 goes to TL for the sweeps.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
-                            FactorPlan P, B4<T> *LU, B4<T> *TL, int c_start,
-                            int *jsing, int mode, const int *active, int budget,
-                            int force) {
+__device__ __forceinline__ void lu_factor_body(DevTopo g, InstState *st, int count_iter,
+                                               FactorPlan P, B4<T> *LU, B4<T> *TL,
+                                               int c_start, int *jsing, int mode,
+                                               const int *active, int budget,
+                                               int force) {
   static_assert(TAIL_MAX * TAIL_MAX == CHUNK_PAIRS,
                 "the chunk products and the dense tail share one LDS buffer");
   // mode 0: one workgroup per instance runs the main chunks from c_start and
@@ -1588,6 +1588,26 @@ __global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter,
     s.it_total += count_iter;
     st[t] = s;
   }
+}
+
+// one launch per factorization (mode 0), or the first of the split pair
+// (mode 1: the subtrees, or everything when the batch does not split)
+template <typename T>
+__global__ void k_lu_factor(DevTopo g, InstState *st, int count_iter, FactorPlan P,
+                            B4<T> *LU, B4<T> *TL, int c_start, int *jsing, int mode,
+                            const int *active, int budget, int force) {
+  lu_factor_body<T>(g, st, count_iter, P, LU, TL, c_start, jsing, mode, active,
+                    budget, force);
+}
+
+// the second of the split pair: the chunks above the cut and the dense tail
+// (returns at once when the first launch did not split)
+template <typename T>
+__global__ void k_lu_factor_top(DevTopo g, InstState *st, int count_iter, FactorPlan P,
+                                B4<T> *LU, B4<T> *TL, int c_start, int *jsing,
+                                const int *active, int budget, int force) {
+  lu_factor_body<T>(g, st, count_iter, P, LU, TL, c_start, jsing, 2, active,
+                    budget, force);
 }
 
 // Tail rows of a sweep: rs = b_S - L_SN y_N (trans: b_S - U_NS^T y_N), then
@@ -2297,6 +2317,10 @@ __global__ void k_residual(DevTopo g, DevInst d) {
 // the reference's angle is atan(y/x), complexUtils.h:21-23).
 // ---------------------------------------------------------------------------
 __global__ void k_update(DevTopo g, DevInst d) {
+  // the active-instance count restarts here for the verdicts that follow:
+  // until now it held the last verdicts' count, which the factorization of
+  // this iteration reads to choose its split
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d.active_count = 0;
   const XSlot sl = xcd_slot(parts_of(g.n));
   const int t = sl.t, i = sl.part * blockDim.x + threadIdx.x;
   if (t >= d.T || i >= g.n) return;
@@ -2686,6 +2710,8 @@ template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *, int);
 template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *, int);
 template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *, int, const int *, int, int);
 template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *, int, int *, int, const int *, int, int);
+template __global__ void k_lu_factor_top<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *, const int *, int, int);
+template __global__ void k_lu_factor_top<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *, int, int *, const int *, int, int);
 template __global__ void k_lu_solve<float, 0>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
 template __global__ void k_lu_solve<float, 2>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
 template __global__ void k_lu_solve<float, 4>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);

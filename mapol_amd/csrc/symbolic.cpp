@@ -791,7 +791,8 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   // instance's chunk time measured on MI355X with cycle stamps (about
   // 2.6 us per chunk plus 1 ns per product) and about 4 us for the launch of
   // the top workgroup: per variant the cut with the least modelled time,
-  // kept when it beats the main schedule.
+  // kept when it saves at least 15 % of the main schedule's (below that,
+  // as at 1354 buses, the second launch costs more than the model says).
   {
     constexpr double CHUNK_US = 2.6, PRODUCT_US = 1.0e-3, LAUNCH_US = 4.0;
     auto chunk_us = [&](double prods) { return CHUNK_US + PRODUCT_US * prods; };
@@ -891,7 +892,7 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
         if (bl < 0 || t < best) { best = t; bl = Lc; }
       }
       S.sp_model_us[v + 1] = best;
-      if (bl >= 0 && best < main_us) {
+      if (bl >= 0 && best < 0.85 * main_us) {
         partition(bl, K);
         S.sp_level[v] = bl;
         S.sp_rest[v] = S.lv_chunk[bl];

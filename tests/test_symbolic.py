@@ -670,8 +670,8 @@ def test_split_factorization_schedules(tmp_path_factory):
                     os.path.join(root, "mapol_amd", "csrc", "symbolic.cpp"),
                     "-o", exe], check=True, capture_output=True)
     for name, orders, planned in [("pf118x200", (0,), 0),
-                                  ("pf1354x200", (0, 1), 3),
-                                  ("pf2869x100", (0,), 3),
+                                  ("pf1354x200", (0, 1), 0),
+                                  ("pf2869x100", (0, 1), 2),
                                   ("pf9241x200", (0,), 3)]:
         case, _ = synth.workload(name, T=2)
         for o in orders:
