@@ -773,12 +773,15 @@ __device__ __forceinline__ bool pair_inv(const B4<T> (&m)[2][2], B4<T> (&r)[2][2
 // which leaves Q (the pivot block's inverse) on the pivot block, Q row on the
 // pivot rows, -col Q in the pivot columns and M - col Q row elsewhere: two
 // MFMAs per wave.  The next quad's columns and rows, and its inverse
-// (pair_inv by lane 0 of the wave holding its pivot block, after gathering
-// the block with shuffles), pass through LDS double buffers: one barrier per
-// step.  A singular pivot block sets sing.
+// (pair_inv by lane 0 of the wave holding its pivot block, after reading the
+// block from its lanes with v_readlane), pass through LDS double buffers:
+// one barrier per step.  The columns are kept column-major and the rows
+// row-major, so that a lane writes its four consecutive column entries, and
+// reads or writes its column's four row entries, as one 16-byte access.  A
+// singular pivot block sets sing.
 constexpr int TAIL_NS = 2 * TAIL_MAX;   // scalars per side of the dense tail
-struct TailQuads {
-  float col[2][TAIL_NS][4], row[2][4][TAIL_NS], q[2][4][4];
+struct alignas(16) TailQuads {
+  float col[2][4][TAIL_NS], row[2][TAIL_NS][4], q[2][4][4];
 };
 
 __device__ __forceinline__ void tail_inverse_mfma(const B4<float> *D, int s, int nt,
@@ -818,8 +821,8 @@ __device__ __forceinline__ void tail_inverse_mfma(const B4<float> *D, int s, int
     const int q = (int)threadIdx.x;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      mq.col[0][q][k] = mget(q, k);
-      mq.row[0][k][q] = mget(k, q);
+      mq.col[0][k][q] = mget(q, k);
+      mq.row[0][q][k] = mget(k, q);
     }
   }
   if (threadIdx.x == 0) {
@@ -841,9 +844,10 @@ __device__ __forceinline__ void tail_inverse_mfma(const B4<float> *D, int s, int
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int k = 2 * h + half;
-      a[h] = prow ? ((ia & 3) == k ? 1.f : 0.f) : -mq.col[cur][ia][k];
-      const float qr = mq.q[cur][k][0] * mq.row[cur][0][jc] + mq.q[cur][k][1] * mq.row[cur][1][jc] +
-                       mq.q[cur][k][2] * mq.row[cur][2][jc] + mq.q[cur][k][3] * mq.row[cur][3][jc];
+      a[h] = prow ? ((ia & 3) == k ? 1.f : 0.f) : -mq.col[cur][k][ia];
+      const float4 rw = *(const float4 *)mq.row[cur][jc];
+      const float4 qk = *(const float4 *)mq.q[cur][k];
+      const float qr = qk.x * rw.x + qk.y * rw.y + qk.z * rw.z + qk.w * rw.w;
       b[h] = pcol ? mq.q[cur][k][jc & 3] : qr;
     }
 #pragma unroll
@@ -870,15 +874,16 @@ __device__ __forceinline__ void tail_inverse_mfma(const B4<float> *D, int s, int
           for (int k = 0; k < 4; k++) rv[k] = acc[12 + k];
         }
         if (half == (qq & 1))
-#pragma unroll
-          for (int k = 0; k < 4; k++) mq.row[nxt][k][jc] = rv[k];
+          *(float4 *)mq.row[nxt][jc] = make_float4(rv[0], rv[1], rv[2], rv[3]);
         if (nb >= cb && nb < cb + 32) {   // and its pivot block
-          const int src = 32 * (qq & 1) + (nb - cb);
+          const int src = 32 * (qq & 1) + (nb - cb);   // the same in every lane
           float e[4][4];
 #pragma unroll
           for (int k = 0; k < 4; k++)
 #pragma unroll
-            for (int m = 0; m < 4; m++) e[k][m] = __shfl(rv[k], src + m, 64);
+            for (int m = 0; m < 4; m++)
+              e[k][m] = __int_as_float(
+                  __builtin_amdgcn_readlane(__float_as_int(rv[k]), src + m));
           if (lane == 0) {
             B4<float> m[2][2], r[2][2];
 #pragma unroll
@@ -896,7 +901,9 @@ __device__ __forceinline__ void tail_inverse_mfma(const B4<float> *D, int s, int
         const int k = (lane & 31) - (nb - cb);
         if (k >= 0 && k < 4)
 #pragma unroll
-          for (int v = 0; v < 16; v++) mq.col[nxt][acc_row(v)][k] = acc[v];
+          for (int v = 0; v < 16; v += 4)
+            *(float4 *)&mq.col[nxt][k][acc_row(v)] =
+                make_float4(acc[v], acc[v + 1], acc[v + 2], acc[v + 3]);
       }
     }
     __syncthreads();
