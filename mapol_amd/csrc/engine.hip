@@ -911,7 +911,8 @@ static void launch_mismatch(bpf_engine *e, int mode) {
 }
 
 // one sweep pass of k_lu_solve; the slot form (ep 2) has one kernel per
-// workgroup size (SLOT_CHUNK / threads slots per lane)
+// count of slots per lane: the fewest (1, 2, 4 or 8) whose lanes cover the
+// plan's largest chunk (fewer slots per lane, fewer instructions per chunk)
 template <typename T>
 static void launch_solve(bpf_engine *e, int T_, const void *LU, int rhs_mode,
                          int accumulate, int use_lds, int ep,
@@ -923,10 +924,16 @@ static void launch_solve(bpf_engine *e, int T_, const void *LU, int rhs_mode,
                        (const B4<T> *)LU, (const B4<T> *)TL, rhs_mode,
                        accumulate, e->splan, use_lds, ep, (T *)xglob);
   };
-  if (ep == 2 && th == 1024) go(k_lu_solve<T, SLOT_CHUNK / 1024>);
-  else if (ep == 2 && th == 512) go(k_lu_solve<T, SLOT_CHUNK / 512>);
-  else if (ep == 2 && th == 256) go(k_lu_solve<T, SLOT_CHUNK / 256>);
-  else go(k_lu_solve<T, 0>);
+  int sp = 1;
+  while (sp < 8 && sp * th < e->S.prm.slot_chunk) sp *= 2;
+  if (ep == 2 && sp * th >= e->S.prm.slot_chunk) {
+    if (sp == 1) go(k_lu_solve<T, 1>);
+    else if (sp == 2) go(k_lu_solve<T, 2>);
+    else if (sp == 4) go(k_lu_solve<T, 4>);
+    else go(k_lu_solve<T, 8>);
+  } else {
+    go(k_lu_solve<T, 0>);
+  }
 }
 
 template <typename T>

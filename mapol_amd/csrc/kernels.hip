@@ -2218,10 +2218,11 @@ __device__ __forceinline__ void solve_pass(const DevTopo &g, const B4<T> *A,
   }
 }
 
-// SP > 0: slot form with SLOT_CHUNK / SP lanes per workgroup (the launch
-// bound lets the larger slot counts of narrower workgroups stay in registers)
+// SP > 0: slot form with SP slots per lane, for chunks of at most SP x the
+// workgroup's lanes (the launch bound lets the larger slot counts of
+// narrower workgroups stay in registers)
 template <typename T, int SP>
-__global__ void __launch_bounds__(SP > 0 ? SLOT_CHUNK / SP : 1024)
+__global__ void __launch_bounds__(SP > 0 && SLOT_CHUNK / SP < 1024 ? SLOT_CHUNK / SP : 1024)
 k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, const B4<T> *TL,
            int rhs_mode, int accumulate, SolvePlan P, int use_lds, int ep,
            T *xglob) {
@@ -2720,10 +2721,12 @@ template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPl
 template __global__ void k_lu_factor_top<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *, const int *, int, int);
 template __global__ void k_lu_factor_top<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *, int, int *, const int *, int, int);
 template __global__ void k_lu_solve<float, 0>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
+template __global__ void k_lu_solve<float, 1>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
 template __global__ void k_lu_solve<float, 2>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
 template __global__ void k_lu_solve<float, 4>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
 template __global__ void k_lu_solve<float, 8>(DevTopo, DevInst, const B4<float> *, const B4<float> *, int, int, SolvePlan, int, int, float *);
 template __global__ void k_lu_solve<double, 0>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
+template __global__ void k_lu_solve<double, 1>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
 template __global__ void k_lu_solve<double, 2>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
 template __global__ void k_lu_solve<double, 4>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
 template __global__ void k_lu_solve<double, 8>(DevTopo, DevInst, const B4<double> *, const B4<double> *, int, int, SolvePlan, int, int, double *);
