@@ -32,7 +32,7 @@ TILE_CANDIDATES = {
     "ordering": (0, 1),            # minimum degree; its least-height reorder
     "tail_limit": (64, 56, 48, 40),
     "chunk_products": (4096, 3072, 2048),
-    "slot_chunk": (2048, 1024),
+    "slot_chunk": (2048, 1024, 512),
 }
 SCHEDULE_CANDIDATES = {
     "newton_per_graph": (4, 2, 8),
