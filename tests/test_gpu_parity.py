@@ -297,7 +297,7 @@ def test_graph_replay_matches_direct_launches():
 
 @pytest.mark.parametrize("tiles", [
     dict(ordering=1), dict(tail_limit=40), dict(tail_limit=47),
-    dict(chunk_products=2048), dict(slot_chunk=1024),
+    dict(chunk_products=2048), dict(slot_chunk=1024), dict(slot_chunk=512),
     dict(ordering=1, tail_limit=48, chunk_products=3072, slot_chunk=1024)])
 def test_tile_options_match_oracle(tiles):
     """every schedule the autotuner may choose (ordering, dense-tail size,
