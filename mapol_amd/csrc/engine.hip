@@ -75,6 +75,11 @@ struct bpf_engine {
   // ascending) and, for each of its entries, the engine's Ybus position
   std::vector<int> bus_ext, bus_int;
   std::vector<int> yx_row, yx_col, yx_pos;
+  // the same maps on the device, and a staging buffer for per-bus rows
+  // crossing the interface (grown on demand, freed with the engine)
+  const int *d_bus_ext = nullptr, *d_bus_int = nullptr;
+  void *d_bus_stage = nullptr;
+  size_t bus_stage_bytes = 0;
   std::vector<double> Pd, Qd, Gs, Bs, Vm, Va, R, X, B, tap, shift, Pg, Qg,
       Qmax, Qmin, VG;
   std::vector<uint8_t> br_status, gen_status;
@@ -261,6 +266,7 @@ extern "C" void bpf_destroy(bpf_engine *e) {
   free_pool(e->inst_owned);
   free_pool(e->ctl_owned);
   free_pool(e->owned);
+  if (e->d_bus_stage) hipFree(e->d_bus_stage);
   if (e->h_count) hipHostFree(e->h_count);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
@@ -271,27 +277,55 @@ static std::vector<T> vcopy(const T *p, int n) {
   return p ? std::vector<T>(p, p + n) : std::vector<T>();
 }
 
-// rows of T per-bus values between the caller's bus numbering and the
-// engine's: in, dst[t][k] = src[t][bus_ext[k]]; out, the inverse
-template <class V>
-static void bus_rows_in(const bpf_engine *e, const V *src, V *dst, size_t T) {
-  const size_t n = e->n;
-  for (size_t t = 0; t < T; t++)
-    for (size_t k = 0; k < n; k++) dst[t * n + k] = src[t * n + e->bus_ext[k]];
-}
-template <class V>
-static void bus_rows_out(const bpf_engine *e, const V *src, V *dst, size_t T) {
-  const size_t n = e->n;
-  for (size_t t = 0; t < T; t++)
-    for (size_t k = 0; k < n; k++) dst[t * n + e->bus_ext[k]] = src[t * n + k];
+// T rows of per-bus values between the caller's bus numbering and the
+// engine's, permuted on the device through a staging buffer: out of the
+// engine, dst[t][x] = src[t][bus_int[x]]; into it, dst[t][k] = src[t][bus_ext[k]]
+static bpf_status bus_stage(bpf_engine *e, size_t bytes) {
+  if (e->bus_stage_bytes >= bytes) return BPF_OK;
+  if (e->d_bus_stage) HIPCHK(hipFree(e->d_bus_stage));
+  e->d_bus_stage = nullptr;
+  e->bus_stage_bytes = 0;
+  HIPCHK(hipMalloc(&e->d_bus_stage, bytes));
+  e->bus_stage_bytes = bytes;
+  return BPF_OK;
 }
 
-// T rows of per-bus values from the device, in the caller's numbering
 template <class V>
 static bpf_status bus_rows_download(bpf_engine *e, const void *dsrc, V *dst, size_t T) {
-  std::vector<V> h(T * e->n);
-  HIPCHK(hipMemcpy(h.data(), dsrc, sizeof(V) * h.size(), hipMemcpyDeviceToHost));
-  bus_rows_out(e, h.data(), dst, T);
+  const size_t cnt = T * e->n, bytes = sizeof(V) * cnt;
+  if (cnt == 0) return BPF_OK;
+  if (!e->opt.bus_numbering) {
+    HIPCHK(hipMemcpyAsync(dst, dsrc, bytes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return BPF_OK;
+  }
+  bpf_status s = bus_stage(e, bytes);
+  if (s) return s;
+  hipLaunchKernelGGL(k_bus_gather<V>, dim3(div_up(cnt, 256)), dim3(256), 0, e->stream,
+                     (const V *)dsrc, (V *)e->d_bus_stage, e->d_bus_int, e->n, (int)T);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dst, e->d_bus_stage, bytes, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return BPF_OK;
+}
+
+// the caller's T rows of voltages into the engine's V (queued on the
+// engine's stream; the host array is read before the call returns)
+static bpf_status bus_rows_upload(bpf_engine *e, const cd *src, cd *ddst, size_t T) {
+  const size_t cnt = T * e->n, bytes = sizeof(cd) * cnt;
+  if (cnt == 0) return BPF_OK;
+  if (!e->opt.bus_numbering) {
+    HIPCHK(hipMemcpyAsync(ddst, src, bytes, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return BPF_OK;
+  }
+  bpf_status s = bus_stage(e, bytes);
+  if (s) return s;
+  HIPCHK(hipMemcpyAsync(e->d_bus_stage, src, bytes, hipMemcpyHostToDevice, e->stream));
+  hipLaunchKernelGGL(k_bus_gather<cd>, dim3(div_up(cnt, 256)), dim3(256), 0, e->stream,
+                     (const cd *)e->d_bus_stage, ddst, e->d_bus_ext, e->n, (int)T);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
   return BPF_OK;
 }
 
@@ -432,6 +466,7 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
 #define UP(dst, vec) \
   if ((s = upload(e, e->owned, vec, &dst))) return s;
   UP(g.yrow, S.yrow); UP(g.ycol, S.ycol); UP(g.yrowof, S.yrowof);
+  UP(e->d_bus_ext, e->bus_ext); UP(e->d_bus_int, e->bus_int);
   {
     std::vector<int> yblk(S.ycol.size());
     for (size_t p = 0; p < yblk.size(); p++) yblk[p] = S.blk_of_bus[S.ycol[p]];
@@ -1666,9 +1701,8 @@ extern "C" bpf_status bpf_get_ybus(bpf_engine *e, int32_t t, int32_t *rowptr,
 // upload voltages and evaluate the mismatch (mode 2: no verdicts)
 static bpf_status stage_eval(bpf_engine *e, const double *V) {
   if (!e->solved) { e->err = "call bpf_solve first (Ybus is assembled there)"; return BPF_ERR_STATE; }
-  std::vector<cd> Vi((size_t)e->T * e->n);
-  bus_rows_in(e, (const cd *)V, Vi.data(), e->T);
-  HIPCHK(hipMemcpy(e->d.V, Vi.data(), sizeof(cd) * Vi.size(), hipMemcpyHostToDevice));
+  bpf_status s = bus_rows_upload(e, (const cd *)V, e->d.V, e->T);
+  if (s) return s;
   mismatch_pair(e, 2, e->opt.maxit_nr, 0, 0);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
@@ -1916,12 +1950,9 @@ static bpf_status stage_start(bpf_engine *e, const double *V) {
                      e->g, e->d, (const int8_t *)e->d_type0,
                      (const double *)e->d_Qinj0);
   HIPCHK(hipGetLastError());
-  std::vector<cd> Vi;
   if (V) {
-    Vi.resize((size_t)e->T * e->n);
-    bus_rows_in(e, (const cd *)V, Vi.data(), e->T);
-    HIPCHK(hipMemcpyAsync(e->d.V, Vi.data(), sizeof(cd) * Vi.size(),
-                          hipMemcpyHostToDevice, e->stream));
+    bpf_status s = bus_rows_upload(e, (const cd *)V, e->d.V, e->T);
+    if (s) return s;
   }
   HIPCHK(hipStreamSynchronize(e->stream));
   e->solved = false;   // the state of the last solve is overwritten

@@ -2713,7 +2713,20 @@ __global__ void k_branch_report(DevTopo g, int T, const cd *Sf, const cd *St,
   }
 }
 
+// T rows of per-bus values from one bus numbering to another: dst[t][k] =
+// src[t][map[k]] (the interface's per-bus inputs and outputs)
+template <typename V>
+__global__ void k_bus_gather(const V *src, V *dst, const int *map, int n, int T) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)n * T) return;
+  const size_t t = q / (size_t)n, k = q - t * (size_t)n;
+  dst[q] = src[t * (size_t)n + map[k]];
+}
+
 // explicit instantiations used by the engine
+template __global__ void k_bus_gather<cd>(const cd *, cd *, const int *, int, int);
+template __global__ void k_bus_gather<double>(const double *, double *, const int *, int, int);
+template __global__ void k_bus_gather<int8_t>(const int8_t *, int8_t *, const int *, int, int);
 template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *, int);
 template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *, int);
 template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *, int, const int *, int, int);
