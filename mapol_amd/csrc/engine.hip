@@ -986,7 +986,11 @@ static void launch_newton_step(bpf_engine *e) {
     // per instance when few instances are active, then the top; the
     // workgroup count follows the batch's active-instance counter (the
     // compute units shared among the parts that run side by side)
-    const int force = e->split_mode == 0 ? -2 : (e->split_mode == 2 ? 0 : e->split_mode == 4 ? 1 : 2);
+    // the kernel's choice: -2 adaptive, else the variant of 2, 4 or 8 groups
+    int force = -2;
+    if (e->split_mode == 2) force = 0;
+    else if (e->split_mode == 4) force = 1;
+    else if (e->split_mode == 8) force = 2;
     const int budget = std::max(1, e->num_cu / std::max(1, e->parts_running));
     hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T * SPLIT_MAX), dim3(e->lu_threads), 0,
                        e->stream, e->g, e->d.st, 1, e->fplan, LU,
