@@ -1937,6 +1937,18 @@ __device__ __forceinline__ void lu_sweeps_ep(const B4<T> *A, const B4<T> *Ti,
 // Every lane of the wave must take part.
 template <typename T>
 __device__ __forceinline__ void slot_group_sum(T &p0, T &p1, int lg, int glg) {
+  // every lane of the wave in a group of the first lane's size: after glg
+  // steps each group's first lane holds its sum (the same adds as below,
+  // without picking each lane's step)
+  if (__builtin_amdgcn_ballot_w64(lg != glg) == 0) {
+    if (glg > 0) { p0 += dpp_row<0x101>(p0); p1 += dpp_row<0x101>(p1); }
+    if (glg > 1) { p0 += dpp_row<0x102>(p0); p1 += dpp_row<0x102>(p1); }
+    if (glg > 2) { p0 += dpp_row<0x104>(p0); p1 += dpp_row<0x104>(p1); }
+    if (glg > 3) { p0 += dpp_row<0x108>(p0); p1 += dpp_row<0x108>(p1); }
+    if (glg > 4) { p0 += lane_up16(p0); p1 += lane_up16(p1); }
+    if (glg > 5) { p0 += lane_up32(p0); p1 += lane_up32(p1); }
+    return;
+  }
   T r0 = p0, r1 = p1;
   // DPP row_shl:n (0x100 + n): lane i reads lane i + n of its 16-lane row
   if (glg > 0) {
