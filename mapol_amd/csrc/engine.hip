@@ -605,12 +605,19 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   }
   F.tn = S.tail_n;
   F.tnent = (int)S.tl_pos.size();
-  F.tmax = S.tl_max;
   UP(F.tdense, S.tl_dense); UP(F.tpos, S.tl_pos); UP(F.tpp, S.tl_pp);
   UP(F.tpairL, S.tl_pairL); UP(F.tpairU, S.tl_pairU);
   F.tstage_n = (int)S.tl_stage.size();
   F.tail_mfma = 1;   // bpf_set_schedule chooses
   UP(F.tstage, nonempty(S.tl_stage, 1)); UP(F.tpairS, nonempty(S.tl_pairS, 1));
+  {
+    const std::vector<int> js = tail_jobs(S.tl_pp, 12), jg = tail_jobs(S.tl_pp, 4);
+    F.tjobs_s = (int)js.size() / 2;
+    F.tjobs_g = (int)jg.size() / 2;
+    const int *p;
+    UP(p, nonempty(js, 2)); F.tjob_s = (const int2 *)p;
+    UP(p, nonempty(jg, 2)); F.tjob_g = (const int2 *)p;
+  }
   SolvePlan &SP = e->splan;
   SP.nfw = S.nfw;
   SP.nbw = S.nbw;

@@ -514,10 +514,15 @@ struct FactorPlan {
   // the chunk's first product | length << TK_LEN_SHIFT
   const int4 *tk;
   const int2 *pairs;                   // {L, U} positions (U < 0: init entry)
-  int tn, tnent, tmax;                 // dense tail: pivots, entries, longest list
+  int tn, tnent;                       // dense tail: pivots, entries
   const int *tdense, *tpos, *tpp, *tpairL, *tpairU;
   int tstage_n;                        // staged tail blocks (0: no staging)
   const int *tstage, *tpairS;          // their positions; pair -> L | U << 16
+  // the tail entries in jobs of one wave each, {first entry, end entry << 3
+  // | log2 lane group size}: for the staged form (up to 12 products per
+  // lane) and for the form that reads operands from memory (up to 4)
+  int tjobs_s, tjobs_g;
+  const int2 *tjob_s, *tjob_g;
   int tail_mfma;                       // FP32 tail inverse on matrix cores
   // split factorization: variant v (2 << v workgroups per instance) leaves
   // the main chunks from sp_rest[v] to the top workgroup (-1: no such
@@ -1251,20 +1256,23 @@ __device__ __forceinline__ void lu_factor_body(DevTopo g, InstState *st, int cou
       // from LDS in the order of its pair list (the sums of the plain loop)
       constexpr int SU = 6;
       const int ns = P.tstage_n, bd = (int)blockDim.x;
-      const int gsz = __builtin_amdgcn_readfirstlane(lane_group(P.tnent, P.tmax));
-      const int lane = threadIdx.x & (gsz - 1);
-      // an entry's descriptor is requested a pass ahead: the first pass's
-      // with the staging loads, the next pass's while a pass runs (clamped
-      // to the last entry; every tail has at least one)
-      int dd_n, pos_n, qa_n, q1_n;
-      auto descriptor = [&](int k) {
-        k = min(k, P.tnent - 1);
-        dd_n = P.tdense[k];
-        pos_n = P.tpos[k];
-        qa_n = P.tpp[k];
-        q1_n = P.tpp[k + 1];
+      // an entry's descriptor is requested a job ahead: the first job's
+      // with the staging loads, the next job's while a job runs
+      const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+      const int nw = (int)blockDim.x >> 6, wl = (int)threadIdx.x & 63;
+      int dd_n, pos_n, qa_n, q1_n, lg_n;
+      auto descriptor = [&](int j) {
+        const int2 jb = P.tjob_s[min(j, P.tjobs_s - 1)];
+        lg_n = jb.y & 7;
+        const int k = jb.x + (wl >> lg_n);
+        const int kc = min(k, P.tnent - 1);
+        dd_n = P.tdense[kc];
+        pos_n = P.tpos[kc];
+        qa_n = P.tpp[kc];
+        q1_n = k < (jb.y >> 3) ? P.tpp[kc + 1] : qa_n;   // past the job: no products
+        if (k >= (jb.y >> 3)) dd_n = -1;
       };
-      descriptor((int)threadIdx.x / gsz);
+      descriptor(wv);
       for (int q0 = threadIdx.x; q0 < ns; q0 += SU * bd) {
         int sp[SU];
 #pragma unroll
@@ -1277,15 +1285,17 @@ __device__ __forceinline__ void lu_factor_body(DevTopo g, InstState *st, int cou
           if (q0 + u * bd < ns) sh.stage[q0 + u * bd] = sv[u];
       }
       __syncthreads();
-      // pair indices of a lane requested together: one round trip per 12
-      // products (16 would spill; the entries come longest first, so the
-      // lanes of a wave have lists of about the same length)
+      // a job's lists take one round of pair-index requests (12 per lane;
+      // 16 would spill) and a group sum of log2(group size) steps
       constexpr int TU = 12;
-      for (int k = (int)threadIdx.x / gsz; k < P.tnent; k += (int)blockDim.x / gsz) {
+      for (int j = wv; j < P.tjobs_s; j += nw) {
         const int dd = dd_n, pos = pos_n, qa = qa_n, q1 = q1_n;
-        descriptor(k + (int)blockDim.x / gsz);
-        const B4<T> a0 = A[(dd & TASK_HAS_A) ? pos : 0];
-        B4<T> acc = blk_sel(lane == 0 && (dd & TASK_HAS_A), a0, blk_zero<T>());
+        const int gsz = __builtin_amdgcn_readfirstlane(1 << lg_n);
+        descriptor(j + nw);
+        const int lane = wl & (gsz - 1);
+        const bool has_a = dd >= 0 && (dd & TASK_HAS_A);
+        const B4<T> a0 = A[has_a ? pos : 0];
+        B4<T> acc = blk_sel(lane == 0 && has_a, a0, blk_zero<T>());
         for (int q = qa + lane; q < q1; q += TU * gsz) {
           int ps[TU];
 #pragma unroll
@@ -1296,18 +1306,24 @@ __device__ __forceinline__ void lu_factor_body(DevTopo g, InstState *st, int cou
                     sh.stage[ps[u] >> 16]);
         }
         group_sum4_first(acc, gsz);
-        if (lane == 0) D[dd & TASK_POS_MASK] = acc;
+        if (lane == 0 && dd >= 0) D[dd & TASK_POS_MASK] = acc;
       }
     } else {
-      const int gsz = __builtin_amdgcn_readfirstlane(lane_group(P.tnent, P.tmax));
-      const int lane = threadIdx.x & (gsz - 1);
+      const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+      const int nw = (int)blockDim.x >> 6, wl = (int)threadIdx.x & 63;
       constexpr int TU = 4;  // products of one lane requested together
-      for (int k = (int)threadIdx.x / gsz; k < P.tnent; k += (int)blockDim.x / gsz) {
-        const int dd = P.tdense[k];
-        const int pos = P.tpos[k];
-        const int qa = P.tpp[k], q1 = P.tpp[k + 1];
+      for (int j = wv; j < P.tjobs_g; j += nw) {
+        const int2 jb = P.tjob_g[j];
+        const int lg = jb.y & 7, gsz = __builtin_amdgcn_readfirstlane(1 << lg);
+        const int lane = wl & (gsz - 1);
+        const int k = jb.x + (wl >> lg);
+        const bool on = k < (jb.y >> 3);
+        const int kc = min(k, P.tnent - 1);
+        const int dd = P.tdense[kc];
+        const int pos = P.tpos[kc];
+        const int qa = P.tpp[kc], q1 = on ? P.tpp[kc + 1] : qa;
         const B4<T> a0 = A[(dd & TASK_HAS_A) ? pos : 0];
-        B4<T> acc = blk_sel(lane == 0 && (dd & TASK_HAS_A), a0, blk_zero<T>());
+        B4<T> acc = blk_sel(on && lane == 0 && (dd & TASK_HAS_A), a0, blk_zero<T>());
         for (int q = qa + lane; q < q1; q += TU * gsz) {
           int il[TU], iu[TU];
 #pragma unroll
@@ -1329,7 +1345,7 @@ __device__ __forceinline__ void lu_factor_body(DevTopo g, InstState *st, int cou
             blk_fms(acc, blk_sel(q + u * gsz < q1, lv[u], blk_zero<T>()), uv[u]);
         }
         group_sum4_first(acc, gsz);
-        if (lane == 0) D[dd & TASK_POS_MASK] = acc;
+        if (on && lane == 0) D[dd & TASK_POS_MASK] = acc;
       }
     }
     __syncthreads();

@@ -311,6 +311,21 @@ std::vector<int> elimination_order(int n, int m, const int *from, const int *to,
   return ord;
 }
 
+std::vector<int> tail_jobs(const std::vector<int> &pp, int per_lane) {
+  std::vector<int> jobs;
+  const int ne = (int)pp.size() - 1;
+  for (int k = 0; k < ne;) {
+    const int len = pp[k + 1] - pp[k];
+    int lg = 0;
+    while (lg < 6 && (len + (1 << lg) - 1) >> lg > per_lane) lg++;
+    const int e1 = std::min(ne, k + (64 >> lg));
+    jobs.push_back(k);
+    jobs.push_back(e1 << 3 | lg);
+    k = e1;
+  }
+  return jobs;
+}
+
 std::vector<int> rcm_numbering(int n, int m, const int *from, const int *to) {
   std::vector<std::vector<int>> adj(n);
   for (int b = 0; b < m; b++)
@@ -947,10 +962,9 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
       }
     }
     // longest product lists first (stable): the factorization takes the
-    // entries one per lane in passes over the workgroup, and a wave runs as
-    // long as its longest list, so the long lists share the first waves of
-    // the first pass and the later passes hold only short ones; every entry
-    // keeps its own list and order, so its sum is the same
+    // entries in jobs of one wave (tail_jobs), each job's lane groups sized
+    // by its first list, so the lists of a job are of about one length;
+    // every entry keeps its own list and order
     const int ne = (int)S.tl_pos.size();
     std::vector<int> ord(ne);
     for (int q = 0; q < ne; q++) ord[q] = q;

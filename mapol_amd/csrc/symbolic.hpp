@@ -169,6 +169,13 @@ std::vector<int> elimination_order(int n, int m, const int *from, const int *to,
 // near numbers, so gathers over a bus's branches touch few cache lines.
 std::vector<int> rcm_numbering(int n, int m, const int *from, const int *to);
 
+// the tail entries (longest pair lists first, list ends `pp`) in jobs of
+// one wave each: a job takes consecutive entries with one lane group size,
+// the smallest power of two (up to 64) that covers its first, longest list
+// with at most `per_lane` products per lane.  Packed {first entry,
+// end entry << 3 | log2 group size}.
+std::vector<int> tail_jobs(const std::vector<int> &pp, int per_lane);
+
 // minimum-degree order of a symmetric graph given as adjacency lists
 std::vector<int> min_degree_order(std::vector<std::vector<int>> adj);
 

@@ -10,7 +10,9 @@
 //   output is summed in the same order as in the main schedule;
 // * a group reads no output of another group nor of a chunk above the cut,
 //   and reads an output of its own only from an earlier chunk;
-// * the group chunk tables are contiguous and in range.
+// * the group chunk tables are contiguous and in range;
+// * the dense tail's wave jobs cover every tail entry once, each list
+//   within its job's lane groups.
 //
 // Reads "n m ordering", n bus types and m branch end pairs from stdin;
 // prints one line per variant and "ok" (exit 0) or the first violation
@@ -148,6 +150,30 @@ int main() {
     if ((int)seen.size() != below) return fail("outputs missing", v, (int)seen.size(), below);
     printf("variant K=%d: cut level %d, first top chunk %d, %d chunks, model %.1f us against %.1f\n",
            K, S.sp_level[v], cut, gp[K] - gp[0], S.sp_model_us[v + 1], S.sp_model_us[0]);
+  }
+  // the dense tail's wave jobs: consecutive entries covering every one once,
+  // at most a wave of lanes each, every list within per_lane products per
+  // lane of its group (or a 64-lane group), the group size the least that
+  // holds the job's first list
+  for (int per_lane : {12, 4}) {
+    const std::vector<int> jobs = tail_jobs(S.tl_pp, per_lane);
+    const int ne = (int)S.tl_pp.size() - 1;
+    int next = 0;
+    for (size_t j = 0; j + 1 < jobs.size(); j += 2) {
+      const int e0 = jobs[j], e1 = jobs[j + 1] >> 3, lg = jobs[j + 1] & 7;
+      if (e0 != next || e1 <= e0) return fail("tail jobs not contiguous", -1, e0, next);
+      if (lg > 6 || e1 - e0 > (64 >> lg)) return fail("tail job wider than a wave", -1, e0, lg);
+      for (int k = e0; k < e1; k++) {
+        const int len = S.tl_pp[k + 1] - S.tl_pp[k];
+        if (lg < 6 && (len + (1 << lg) - 1) >> lg > per_lane)
+          return fail("tail list beyond its lane group", -1, k, len);
+      }
+      const int len0 = S.tl_pp[e0 + 1] - S.tl_pp[e0];
+      if (lg > 0 && (len0 + (1 << (lg - 1)) - 1) >> (lg - 1) <= per_lane)
+        return fail("tail job group larger than needed", -1, e0, lg);
+      next = e1;
+    }
+    if (next != ne) return fail("tail entries not covered", -1, next, ne);
   }
   printf("ok, %d variants\n", planned);
   return 0;

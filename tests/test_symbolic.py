@@ -660,7 +660,8 @@ def test_split_factorization_schedules(tmp_path_factory):
     """the split factorization's group schedules (tests/cpp/split_check.cpp):
     every output below the cut exactly once with the main schedule's
     products, order and lane groups; no group reads another group's outputs
-    or its own before they are made"""
+    or its own before they are made; the dense tail's wave jobs cover every
+    tail entry once"""
     import os
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
