@@ -142,6 +142,10 @@ typedef struct {
                                       (a new capture follows a change of
                                       instances, launch shapes or factor
                                       precision) */
+  int32_t fused_forward;           /* the last Newton step's factorization
+                                      also formed the forward substitution
+                                      of the mismatch, and its first solve
+                                      started from it (the slot-form sweeps) */
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */

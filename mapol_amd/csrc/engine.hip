@@ -137,6 +137,7 @@ struct bpf_engine {
   int graph_used = 0;          // the last solve replayed graphs
   int graph_fallback = 0;      // the last solve abandoned replay (graph_err says why)
   int graph_captures = 0;      // captures since bpf_create
+  int fused_forward = 0;       // the last Newton step's first solve skipped the forward sweep
   // timing
   int timing = 0;
   std::vector<TimedEvent> events;
@@ -461,7 +462,8 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   DevTopo &g = e->g;
   g.n = e->n; g.m = e->m; g.nb = S.nb;
   g.nnzY = (int)S.ycol.size();
-  g.nnzLU = (int)S.lcol.size();
+  g.rhs0 = (int)S.lcol.size();
+  g.nnzLU = g.rhs0 + S.nb;
   bpf_status s;
 #define UP(dst, vec) \
   if ((s = upload(e, e->owned, vec, &dst))) return s;
@@ -952,6 +954,13 @@ static void launch_mismatch(bpf_engine *e, int mode) {
   tend(e);
 }
 
+// slots per lane of the slot form's kernel (0: another form runs)
+static int slot_lanes(const bpf_engine *e, int ep) {
+  int sp = 1;
+  while (sp < 8 && sp * e->solve_threads < e->S.prm.slot_chunk) sp *= 2;
+  return ep == 2 && sp * e->solve_threads >= e->S.prm.slot_chunk ? sp : 0;
+}
+
 // one sweep pass of k_lu_solve; the slot form (ep 2) has one kernel per
 // count of slots per lane: the fewest (1, 2, 4 or 8) whose lanes cover the
 // plan's largest chunk (fewer slots per lane, fewer instructions per chunk)
@@ -966,9 +975,8 @@ static void launch_solve(bpf_engine *e, int T_, const void *LU, int rhs_mode,
                        (const B4<T> *)LU, (const B4<T> *)TL, rhs_mode,
                        accumulate, e->splan, use_lds, ep, (T *)xglob);
   };
-  int sp = 1;
-  while (sp < 8 && sp * th < e->S.prm.slot_chunk) sp *= 2;
-  if (ep == 2 && sp * th >= e->S.prm.slot_chunk) {
+  const int sp = slot_lanes(e, ep);
+  if (sp > 0) {
     if (sp == 1) go(k_lu_solve<T, 1>);
     else if (sp == 2) go(k_lu_solve<T, 2>);
     else if (sp == 4) go(k_lu_solve<T, 4>);
@@ -983,9 +991,17 @@ static void launch_newton_step(bpf_engine *e) {
   B4<T> *LU = (B4<T> *)e->d_LU;
   // the fill also finishes level 0 when the analysis allows it
   const int c0 = e->S.l0_chunks;
+  size_t xs = sizeof(T) * 2 * (size_t)e->S.nb;
+  int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
+  const int ep = sweep_mode(e->S, sizeof(T), use_lds, e->sweep_form, e->solve_threads);
+  // with the slot-form sweeps the factorization also forms the forward
+  // substitution of the mismatch (the fill writes its right-hand side), and
+  // the first solve starts from it
+  const int fused = slot_lanes(e, ep) > 0 ? 1 : 0;
+  e->fused_forward = fused;
   tbegin(e, "jacobian");
   hipLaunchKernelGGL(k_jacobian<T>, xgrid(e->T, ((size_t)e->g.nJb + 1) / 2, 256),
-                     dim3(256), 0, e->stream, e->g, e->d, LU, c0 > 0 ? 1 : 0);
+                     dim3(256), 0, e->stream, e->g, e->d, LU, c0 > 0 ? 1 : 0, fused);
   tend(e);
   tbegin(e, "lu_factor");
   if (e->split_mode != 1 && e->fplan_split) {
@@ -1013,10 +1029,7 @@ static void launch_newton_step(bpf_engine *e) {
                        (B4<T> *)e->d_TL, c0, e->d.jsing, 0, nullptr, 0, -1);
   }
   tend(e);
-  size_t xs = sizeof(T) * 2 * (size_t)e->S.nb;
-  int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
   const int refine = (sizeof(T) == 4) ? e->opt.refine_steps : 0;
-  const int ep = sweep_mode(e->S, sizeof(T), use_lds, e->sweep_form, e->solve_threads);
   // one solve on the mismatch, then per refinement step the FP64 residual
   // (many workgroups per instance) and a solve on it, accumulated into dx
   for (int s = 0; s <= refine; s++) {
@@ -1027,7 +1040,7 @@ static void launch_newton_step(bpf_engine *e) {
       tend(e);
     }
     tbegin(e, "lu_solve");
-    launch_solve<T>(e, e->T, LU, s == 0 ? 0 : 3, s > 0 ? 1 : 0, use_lds, ep);
+    launch_solve<T>(e, e->T, LU, s == 0 ? (fused ? 4 : 0) : 3, s > 0 ? 1 : 0, use_lds, ep);
     tend(e);
   }
 }
@@ -1669,6 +1682,7 @@ extern "C" bpf_status bpf_get_info(const bpf_engine *e, bpf_info *info) {
   info->graph_used = e->graph_used;
   info->graph_fallback = e->graph_fallback;
   info->graph_captures = e->graph_captures;
+  info->fused_forward = e->fused_forward;
   int fp64 = e->opt.precision == BPF_FP64;
   info->lds_solve = ((fp64 ? 8 : 4) * 2 * (size_t)S.nb) <= (size_t)SOLVE_LDS_BYTES;
   return BPF_OK;
@@ -1776,7 +1790,7 @@ extern "C" bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V,
   HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(LU, 0, sizeof(B4<double>) * (size_t)e->T * e->g.nnzLU));
   hipLaunchKernelGGL(k_jacobian<double>, xgrid(e->T, ((size_t)e->g.nJb + 1) / 2, 256),
-                     dim3(256), 0, e->stream, e->g, e->d, LU, 0);
+                     dim3(256), 0, e->stream, e->g, e->d, LU, 0, 0);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(e->d.st, st.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
@@ -1893,7 +1907,7 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   const unsigned TT = (unsigned)T;
   tbegin(e, "adjoint");
   hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, ((size_t)e->g.nJb + 1) / 2, 256),
-                     dim3(256), 0, e->stream, e->g, e->d, LU, 0);
+                     dim3(256), 0, e->stream, e->g, e->d, LU, 0, 0);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
                      e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing, 0, nullptr, 0, -1);
   hipLaunchKernelGGL(k_adj_rhs, dim3(div_up(e->S.nb, 256), TT), dim3(256), 0,

@@ -959,8 +959,10 @@ __device__ __forceinline__ B4<T> jac_block(const DevTopo &g, const DevInst &d,
   return v;
 }
 
+// rhs: also write the right-hand-side blocks {F_P, 0; F_Q, 0} after the
+// factor's, where the factorization's forward substitution starts
 template <typename T>
-__global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
+__global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0, int rhs) {
   // one thread per Jacobian block, in block-position order: the writes into
   // the factor slab are nearly contiguous, the reads of Ybus are gathered
   // JU blocks per thread (rows r and r + blockDim.x of a part of JU *
@@ -988,6 +990,12 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0) {
       v = blk_mul(v, inv);
     }
     LU[(size_t)t * g.nnzLU + jb.x] = v;
+    if (rhs && r < g.nb) {
+      const double2 f = ((const double2 *)(d.F + (size_t)t * 2 * g.nb))[r];
+      B4<T> b;
+      b.a = (T)f.x; b.b = (T)0; b.c = (T)f.y; b.d = (T)0;
+      LU[(size_t)t * g.nnzLU + g.rhs0 + r] = b;
+    }
   }
 }
 
@@ -2144,27 +2152,30 @@ __device__ __forceinline__ int4 slot_chunk(const int4 *slot, const int4 *swc,
 template <typename T, int SP, bool TR = false>
 __device__ __forceinline__ void lu_sweeps_slot(const B4<T> *A, const B4<T> *Ti,
                                                T *X, T *rs, const int4 *swc,
-                                               const SolvePlan &P) {
+                                               const SolvePlan &P, bool skip_fw) {
   const int nfw = TR ? P.sl_nfw_t : P.sl_nfw;
   const int nc = nfw + (TR ? P.sl_nbw_t : P.sl_nbw);
   const bool tail = P.tn > 0;
   const int4 *slots = TR ? P.sl_slot_t : P.sl_slot;
-  if (nc == 0) {
+  // skip_fw: X holds the forward substitution's result already (the tail
+  // rows still hold their right-hand side)
+  const int c0 = skip_fw ? nfw : 0;
+  if (c0 >= nc) {
     if (tail) tail_rows<T>(A, Ti, X, rs, P, TR);
     return;
   }
   const int cl = nc - 1;
   SlotIdx<SP> ia, ib;
   SlotVal<T, SP> va, vb;
-  int4 m = swc[0];
+  int4 m = swc[c0];
   m.x = __builtin_amdgcn_readfirstlane(m.x);
   m.y = __builtin_amdgcn_readfirstlane(m.y);
   m.z = __builtin_amdgcn_readfirstlane(m.z);
   ia = slot_index<SP>(slots, m);
   __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
-  va = slot_values<T, SP, TR>(A, m, 0 >= nfw, ia);
-  ib = slot_index<SP>(slots, swc[min(1, cl)]);
-  for (int c = 0; c < nc; c += 2) {
+  va = slot_values<T, SP, TR>(A, m, c0 >= nfw, ia);
+  ib = slot_index<SP>(slots, swc[min(c0 + 1, cl)]);
+  for (int c = c0; c < nc; c += 2) {
     if (c == nfw && tail) tail_rows<T>(A, Ti, X, rs, P, TR);
     m = slot_chunk<T, SP, TR>(slots, swc, A, X, m, c >= nfw, ia, va,
                               min(c + 1, cl), min(c + 1, cl) >= nfw, ib, vb,
@@ -2207,7 +2218,8 @@ __device__ __forceinline__ void solve_pass(const DevTopo &g, const B4<T> *A,
                                            int rhs_mode, int accumulate,
                                            double *dx) {
   // rhs_mode 0: (P, Q) mismatch; 1: (P, 0) and 2: (0, Q) for the decoupled
-  // half steps; 3: the refinement residual of k_residual.  Vector loads in
+  // half steps; 3: the refinement residual of k_residual; 4: the mismatch,
+  // forward-substituted by the factorization.  Vector loads in
   // groups of VU per thread, all requested before the first is used: with
   // 1024 lanes one round trip covers 20480 scalars (10240 blocks), the
   // right-hand side and the accumulated dx of a 9241-bus grid each in one
@@ -2215,19 +2227,37 @@ __device__ __forceinline__ void solve_pass(const DevTopo &g, const B4<T> *A,
   constexpr int VU = 20;
   const int n2 = 2 * g.nb;
   const int stride = (int)blockDim.x;
-  for (int k0 = threadIdx.x; k0 < n2; k0 += VU * stride) {
-    double v[VU];
+  if (SP > 0 && rhs_mode == 4) {
+    // rhs_mode 4 (slot form only): the factorization formed the forward
+    // substitution of the mismatch in the right-hand-side blocks after the
+    // factor (y in their first column; a tail row's block holds its b)
+    constexpr int YU = sizeof(T) == 4 ? 10 : 5;
+    const B4<T> *Y = A + g.rhs0;
+    for (int k0 = threadIdx.x; k0 < g.nb; k0 += YU * stride) {
+      B4<T> y[YU];
 #pragma unroll
-    for (int u = 0; u < VU; u++) v[u] = src[min(k0 + u * stride, n2 - 1)];
+      for (int u = 0; u < YU; u++) y[u] = Y[min(k0 + u * stride, g.nb - 1)];
 #pragma unroll
-    for (int u = 0; u < VU; u++) {
-      const int k = k0 + u * stride;
-      const bool keep = rhs_mode == 0 || rhs_mode == 3 || (rhs_mode == 1) == ((k & 1) == 0);
-      if (k < n2) X[k] = keep ? (T)v[u] : (T)0;
+      for (int u = 0; u < YU; u++) {
+        const int k = k0 + u * stride;
+        if (k < g.nb) { X[2 * k] = y[u].a; X[2 * k + 1] = y[u].c; }
+      }
+    }
+  } else {
+    for (int k0 = threadIdx.x; k0 < n2; k0 += VU * stride) {
+      double v[VU];
+#pragma unroll
+      for (int u = 0; u < VU; u++) v[u] = src[min(k0 + u * stride, n2 - 1)];
+#pragma unroll
+      for (int u = 0; u < VU; u++) {
+        const int k = k0 + u * stride;
+        const bool keep = rhs_mode == 0 || rhs_mode == 3 || (rhs_mode == 1) == ((k & 1) == 0);
+        if (k < n2) X[k] = keep ? (T)v[u] : (T)0;
+      }
     }
   }
   __syncthreads();
-  if (SP > 0) lu_sweeps_slot<T, (SP > 0 ? SP : 1)>(A, Ti, X, rs, swc, P);
+  if (SP > 0) lu_sweeps_slot<T, (SP > 0 ? SP : 1)>(A, Ti, X, rs, swc, P, rhs_mode == 4);
   else if (LDS && ep) lu_sweeps_ep<T>(A, Ti, X, prod, rs, swc, swm, P);
   else lu_sweeps<T>(g, A, Ti, X, rs, P);
   if (accumulate) {
@@ -2610,7 +2640,7 @@ k_adj_solve_slot(DevTopo g, DevInst d, const B4<double> *LU, const B4<double> *T
     s_swc[q] = P.sl_chunk_t[q];
   for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) X[k] = Gt[k];
   __syncthreads();
-  lu_sweeps_slot<double, SP, true>(A, Ti, X, rs, s_swc, P);
+  lu_sweeps_slot<double, SP, true>(A, Ti, X, rs, s_swc, P, false);
   __syncthreads();
   double *lt = lam + (size_t)t * 2 * g.nb;
   for (int k = threadIdx.x; k < 2 * g.nb; k += blockDim.x) lt[k] = X[k];
@@ -2755,8 +2785,8 @@ __global__ void k_bus_gather(const V *src, V *dst, const int *map, int n, int T)
 template __global__ void k_bus_gather<cd>(const cd *, cd *, const int *, int, int);
 template __global__ void k_bus_gather<double>(const double *, double *, const int *, int, int);
 template __global__ void k_bus_gather<int8_t>(const int8_t *, int8_t *, const int *, int, int);
-template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *, int);
-template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *, int);
+template __global__ void k_jacobian<float>(DevTopo, DevInst, B4<float> *, int, int);
+template __global__ void k_jacobian<double>(DevTopo, DevInst, B4<double> *, int, int);
 template __global__ void k_lu_factor<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *, int, const int *, int, int);
 template __global__ void k_lu_factor<double>(DevTopo, InstState *, int, FactorPlan, B4<double> *, B4<double> *, int, int *, int, const int *, int, int);
 template __global__ void k_lu_factor_top<float>(DevTopo, InstState *, int, FactorPlan, B4<float> *, B4<float> *, int, int *, const int *, int, int);

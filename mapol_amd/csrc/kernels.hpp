@@ -30,7 +30,10 @@ struct InstState {
 
 // shared (topology) arrays on the device
 struct DevTopo {
-  int n, m, nb, nnzY, nnzLU;
+  // nnzLU: blocks per instance of the factor storage, the factor's own
+  // (rhs0 of them) followed by nb right-hand-side blocks (the forward
+  // substitution the factorization forms for the Newton step)
+  int n, m, nb, nnzY, nnzLU, rhs0;
   const int *yrow, *ycol, *yrowof, *ydiag_bus;
   const int *yblk;         // per Ybus entry: block of its column bus (-1: slack)
   const int *yf_ptr, *yf_item, *yt_ptr, *yt_item;

@@ -726,6 +726,22 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
         t.U.insert(t.U.begin(), -1);
         t.dst &= ~TASK_HAS_A;
       }
+      // the forward substitution's entry of the pivot, y_k = b_k - sum over
+      // its L row of L(k,m) y_m, as one more output of the group.  The
+      // right-hand-side blocks follow the factor's (position nnzLU + k,
+      // y in the first column, zero in the second) and start from b_k,
+      // which the Jacobian fill writes; a pivot with an empty L row keeps b_k.
+      if (!lrowv[k].empty()) {
+        Task t;
+        t.dst = nnzLU + k;
+        t.L.push_back(nnzLU + k);
+        t.U.push_back(-1);
+        for (int mm : lrowv[k]) {
+          t.L.push_back(pos(k, mm));
+          t.U.push_back(nnzLU + mm);
+        }
+        out.push_back(std::move(t));
+      }
       groups.push_back({k, std::move(out)});
     }
     std::stable_sort(groups.begin(), groups.end(),

@@ -82,7 +82,7 @@ class BpfInfo(C.Structure):
                 ("fill_level0_chunks", C.c_int32), ("tail_stage", C.c_int32),
                 ("slot_fw_chunks", C.c_int32), ("slot_bw_chunks", C.c_int32),
                 ("slots", C.c_int32), ("graph_fallback", C.c_int32),
-                ("graph_captures", C.c_int32)]
+                ("graph_captures", C.c_int32), ("fused_forward", C.c_int32)]
 
 
 _LIB = None

@@ -22,7 +22,9 @@ pytestmark = pytest.mark.skipif(not __import__("os").path.exists(engine.LIB_PATH
 
 def _blocks(case, a, J, F):
     """Reference-layout J, F -> 2x2 blocks at L+U positions, block-order rhs
-    (PV buses: identity Q row, zero magnitude column)."""
+    (PV buses: identity Q row, zero magnitude column).  The right-hand-side
+    blocks {b_P, 0; b_Q, 0} follow the factor's, as the Jacobian fill writes
+    them for the factorization's forward substitution."""
     nb = a["info"]["nblocks"]
     blk = np.full(case.nbus, -1)
     blk[a["bus_of_blk"]] = np.arange(nb)
@@ -39,7 +41,7 @@ def _blocks(case, a, J, F):
         assert k < len(seg) and seg[k] == c, "J entry outside the L+U pattern"
         return rp[r] + k
 
-    A = np.zeros((len(ci), 2, 2))
+    A = np.zeros((len(ci) + nb, 2, 2))
     J = J.tocoo()
     for r, c, v in zip(J.row, J.col, J.data):
         (br, cr), (bc, cc) = rowmap[r], rowmap[c]
@@ -49,6 +51,7 @@ def _blocks(case, a, J, F):
     rhs = np.zeros(2 * nb)
     for r, (br, cr) in enumerate(rowmap):
         rhs[2 * br + cr] = F[r]
+    A[len(ci):, :, 0] = rhs.reshape(-1, 2)
     return A, rhs, rowmap
 
 
@@ -196,11 +199,13 @@ def _sweeps_ep(a, A, Sinv, b):
     return x.ravel()
 
 
-def _sweeps_slot(a, A, Sinv, b):
+def _sweeps_slot(a, A, Sinv, b, skip_fw=False):
     """the slot-form sweeps: per chunk, every lane's entry product, then each
     row's lane group summed pairwise into its first lane (lane i adds lane
     i + 2^s for s < g), which updates x; the tail rows between the forward
-    and the backward chunks"""
+    and the backward chunks.  skip_fw: b is the factorization's forward
+    substitution (the tail rows' still their right-hand side), so the
+    forward chunks are left out."""
     x = b.reshape(-1, 2).copy()
     nfw = a["info"]["slot_fw_chunks"]
     ch, sl = a["sl_chunk"], a["sl_slot"]
@@ -217,7 +222,7 @@ def _sweeps_slot(a, A, Sinv, b):
         for k in range(s):
             x[piv[k]] = sum(Sinv[k, j] @ rs[j] for j in range(s))
 
-    for c in range(len(ch)):
+    for c in range(nfw if skip_fw else 0, len(ch)):
         if c == nfw:
             tail()
         s0, s1 = ch[c, 0], ch[c, 1]
@@ -321,6 +326,42 @@ def test_schedule_replay_gives_the_newton_step(name):
     # the PV magnitude unknowns of the superset system come out exactly zero
     pv_blocks = [r[0] for r in rowmap[:int((case.bus_type == 2).sum())]]
     assert np.all(x[2 * np.array(pv_blocks, int) + 1] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["pf118x200", "pf1354x200", "pf9241x200"])
+def test_factorization_forms_the_forward_substitution(name):
+    """The right-hand-side outputs of the factorization's chunks hold the
+    forward substitution y = L^-1 b of every row below the dense tail
+    (checked against a plain unit-lower sweep over the factor's L rows and
+    against the kernels' forward chunks); a tail row's block keeps its b;
+    the slot-form pass that starts from them, without its forward chunks,
+    gives the Newton step of the full pass"""
+    case, a, J, F, A, rhs, rowmap = _system(name)
+    nb, nnz = a["info"]["nblocks"], a["info"]["nnz_lu_blocks"]
+    LU, Sinv = _crout(a, A)
+    y = LU[nnz:, :, 0]
+    # the textbook forward substitution with the unit lower factor
+    rp, ci = a["lu_rowptr"], a["lu_colind"]
+    diag = _diag(a)
+    tail = set(int(k) for k in a["tail_piv"])
+    yr = rhs.reshape(-1, 2).copy()
+    for i in range(nb):
+        if i in tail:
+            continue
+        for p in range(rp[i], diag[i]):
+            yr[i] -= LU[p] @ yr[ci[p]]
+    keep = np.array([i not in tail for i in range(nb)])
+    scale = max(1.0, np.abs(yr).max())
+    assert np.abs(y[keep] - yr[keep]).max() <= 1e-10 * scale
+    assert np.array_equal(y[~keep], rhs.reshape(-1, 2)[~keep])
+    assert np.all(LU[nnz:, :, 1] == 0.0)
+    # the full slot pass and the one that starts from y
+    x = _sweeps_slot(a, LU, Sinv, rhs)
+    xf = _sweeps_slot(a, LU, Sinv, y.ravel(), skip_fw=True)
+    assert np.abs(xf - x).max() <= 1e-10 * max(1.0, np.abs(x).max())
+    ref = sla.spsolve(J.tocsc(), F)
+    got = np.array([xf[2 * b + c] for b, c in rowmap])
+    assert np.abs(got - ref).max() <= 1e-10 * max(1, np.abs(ref).max())
 
 
 @pytest.mark.parametrize("name", ["pf14", "pf118x200", "pf1354x200", "pf9241x200"])
@@ -445,11 +486,17 @@ def test_schedule_invariants():
     # every position is written at most once, by a chunk output; the blocks
     # of the dense tail are not written (the tail keeps its inverse) and a
     # U block without products keeps its Jacobian value
-    dst = a["tk_a"][:, 0] & MASK
+    # (positions nnz + k: the forward substitution's right-hand-side block
+    # of pivot k, an output for every pivot below the tail with an L row)
+    dst_all = a["tk_a"][:, 0] & MASK
+    assert len(np.unique(dst_all)) == len(dst_all)
     tail = set(int(k) for k in a["tail_piv"])
+    rhs_k = np.sort(dst_all[dst_all >= nnz] - nnz)
+    has_l = [k for k in range(nb) if k not in tail and ci[rp[k]] < k]
+    assert np.array_equal(rhs_k, has_l)
+    dst = dst_all[dst_all < nnz]
     in_tail = np.array([(r in tail and ci[p] in tail)
                         for r in range(nb) for p in range(rp[r], rp[r + 1])])
-    assert len(np.unique(dst)) == len(dst)
     assert not np.any(in_tail[dst])
     rowof = np.repeat(np.arange(nb), np.diff(rp))
     unwritten = np.setdiff1d(np.flatnonzero(~in_tail), dst)
@@ -458,10 +505,11 @@ def test_schedule_invariants():
     # a product only reads positions finished in an earlier chunk (or its
     # own pivot's block, as an init entry); an L output's pivot range lies
     # in its own chunk
-    done_at = np.full(nnz, 10 ** 9)
+    done_at = np.full(nnz + nb, 10 ** 9)
     done_at[unwritten] = -1          # final from the start
+    done_at[nnz:] = -1               # b, for a pivot without an L row
     for c in range(i["chunks"]):
-        done_at[dst[a["ch_task"][c]:a["ch_task"][c + 1]]] = c
+        done_at[dst_all[a["ch_task"][c]:a["ch_task"][c + 1]]] = c
     for c in range(i["chunks"]):
         q0, q1 = a["ch_pair"][c], a["ch_pair"][c + 1]
         assert q1 - q0 <= 4096
@@ -477,7 +525,7 @@ def test_schedule_invariants():
                 assert q0 <= a["tk_b"][k][1] <= a["tk_b"][k][2] <= q1
     # every block of the Jacobian pattern outside the tail holds a Jacobian
     # value: directly (HAS_A) or as an init entry
-    n_init = int((a["pairs"][:, 1] < 0).sum())
+    n_init = int(((a["pairs"][:, 1] < 0) & (a["pairs"][:, 0] < nnz)).sum())
     assert n_init + int((a["tk_a"][:, 0] & HAS_A > 0).sum()) + len(unwritten) \
         + int((a["tl_dense"] & HAS_A > 0).sum()) == i["nnz_jac_blocks"]
 
@@ -670,12 +718,13 @@ def test_split_factorization_schedules(tmp_path_factory):
                     os.path.join(root, "tests", "cpp", "split_check.cpp"),
                     os.path.join(root, "mapol_amd", "csrc", "symbolic.cpp"),
                     "-o", exe], check=True, capture_output=True)
-    for name, orders, planned in [("pf118x200", (0,), 0),
-                                  ("pf1354x200", (0, 1), 0),
-                                  ("pf2869x100", (0, 1), 2),
-                                  ("pf9241x200", (0,), 3)]:
+    # (ordering, variants planned) per case
+    for name, plans in [("pf118x200", ((0, 0),)),
+                        ("pf1354x200", ((0, 0), (1, 0))),
+                        ("pf2869x100", ((0, 2), (1, 3))),
+                        ("pf9241x200", ((0, 3),))]:
         case, _ = synth.workload(name, T=2)
-        for o in orders:
+        for o, planned in plans:
             r = _run_check(exe, case.nbus, case.nbranch, case.bus_type,
                            case.br_from, case.br_to, o)
             assert r.returncode == 0, (name, o, r.stdout, r.stderr)
