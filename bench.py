@@ -39,15 +39,17 @@ def algorithmic_bytes(info, case, T, its_total, s, refine):
 
     ``its_total`` = sum over instances of Newton iterations (instance-
     iterations); ``s`` = bytes per factor scalar (4 mixed, 8 FP64).  With
-    ``info["fused_forward"]`` the fill also reads the mismatch and writes
-    the right-hand-side blocks, the factorization also reads and writes
-    them, and the first solve reads them (not the mismatch) and skips the
-    forward chunks: every strictly lower block is counted out of its factor
-    read (the few the tail rows still read included, so a lower bound)."""
+    the forward substitution in the factorization (``info["fused_forward"]``
+    >= 0, an active-instance limit, so per launch either form may run) each
+    kernel counts the smaller of its two forms, a lower bound: the fill and
+    the factorization without the right-hand-side blocks, the first solve
+    without the forward chunks (every strictly lower block counted out of
+    its factor read, the few the tail rows still read included) and with
+    the right-hand-side blocks in place of the mismatch."""
     n, m = case.nbus, case.nbranch
     nb, nY = info["nblocks"], info["nnz_ybus"]
     nJ, nLU = info["nnz_jac_blocks"], info["nnz_lu_blocks"]
-    fused = bool(info.get("fused_forward", 0))
+    fused = info.get("fused_forward", -1) >= 0
     # L blocks strictly below the diagonal: (nLU - nb) / 2 of the pattern
     nL = (nLU - nb) // 2
     first = (4 * s * (nLU - nL) + 4 * s * nb + 16 * nb) if fused \
@@ -55,8 +57,8 @@ def algorithmic_bytes(info, case, T, its_total, s, refine):
     per_inst = {
         "ybus": 16 * nY + 9 * m + 8 * n,
         "mismatch": 16 * nY + 48 * n + 16 * nb + n,
-        "jacobian": 16 * nY + 32 * n + n + 4 * s * nJ + (16 * nb + 4 * s * nb if fused else 0),
-        "lu_factor": 4 * s * nJ + 4 * s * nLU + (8 * s * nb if fused else 0),
+        "jacobian": 16 * nY + 32 * n + n + 4 * s * nJ,
+        "lu_factor": 4 * s * nJ + 4 * s * nLU,
         # every pass reads the factor and its right-hand side and writes dx;
         # the refinement passes also read dx (they accumulate)
         "lu_solve": first + refine * (4 * s * nLU + 32 * nb) + refine * 16 * nb,

@@ -142,10 +142,13 @@ typedef struct {
                                       (a new capture follows a change of
                                       instances, launch shapes or factor
                                       precision) */
-  int32_t fused_forward;           /* the last Newton step's factorization
-                                      also formed the forward substitution
-                                      of the mismatch, and its first solve
-                                      started from it (the slot-form sweeps) */
+  int32_t fused_forward;           /* the active-instance limit of the last
+                                      Newton step: its factorization also
+                                      formed the forward substitution of the
+                                      mismatch, and its first solve started
+                                      from it, while at most this many
+                                      instances were active (-1: never; only
+                                      with the slot-form sweeps) */
 } bpf_info;
 
 /* Fill an options struct with the reference's defaults (runpf.h:45-47). */
@@ -450,6 +453,14 @@ bpf_status bpf_set_schedule(bpf_engine *e, int32_t newton_per_graph,
  * planned them.  Results do not depend on it: every output is summed in
  * the same order. */
 bpf_status bpf_set_factor_split(bpf_engine *e, int32_t workgroups);
+
+/* Forward substitution inside the factorization (no counterpart in the
+ * reference, whose solves run both sweeps): while at most `max_active`
+ * instances of a batch part are active, the Newton step's factorization
+ * also forms L^-1 of the mismatch and the first solve skips its forward
+ * sweep; above that both run as usual (-1: never; default 32).  Only the
+ * slot-form sweeps use it.  Results agree to rounding either way. */
+bpf_status bpf_set_fused_forward(bpf_engine *e, int32_t max_active);
 
 #ifdef __cplusplus
 }

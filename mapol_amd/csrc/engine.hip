@@ -544,7 +544,7 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
     // after the main ones)
     const int nt0 = (int)S.tk_a.size() / 4, np0 = (int)S.pairs.size() / 2;
     const int nsp = (int)S.sp_lg.size() / 2, nall = S.nchunk + nsp;
-    std::vector<int> ch(4 * nall), lg(2 * nall);
+    std::vector<int> ch(4 * nall), lg(2 * nall), rhs(2 * nall);
     std::vector<int> tk(S.tk_a.size() + S.sp_tk_a.size());
     // one descriptor per output; an L output's pivot range packed as its
     // offset from the chunk's first product | length << TK_LEN_SHIFT
@@ -571,6 +571,8 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
       std::copy(r, r + 4, &ch[4 * c]);
       lg[2 * c] = r[1] - r[0];
       lg[2 * c + 1] = S.ch_max[c];
+      rhs[2 * c] = S.ch_rhs[2 * c];
+      rhs[2 * c + 1] = S.ch_rhs[2 * c + 1];
       ok = ok && pack(S.tk_a, S.tk_b, r[0], r[1], r[2], 0, 0);
     }
     for (int c = 0; c < nsp; c++) {
@@ -579,7 +581,19 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
       d[0] = r[0] + nt0; d[1] = r[1] + nt0; d[2] = r[2] + np0; d[3] = r[3] + np0;
       lg[2 * (S.nchunk + c)] = S.sp_lg[2 * c];
       lg[2 * (S.nchunk + c) + 1] = S.sp_lg[2 * c + 1];
+      rhs[2 * (S.nchunk + c)] = S.sp_rhs[2 * c];
+      rhs[2 * (S.nchunk + c) + 1] = S.sp_rhs[2 * c + 1];
       ok = ok && pack(S.sp_tk_a, S.sp_tk_b, r[0], r[1], r[2], nt0, np0);
+    }
+    // the lane-group key {tasks, longest reduction} of a chunk carries the
+    // count of its forward-substitution outputs and products in the upper
+    // 16 bits (the LDS chunk tables have no room for another array)
+    for (size_t q = 0; q < lg.size(); q++) {
+      if (lg[q] < 0 || lg[q] >= (1 << 16) || rhs[q] < 0 || rhs[q] >= (1 << 15)) {
+        e->err = "internal: a factorization chunk's counts exceed 16 bits";
+        return BPF_ERR_UNSUPPORTED;
+      }
+      lg[q] |= rhs[q] << 16;
     }
     if (!ok) {
       e->err = "internal: a pivot's product range outside its factorization chunk";
@@ -611,6 +625,7 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
   UP(F.tpairL, S.tl_pairL); UP(F.tpairU, S.tl_pairU);
   F.tstage_n = (int)S.tl_stage.size();
   F.tail_mfma = 1;   // bpf_set_schedule chooses
+  F.fused_limit = FUSED_LIMIT_DEFAULT;   // bpf_set_fused_forward chooses
   UP(F.tstage, nonempty(S.tl_stage, 1)); UP(F.tpairS, nonempty(S.tl_pairS, 1));
   {
     const std::vector<int> js = tail_jobs(S.tl_pp, 12), jg = tail_jobs(S.tl_pp, 4);
@@ -621,6 +636,7 @@ extern "C" bpf_status bpf_create(const bpf_case *c, const bpf_options *o,
     UP(p, nonempty(jg, 2)); F.tjob_g = (const int2 *)p;
   }
   SolvePlan &SP = e->splan;
+  SP.fused_limit = FUSED_LIMIT_DEFAULT;
   SP.nfw = S.nfw;
   SP.nbw = S.nbw;
   UP(SP.fw_ptr, S.fw_ptr); UP(SP.fw_rows, S.fw_rows); UP(SP.fw_max, S.fw_max);
@@ -683,6 +699,18 @@ extern "C" bpf_status bpf_set_factor_split(bpf_engine *e, int32_t workgroups) {
     return BPF_ERR_INVALID_ARG;
   }
   e->split_mode = workgroups;
+  drop_graphs(e);
+  return BPF_OK;
+}
+
+extern "C" bpf_status bpf_set_fused_forward(bpf_engine *e, int32_t max_active) {
+  if (!e) return BPF_ERR_INVALID_ARG;
+  if (max_active < -1) {
+    e->err = "fused forward substitution: -1 (never) or the most active instances";
+    return BPF_ERR_INVALID_ARG;
+  }
+  e->fplan.fused_limit = max_active;
+  e->splan.fused_limit = max_active;
   drop_graphs(e);
   return BPF_OK;
 }
@@ -994,11 +1022,14 @@ static void launch_newton_step(bpf_engine *e) {
   size_t xs = sizeof(T) * 2 * (size_t)e->S.nb;
   int use_lds = xs <= (size_t)SOLVE_LDS_BYTES;
   const int ep = sweep_mode(e->S, sizeof(T), use_lds, e->sweep_form, e->solve_threads);
-  // with the slot-form sweeps the factorization also forms the forward
-  // substitution of the mismatch (the fill writes its right-hand side), and
-  // the first solve starts from it
-  const int fused = slot_lanes(e, ep) > 0 ? 1 : 0;
-  e->fused_forward = fused;
+  // with the slot-form sweeps the factorization can also form the forward
+  // substitution of the mismatch (the fill writes its right-hand side, and
+  // the first solve starts from it; the kernels leave it out while more
+  // than the plan's limit of instances are active)
+  const int fused = slot_lanes(e, ep) > 0 && e->fplan.fused_limit >= 0 ? 1 : 0;
+  FactorPlan fp = e->fplan;
+  if (!fused) fp.fused_limit = -1;
+  e->fused_forward = fused ? fp.fused_limit : -1;
   tbegin(e, "jacobian");
   hipLaunchKernelGGL(k_jacobian<T>, xgrid(e->T, ((size_t)e->g.nJb + 1) / 2, 256),
                      dim3(256), 0, e->stream, e->g, e->d, LU, c0 > 0 ? 1 : 0, fused);
@@ -1016,17 +1047,18 @@ static void launch_newton_step(bpf_engine *e) {
     else if (e->split_mode == 8) force = 2;
     const int budget = std::max(1, e->num_cu / std::max(1, e->parts_running));
     hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T * SPLIT_MAX), dim3(e->lu_threads), 0,
-                       e->stream, e->g, e->d.st, 1, e->fplan, LU,
+                       e->stream, e->g, e->d.st, 1, fp, LU,
                        (B4<T> *)e->d_TL, c0, e->d.jsing, 1,
                        (const int *)e->d.active_count, budget, force);
     hipLaunchKernelGGL(k_lu_factor_top<T>, dim3(e->T), dim3(e->lu_threads), 0,
-                       e->stream, e->g, e->d.st, 1, e->fplan, LU,
+                       e->stream, e->g, e->d.st, 1, fp, LU,
                        (B4<T> *)e->d_TL, c0, e->d.jsing,
                        (const int *)e->d.active_count, budget, force);
   } else {
     hipLaunchKernelGGL(k_lu_factor<T>, dim3(e->T), dim3(e->lu_threads), 0,
-                       e->stream, e->g, e->d.st, 1, e->fplan, LU,
-                       (B4<T> *)e->d_TL, c0, e->d.jsing, 0, nullptr, 0, -1);
+                       e->stream, e->g, e->d.st, 1, fp, LU,
+                       (B4<T> *)e->d_TL, c0, e->d.jsing, 0,
+                       (const int *)e->d.active_count, 0, -1);
   }
   tend(e);
   const int refine = (sizeof(T) == 4) ? e->opt.refine_steps : 0;
@@ -1132,6 +1164,13 @@ static bpf_status run_newton(bpf_engine *e, bool *graph_failed) {
   for (int h = 1; h < bpf_engine::MAX_SPLIT; h++) slot[h] = &e->subgraph[h];
   PartView view[bpf_engine::MAX_SPLIT];
   for (int h = 0; h < K; h++) view[h] = part_view(e, h, K);
+  if (K > 1) {
+    // each part's own active count for its first iteration (the first
+    // mismatch counted the whole batch into the first counter)
+    for (int h = 0; h < K; h++)
+      hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, e->stream, view[h].d);
+    HIPCHK(hipStreamSynchronize(e->stream));
+  }
   if (use_graph && !e->graph) {
     for (int h = 0; h < K && use_graph; h++) {
       PartScope ps(e, view[h]);

@@ -524,6 +524,10 @@ struct FactorPlan {
   int tjobs_s, tjobs_g;
   const int2 *tjob_s, *tjob_g;
   int tail_mfma;                       // FP32 tail inverse on matrix cores
+  // the forward substitution's outputs run while at most this many
+  // instances of the batch are active (-1: never); the solve then starts
+  // from them (same limit in the solve plan)
+  int fused_limit;
   // split factorization: variant v (2 << v workgroups per instance) leaves
   // the main chunks from sp_rest[v] to the top workgroup (-1: no such
   // variant); its group g runs chunks sp_grp[v][g] .. sp_grp[v][g + 1] - 1
@@ -544,7 +548,12 @@ __device__ __forceinline__ int split_variant(const FactorPlan &P, const int *act
   return -1;
 }
 
+// the forward substitution runs in the factorization while at most this
+// many instances of a batch (part) are active (bpf_set_fused_forward)
+constexpr int FUSED_LIMIT_DEFAULT = 32;
+
 struct SolvePlan {
+  int fused_limit;                     // as in the factorization plan
   int nfw, nbw;                        // levels over the non-tail rows
   const int *fw_ptr, *fw_rows, *fw_max, *bw_ptr, *bw_rows, *bw_max;
   int tn, tsmax;                       // tail pivots, longest tail-row list
@@ -1090,9 +1099,18 @@ __device__ __forceinline__ void lu_factor_body(DevTopo g, InstState *st, int cou
       jsing[t] = 0;
     }
   }
+  // the forward substitution's outputs (the last of each chunk, their
+  // products too) are left out unless the solve starts from them
+  const bool fwd = active != nullptr && P.fused_limit >= 0 && *active <= P.fused_limit;
   for (int q = threadIdx.x; q < nc; q += blockDim.x) {
-    s_ch[q] = P.chunks[cb + q];
-    s_lg[q] = P.chunk_lg[cb + q];
+    int4 c4 = P.chunks[cb + q];
+    int2 lg = P.chunk_lg[cb + q];
+    if (!fwd) {
+      c4.y -= lg.x >> 16;
+      c4.w -= lg.y >> 16;
+    }
+    s_ch[q] = c4;
+    s_lg[q] = make_int2(lg.x & 0xFFFF, lg.y & 0xFFFF);
   }
   B4<T> *A = LU + (size_t)t * g.nnzLU;
   __syncthreads();
@@ -2306,6 +2324,10 @@ k_lu_solve(DevTopo g, DevInst d, const B4<T> *LU, const B4<T> *TL,
   T *prod = smem + ep_offset(g.nb);
   // the slot form keeps X after a zero pair (read by slots without an entry)
   if (SP > 0 && threadIdx.x < SLOT_X_PAD) smem[threadIdx.x] = (T)0;
+  // rhs_mode 4 falls back to the mismatch (0) when the factorization left
+  // the forward substitution out: the same test on the same active count
+  if (rhs_mode == 4 && !(P.fused_limit >= 0 && *d.active_count <= P.fused_limit))
+    rhs_mode = 0;
   if (SP > 0)
     solve_pass<T, true, SP>(g, A, Ti, smem + SLOT_X_PAD, prod, rs, s_swc, s_swm,
                             P, 0, src, rhs_mode, accumulate, dx);
@@ -2403,6 +2425,20 @@ __global__ void k_update(DevTopo g, DevInst d) {
 }
 
 __global__ void k_zero_count(DevInst d) { *d.active_count = 0; }
+
+// the active (not finished) instances of a batch part, into its counter:
+// the first Newton iteration of a part reads it (the solve's first
+// mismatch counts the whole batch into the first part's counter)
+__global__ void k_count_active(DevInst d) {
+  __shared__ int n;
+  if (threadIdx.x == 0) n = 0;
+  __syncthreads();
+  int c = 0;
+  for (int t = threadIdx.x; t < d.T; t += blockDim.x) c += d.st[t].done ? 0 : 1;
+  if (c) atomicAdd(&n, c);
+  __syncthreads();
+  if (threadIdx.x == 0) *d.active_count = n;
+}
 
 // ---------------------------------------------------------------------------
 // Fast decoupled: half-step updates (fdpf.h:416-448).  part 1: angles of all

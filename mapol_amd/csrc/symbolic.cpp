@@ -634,8 +634,10 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
   // (an L output also reduces its pivot's products; ties keep the order of
   // pks), then their products in task order.  Appends to the three lists;
   // returns {tasks, longest reduction}.
+  int rhs_out[2] = {0, 0};   // the last chunk's forward-substitution outputs, products
   auto emit_chunk = [&](const std::vector<int> &pks, std::vector<int> &tk_a,
                         std::vector<int> &tk_b, std::vector<int> &prs) {
+    rhs_out[0] = rhs_out[1] = 0;
     std::vector<std::pair<int, int>> ord;  // (pivot, index in its group)
     for (int k : pks)
       for (int q = 0; q < (int)pgroup[k].size(); q++) {
@@ -651,8 +653,13 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
       if (t.dst & TASK_LCOL) w = std::max(w, pgroup[x.first][0].L.size());
       return w;
     };
+    // the forward substitution's outputs (positions from nnzLU) last
+    auto rhs_task = [&](const std::pair<int, int> &x) {
+      return (pgroup[x.first][x.second].dst & TASK_POS_MASK) >= nnzLU;
+    };
     std::stable_sort(ord.begin(), ord.end(),
                      [&](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                       if (rhs_task(x) != rhs_task(y)) return rhs_task(y);
                        return work(x) > work(y);
                      });
     // pair offsets of every task (in chunk order)
@@ -675,6 +682,7 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
         prs.push_back(t.U[q]);
       }
       wmax = std::max(wmax, (int)work(x));
+      if (rhs_task(x)) { rhs_out[0]++; rhs_out[1] += (int)t.L.size(); }
     }
     return std::pair<int, int>((int)ord.size(), wmax);
   };
@@ -772,6 +780,7 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
     }
     for (auto &cg : chunks) {
       const auto r = emit_chunk(cg, S.tk_a, S.tk_b, S.pairs);
+      S.ch_rhs.insert(S.ch_rhs.end(), {rhs_out[0], rhs_out[1]});
       S.ch_task.push_back(S.ch_task.back() + r.first);
       S.ch_pair.push_back((int)S.pairs.size() / 2);
       S.ch_max.push_back(r.second);
@@ -932,6 +941,7 @@ bool analyse(int n, int m, const int *from, const int *to, const int *type,
           for (const GChunk &g : group_chunks(bl, j)) {
             const int k0 = (int)S.sp_tk_a.size() / 4, q0 = (int)S.sp_pairs.size() / 2;
             emit_chunk(g.pks, S.sp_tk_a, S.sp_tk_b, S.sp_pairs);
+            S.sp_rhs.insert(S.sp_rhs.end(), {rhs_out[0], rhs_out[1]});
             S.sp_ch.insert(S.sp_ch.end(), {k0, (int)S.sp_tk_a.size() / 4, q0,
                                            (int)S.sp_pairs.size() / 2});
             S.sp_lg.insert(S.sp_lg.end(),

@@ -77,6 +77,11 @@ struct Symbolic {
   std::vector<int> lv_chunk;
   std::vector<int> ch_task, ch_pair;  // task / pair offsets [nchunk+1]
   std::vector<int> ch_max;            // longest reduction of the chunk
+  // the forward substitution's outputs of a chunk come last, their products
+  // too: per chunk {outputs, products} of them (the factorization leaves
+  // them out when the solve runs its own forward sweep); sp_rhs the same
+  // for the split schedules' chunks
+  std::vector<int> ch_rhs, sp_rhs;
   // per task: tk_a = {dst | flags, first pair, end pair, 0};
   // tk_b = {0, the pivot task's first pair, end pair, 0} (L tasks)
   std::vector<int> tk_a, tk_b;

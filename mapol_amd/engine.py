@@ -32,7 +32,7 @@ EXPORTS = [
     "bpf_stage_jacobian", "bpf_stage_newton_step", "bpf_stage_tail_inverse",
     "bpf_adjoint",
     "bpf_set_timing", "bpf_get_kernel_time", "bpf_set_launch",
-    "bpf_set_schedule", "bpf_set_factor_split",
+    "bpf_set_schedule", "bpf_set_factor_split", "bpf_set_fused_forward",
     "bpf_analyse", "bpf_analyse_sweeps", "bpf_analyse_fill", "bpf_analyse_tail",
     "bpf_analyse_slots", "bpf_analyse_slots_transposed",
     "bpf_load_matpower",
@@ -134,6 +134,7 @@ def load_library(path: str = LIB_PATH):
     L.bpf_set_launch.argtypes = [V, C.c_int32, C.c_int32, C.c_int32]
     L.bpf_set_schedule.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     L.bpf_set_factor_split.argtypes = [V, C.c_int32]
+    L.bpf_set_fused_forward.argtypes = [V, C.c_int32]
     L.bpf_analyse.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 21
     L.bpf_analyse_sweeps.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 3
     L.bpf_analyse_fill.argtypes = [C.POINTER(BpfCase), C.POINTER(BpfInfo)] + [_pi] * 2
@@ -596,12 +597,23 @@ class Engine:
         return n.value, ms.value
 
     def set_schedule(self, newton_per_graph=0, tail_inverse=0, sweep_form=0,
-                     parts=0):
-        """run-time schedule choices (``bpf_set_schedule``; 0 keeps one)"""
+                     parts=0, fused_forward=None):
+        """run-time schedule choices (``bpf_set_schedule``; 0 keeps one);
+        ``fused_forward`` (None keeps it): ``set_fused_forward``"""
         self._check(self.L.bpf_set_schedule(self.h, int(newton_per_graph),
                                             int(tail_inverse), int(sweep_form),
                                             int(parts)),
                     "bpf_set_schedule")
+        if fused_forward is not None:
+            self.set_fused_forward(fused_forward)
+
+    def set_fused_forward(self, max_active=32):
+        """the Newton step's factorization forms the forward substitution
+        (and the first solve skips its forward sweep) while at most
+        ``max_active`` instances of a batch part are active; -1 never
+        (``bpf_set_fused_forward``)"""
+        self._check(self.L.bpf_set_fused_forward(self.h, int(max_active)),
+                    "bpf_set_fused_forward")
 
     def set_factor_split(self, workgroups=0):
         """workgroups per instance for the factorization's subtrees while few

@@ -38,6 +38,9 @@ SCHEDULE_CANDIDATES = {
     "newton_per_graph": (4, 2, 8),
     "tail_inverse": (1, 2),        # matrix cores, vector ALUs
     "parts": (1, 2, 4),            # concurrent parts of the batch
+    # the forward substitution in the factorization while at most this
+    # many instances of a part are active (-1 never, 1 << 20 always)
+    "fused_forward": (32, -1, 8, 1 << 20),
 }
 CANDIDATES = (256, 512, 1024)      # workgroup sizes
 FAMILIES = {"lu": "lu_factor", "solve": "lu_solve", "mismatch": "mismatch"}
@@ -126,7 +129,7 @@ def apply(engine, entry):
     la, sc = entry["launch"], entry["schedule"]
     engine.set_launch(la["lu"], la["solve"], la["mismatch"])
     engine.set_schedule(sc["newton_per_graph"], sc["tail_inverse"], 0,
-                        sc.get("parts", 1))
+                        sc.get("parts", 1), sc.get("fused_forward"))
 
 
 def tune(make_engine, key, cache_path=DEFAULT_CACHE, reps=3, verbose=False,
@@ -183,7 +186,7 @@ def tune(make_engine, key, cache_path=DEFAULT_CACHE, reps=3, verbose=False,
             for c in cands:
                 s = dict(schedule, **{name: c})
                 e.set_schedule(s["newton_per_graph"], s["tail_inverse"], 0,
-                               s.get("parts", 1))
+                               s.get("parts", 1), s.get("fused_forward"))
                 res[c] = _solve_ms(e, reps)
                 trials.append(dict(stage="schedule", schedule=s, ms=round(res[c], 3)))
                 log(f"schedule {s}: {res[c]:.3f} ms")

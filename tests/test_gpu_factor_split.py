@@ -3,7 +3,8 @@ per instance on the subtrees below the cut, or chosen by the active-instance
 count (the default), the Newton solves, the Newton step from given voltages
 and the dense tail's inverse equal those of the one-workgroup factorization
 bit for bit, in mixed precision and in FP64, with graph replay and the
-batch run as parts side by side.  Each output keeps its product order and
+batch run as parts side by side (the forward substitution kept in the
+factorization throughout).  Each output keeps its product order and
 lane groups (checked on the CPU by tests/cpp/split_check.cpp), so equality
 is exact, not within a tolerance.  The workloads are grids the analysis
 plans a split for (2869 and 9241 buses; at 1354 buses it saves too little
@@ -21,8 +22,9 @@ def _run(case, cont, V, wg, parts=1, **kw):
     e = Engine(case, enforce_qlim=True, **kw)
     e.set_instances(br_on=cont.br_on, gen_on=cont.gen_on)
     e.set_factor_split(wg)
-    if parts > 1:
-        e.set_schedule(parts=parts)
+    # the forward substitution always in the factorization: whether it runs
+    # there follows each part's active count and changes the rounding
+    e.set_schedule(parts=parts, fused_forward=1 << 20)
     e.solve()
     r = e.results()
     dx = e.stage_newton_step(V)

@@ -26,7 +26,8 @@ class FakeEngine:
     def __init__(self, clock, tiles, log):
         self.clock, self.tiles, self.log = clock, tiles, log
         self.launch = {"lu": 1024, "solve": 1024, "mismatch": 256}
-        self.sched = {"newton_per_graph": 4, "tail_inverse": 1, "parts": 1}
+        self.sched = {"newton_per_graph": 4, "tail_inverse": 1, "parts": 1,
+                      "fused_forward": 32}
         self.timing = False
         self.kt = {}
         self.closed = False
@@ -41,6 +42,7 @@ class FakeEngine:
         c += {2: 0.2, 4: 0.1, 8: 0.0}[s["newton_per_graph"]]  # best 8
         c += 0.05 if s["tail_inverse"] == 1 else 0       # best 2 (VALU)
         c += {1: 0.3, 2: 0.0, 4: 0.1}[s["parts"]]           # best 2
+        c += 0.0 if s["fused_forward"] == 8 else 0.05      # best 8
         return c
 
     def family_us(self):
@@ -62,7 +64,9 @@ class FakeEngine:
                 self.launch[k] = v
 
     def set_schedule(self, newton_per_graph=0, tail_inverse=0, sweep_form=0,
-                     parts=0):
+                     parts=0, fused_forward=None):
+        if fused_forward is not None:
+            self.sched["fused_forward"] = fused_forward
         if newton_per_graph:
             self.sched["newton_per_graph"] = newton_per_graph
         if tail_inverse:
@@ -121,7 +125,8 @@ def test_tune_picks_cheapest_and_caches(tmp_path, clock):
     assert got["tiles"] == {"ordering": 1, "tail_limit": 48, "chunk_products": 3072,
                             "slot_chunk": 2048}
     assert got["launch"] == {"lu": 512, "solve": 1024, "mismatch": 256}
-    assert got["schedule"] == {"newton_per_graph": 8, "tail_inverse": 2, "parts": 2}
+    assert got["schedule"] == {"newton_per_graph": 8, "tail_inverse": 2, "parts": 2,
+                               "fused_forward": 8}
     assert all(e.closed for e in made)
     assert got["ms_per_solve"] == pytest.approx(10.0, abs=1e-9)
     cache = json.load(open(path))
@@ -147,6 +152,8 @@ def test_refused_tiles_are_skipped(tmp_path, clock):
 def test_apply_sets_launch_and_schedule(clock):
     e = FakeEngine(clock, {}, [])
     tune.apply(e, {"launch": {"lu": 256, "solve": 512, "mismatch": 1024},
-                   "schedule": {"newton_per_graph": 2, "tail_inverse": 2, "parts": 4}})
+                   "schedule": {"newton_per_graph": 2, "tail_inverse": 2, "parts": 4,
+                                "fused_forward": -1}})
     assert e.launch == {"lu": 256, "solve": 512, "mismatch": 1024}
-    assert e.sched == {"newton_per_graph": 2, "tail_inverse": 2, "parts": 4}
+    assert e.sched == {"newton_per_graph": 2, "tail_inverse": 2, "parts": 4,
+                       "fused_forward": -1}
