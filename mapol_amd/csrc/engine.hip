@@ -1032,7 +1032,8 @@ static void launch_newton_step(bpf_engine *e) {
   e->fused_forward = fused ? fp.fused_limit : -1;
   tbegin(e, "jacobian");
   hipLaunchKernelGGL(k_jacobian<T>, xgrid(e->T, ((size_t)e->g.nJb + 1) / 2, 256),
-                     dim3(256), 0, e->stream, e->g, e->d, LU, c0 > 0 ? 1 : 0, fused);
+                     dim3(256), 0, e->stream, e->g, e->d, LU, c0 > 0 ? 1 : 0,
+                     fp.fused_limit);
   tend(e);
   tbegin(e, "lu_factor");
   if (e->split_mode != 1 && e->fplan_split) {
@@ -1829,7 +1830,7 @@ extern "C" bpf_status bpf_stage_jacobian(bpf_engine *e, const double *V,
   HIPCHK(hipMemcpy(e->d.st, st2.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(LU, 0, sizeof(B4<double>) * (size_t)e->T * e->g.nnzLU));
   hipLaunchKernelGGL(k_jacobian<double>, xgrid(e->T, ((size_t)e->g.nJb + 1) / 2, 256),
-                     dim3(256), 0, e->stream, e->g, e->d, LU, 0, 0);
+                     dim3(256), 0, e->stream, e->g, e->d, LU, 0, -1);
   HIPCHK(hipStreamSynchronize(e->stream));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(e->d.st, st.data(), sizeof(InstState) * e->T, hipMemcpyHostToDevice));
@@ -1946,7 +1947,7 @@ extern "C" bpf_status bpf_adjoint(bpf_engine *e, const double *rate, double w,
   const unsigned TT = (unsigned)T;
   tbegin(e, "adjoint");
   hipLaunchKernelGGL(k_jacobian<double>, xgrid(TT, ((size_t)e->g.nJb + 1) / 2, 256),
-                     dim3(256), 0, e->stream, e->g, e->d, LU, 0, 0);
+                     dim3(256), 0, e->stream, e->g, e->d, LU, 0, -1);
   hipLaunchKernelGGL(k_lu_factor<double>, dim3(TT), dim3(e->lu_threads), 0,
                      e->stream, e->g, e->d.st, 0, e->fplan, LU, TLd, 0, e->d.jsing, 0, nullptr, 0, -1);
   hipLaunchKernelGGL(k_adj_rhs, dim3(div_up(e->S.nb, 256), TT), dim3(256), 0,

@@ -968,8 +968,9 @@ __device__ __forceinline__ B4<T> jac_block(const DevTopo &g, const DevInst &d,
   return v;
 }
 
-// rhs: also write the right-hand-side blocks {F_P, 0; F_Q, 0} after the
-// factor's, where the factorization's forward substitution starts
+// rhs >= 0: while at most rhs instances of the batch part are active (the
+// factorization's test), also write the right-hand-side blocks
+// {F_P, 0; F_Q, 0} after the factor's, where its forward substitution starts
 template <typename T>
 __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0, int rhs) {
   // one thread per Jacobian block, in block-position order: the writes into
@@ -982,6 +983,7 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0, int rhs) 
   const XSlot sl = xcd_slot((g.nJb + JU * bd - 1) / (JU * bd));
   const int t = sl.t;
   if (t >= d.T || d.st[t].done) return;
+  const bool wr = rhs >= 0 && *d.active_count <= rhs;
 #pragma unroll
   for (int u = 0; u < JU; u++) {
     const int r = sl.part * JU * bd + u * bd + (int)threadIdx.x;
@@ -999,7 +1001,7 @@ __global__ void k_jacobian(DevTopo g, DevInst d, B4<T> *LU, int elim0, int rhs) 
       v = blk_mul(v, inv);
     }
     LU[(size_t)t * g.nnzLU + jb.x] = v;
-    if (rhs && r < g.nb) {
+    if (wr && r < g.nb) {
       const double2 f = ((const double2 *)(d.F + (size_t)t * 2 * g.nb))[r];
       B4<T> b;
       b.a = (T)f.x; b.b = (T)0; b.c = (T)f.y; b.d = (T)0;
