@@ -40,7 +40,7 @@ SCHEDULE_CANDIDATES = {
     "parts": (1, 2, 4),            # concurrent parts of the batch
     # the forward substitution in the factorization while at most this
     # many instances of a part are active (-1 never, 1 << 20 always)
-    "fused_forward": (32, -1, 8, 1 << 20),
+    "fused_forward": (32, -1, 8, 64, 1 << 20),
 }
 CANDIDATES = (256, 512, 1024)      # workgroup sizes
 FAMILIES = {"lu": "lu_factor", "solve": "lu_solve", "mismatch": "mismatch"}
